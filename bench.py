@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the state-vector core on one MI355X.
+
+Workload (the plain per-gate GPU path): a 30-qubit complex128 register, the
+brickwork circuit — each layer is U3 (angles ~ U[0, 2pi)) on every qubit then
+CZ on alternating neighbour pairs — applied one `registry_apply` at a time
+through the drop-in `doki` module (out-of-place, norm fused, as the
+reference's apply_gate).  One step = one layer (30 U3 + 15 or 14 CZ).
+
+value = whole-job HBM GB/s per gate = gates * 32 * 2^n bytes / timed seconds
+(every unfused gate reads and writes each amplitude once: 16 B + 16 B).
+
+Also reported: the gate kernel's own duration from HIP events on the
+library's stream (roofline), per-target kernel GB/s (low and high stride),
+registry_prob on every qubit, and the reference's CPU path timed on the host
+cores over a bounded sample (cpu_baseline).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("HBM GB/s per gate at 30q complex128; served circuits/s + p99 "
+          "latency vs plain")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_layer(d, reg, layer):
+    for g, t, c in layer:
+        reg = d.registry_apply(reg, g, t, c, None, -1, False)
+    return reg
+
+
+def make_layers(d, W, n, count, seed=2025):
+    rng = W.rng_for(seed)
+    layers = []
+    for li in range(count):
+        steps = W.brickwork_layer(rng, n, li)
+        layers.append([(d.gate_new(1, m.tolist(), False), t, set(c))
+                       for m, t, c, a in steps])
+    return layers
+
+
+def gpu_leg(args):
+    import doki_amd
+    from doki_amd import capi, workloads as W
+    d = doki_amd.doki
+    n = args.qubits
+    bpg = 32 * (1 << n)
+    layers = make_layers(d, W, n, args.warmup + args.steps + 1)
+    log(f"[bench] n={n} state={16 * (1 << n) / 1e9:.2f} GB, "
+        f"steps={args.steps} warmup={args.warmup}")
+    reg = d.registry_new(n, False)
+    for li in range(args.warmup):
+        reg = run_layer(d, reg, layers[li])
+    capi.synchronize()
+    log("[bench] warmup done")
+    t0 = time.perf_counter()
+    ngates = 0
+    for li in range(args.warmup, args.warmup + args.steps):
+        reg = run_layer(d, reg, layers[li])
+        ngates += len(layers[li])
+    capi.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    log(f"[bench] timed {ngates} gates in {elapsed:.3f} s")
+
+    # gate-kernel durations from HIP events on the library stream
+    capi.tune("time_apply", 1)
+    kms = []
+    for g, t, c in layers[-1]:
+        reg = d.registry_apply(reg, g, t, c, None, -1, False)
+        kms.append(capi.event_elapsed_ms(62, 63))
+    per_target = {}
+    u3 = layers[-1][0][0]
+    for t in (0, 3, 6, n // 2, n - 1):
+        ms = []
+        for _ in range(5):
+            reg = d.registry_apply(reg, u3, [t], set(), None, -1, False)
+            ms.append(capi.event_elapsed_ms(62, 63))
+        per_target[str(t)] = {"kernel_ms": min(ms),
+                              "GBps": bpg / (min(ms) * 1e-3) / 1e9}
+    capi.tune("time_apply", 0)
+    kavg = sum(kms) / len(kms)
+    log(f"[bench] gate kernel avg {kavg:.3f} ms -> {bpg / kavg / 1e6:.0f} GB/s")
+
+    # registry_prob on every qubit (end of the circuit)
+    tp0 = time.perf_counter()
+    probs = [d.registry_prob(reg, q, -1, False) for q in range(n)]
+    tp1 = time.perf_counter()
+    assert all(0.0 <= p <= 1.0 + 1e-12 for p in probs)
+
+    return {
+        "elapsed": elapsed, "ngates": ngates, "bpg": bpg, "kavg_ms": kavg,
+        "kernel_ms_layer": kms, "per_target": per_target,
+        "prob_all_ms": (tp1 - tp0) * 1e3,
+    }
+
+
+def cpu_leg(args, gb_per_gate_unit=True):
+    """The reference's own CPU path (oracle/_ref, compiled from its sources),
+    OpenMP on the host cores, over a bounded sample of the same workload:
+    brickwork-layer U3 gates at a smaller register, until ~cpu_seconds."""
+    from oracle import oracle as O
+    from doki_amd import workloads as W
+    ref = O.load_reference()
+    kind = "reference"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = args.cpu_qubits
+    rng = W.rng_for(7)
+    steps = W.brickwork_layer(rng, n, 0) + W.brickwork_layer(rng, n, 1)
+    if ref is None:
+        # fall back to the single-threaded oracle port
+        kind, threads = "port", 1
+        st = O.new(n)
+        t0 = time.perf_counter()
+        g = 0
+        while time.perf_counter() - t0 < args.cpu_seconds and g < len(steps):
+            m, t, c, a = steps[g]
+            st = O.apply(st, m, t, sorted(c), sorted(a))
+            g += 1
+        el = time.perf_counter() - t0
+    else:
+        gates = [(ref.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
+        reg = ref.registry_new(n, False)
+        # one untimed gate (first touch of the allocator)
+        reg = ref.registry_apply(reg, gates[0][0], gates[0][1], gates[0][2], None,
+                                 threads, False)
+        t0 = time.perf_counter()
+        g = 0
+        while time.perf_counter() - t0 < args.cpu_seconds and g < len(gates):
+            gg, t, c = gates[g]
+            reg = ref.registry_apply(reg, gg, t, c, None, threads, False)
+            g += 1
+        el = time.perf_counter() - t0
+    val = g * 32 * (1 << n) / el / 1e9
+    return {"value": round(val, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+            "sample": f"{g} brickwork gates (U3/CZ) at n={n} complex128, "
+                      f"{el:.1f} s, registry_apply of the reference's OpenMP path"
+                      if kind == "reference" else
+                      f"{g} gates at n={n} through the single-threaded oracle port"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--cpu-qubits", type=int, default=24)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the state-vector core has no multi-GPU path in this build",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1:
+        sys.exit("--steps must be >= 1")
+
+    g = gpu_leg(args)
+    cpu = None if args.no_cpu else cpu_leg(args)
+    if cpu:
+        log(f"[bench] cpu baseline {cpu}")
+    value = g["ngates"] * g["bpg"] / g["elapsed"] / 1e9
+    achieved = g["bpg"] / (g["kavg_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(g["elapsed"] / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"brickwork circuit, {args.qubits}-qubit complex128 register, "
+                        "plain per-gate registry_apply (out-of-place, fused norm); "
+                        "one step = one layer of U3 on every qubit + CZ pairs",
+            "qubits": args.qubits,
+            "gates_timed": g["ngates"],
+            "bytes_per_gate": g["bpg"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "extras": {
+            "gate_kernel_avg_ms": round(g["kavg_ms"], 4),
+            "per_target_kernel": g["per_target"],
+            "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,258 @@
+"""ctypes binding of the C ABI in include/doki_hip.h (libdoki_hip.so).
+
+This is the binding a non-Python host (or a test) uses to drive the device
+core directly; the `doki` extension calls the same entry points from C.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import LIB_PATH
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HEADER = os.path.join(os.path.dirname(HERE), "include", "doki_hip.h")
+
+_lib = None
+
+
+class DokiHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def header_functions():
+    """Names of the functions declared in include/doki_hip.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dk_\w+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libdoki_hip.so is not built: run `python -m doki_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    PP = C.POINTER(C.c_void_p)
+    dp = C.POINTER(C.c_double)
+    u64 = C.c_uint64
+    sig = {
+        "dk_init": (C.c_int, [C.c_int]),
+        "dk_last_error": (C.c_char_p, []),
+        "dk_synchronize": (C.c_int, []),
+        "dk_memory_budget": (C.c_int, [C.POINTER(u64), C.POINTER(u64)]),
+        "dk_set_memory_budget": (C.c_int, [u64]),
+        "dk_state_new": (C.c_int, [C.c_uint, C.c_int, PP]),
+        "dk_state_from_host": (C.c_int, [C.c_uint, dp, PP]),
+        "dk_state_clone": (C.c_int, [P, PP]),
+        "dk_state_free": (None, [P]),
+        "dk_state_num_qubits": (C.c_uint, [P]),
+        "dk_state_mem_size": (u64, [P]),
+        "dk_state_device_bytes": (u64, [P]),
+        "dk_state_norm": (C.c_int, [P, dp]),
+        "dk_gate_new": (C.c_int, [C.c_uint, dp, PP]),
+        "dk_gate_free": (None, [P]),
+        "dk_apply": (C.c_int, [P, P, C.POINTER(C.c_uint), u64, u64, C.c_int, PP]),
+        "dk_prob": (C.c_int, [P, C.c_uint, dp]),
+        "dk_prob_all": (C.c_int, [P, dp]),
+        "dk_measure": (C.c_int, [P, u64, dp, C.c_size_t,
+                                 C.POINTER(C.c_int8), dp, PP]),
+        "dk_join": (C.c_int, [P, P, PP]),
+        "dk_first_nonzero": (C.c_int, [P, C.POINTER(u64)]),
+        "dk_read_raw": (C.c_int, [P, u64, u64, dp, dp]),
+        "dk_read": (C.c_int, [P, u64, u64, dp]),
+        "dk_reduced_density": (C.c_int, [P, u64, dp]),
+        "dk_reduced_density_elements": (C.c_int, [P, u64, C.c_size_t,
+                                                  C.POINTER(u64), C.POINTER(u64), dp]),
+        "dk_event_record": (C.c_int, [C.c_int]),
+        "dk_event_elapsed": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "dk_tune": (C.c_int, [C.c_char_p, C.c_int64]),
+        "dk_device_info": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise DokiHipError(rc, lib().dk_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class State:
+    """A device state handle (freed on GC)."""
+
+    def __init__(self, ptr):
+        self.ptr = C.c_void_p(ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _lib is not None:
+            _lib.dk_state_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def nq(self):
+        return lib().dk_state_num_qubits(self.ptr)
+
+    def raw(self, first=0, count=None):
+        n = (1 << self.nq) if count is None else count
+        out = np.empty(2 * n)
+        nrm = C.c_double()
+        check(lib().dk_read_raw(self.ptr, first, n, _dp(out), C.byref(nrm)))
+        return out.view(np.complex128), nrm.value
+
+    def values(self):
+        raw, nrm = self.raw()
+        return raw / nrm
+
+    @property
+    def norm(self):
+        v = C.c_double()
+        check(lib().dk_state_norm(self.ptr, C.byref(v)))
+        return v.value
+
+
+class Gate:
+    def __init__(self, matrix):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+        k = int(np.log2(m.shape[0]))
+        p = C.c_void_p()
+        check(lib().dk_gate_new(k, _dp(m.view(np.float64).ravel()), C.byref(p)))
+        self.ptr = p
+        self.k = k
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _lib is not None:
+            _lib.dk_gate_free(self.ptr)
+            self.ptr = None
+
+
+def new(nq):
+    p = C.c_void_p()
+    check(lib().dk_state_new(nq, 1, C.byref(p)))
+    return State(p.value)
+
+
+def from_host(values):
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.complex128).ravel())
+    nq = int(np.log2(v.size))
+    p = C.c_void_p()
+    check(lib().dk_state_from_host(nq, _dp(v.view(np.float64)), C.byref(p)))
+    return State(p.value)
+
+
+def clone(s):
+    p = C.c_void_p()
+    check(lib().dk_state_clone(s.ptr, C.byref(p)))
+    return State(p.value)
+
+
+def mask(qs):
+    m = 0
+    for q in qs:
+        m |= 1 << int(q)
+    return m
+
+
+def apply(s, gate, targets, controls=(), anticontrols=(), in_place=False):
+    t = (C.c_uint * len(targets))(*targets)
+    p = C.c_void_p()
+    check(lib().dk_apply(s.ptr, gate.ptr, t, mask(controls), mask(anticontrols),
+                         int(in_place), C.byref(p)))
+    if in_place:
+        return s
+    return State(p.value)
+
+
+def prob(s, q):
+    v = C.c_double()
+    check(lib().dk_prob(s.ptr, q, C.byref(v)))
+    return v.value
+
+
+def prob_all(s):
+    out = np.empty(s.nq)
+    check(lib().dk_prob_all(s.ptr, _dp(out)))
+    return out
+
+
+def measure(s, mask_, rolls):
+    n = s.nq
+    r = np.ascontiguousarray(np.asarray(rolls, dtype=np.float64))
+    res = (C.c_int8 * n)()
+    p1 = np.zeros(n)
+    p = C.c_void_p()
+    check(lib().dk_measure(s.ptr, mask_, _dp(r) if r.size else None, r.size,
+                           res, _dp(p1), C.byref(p)))
+    st = State(p.value) if p.value else None
+    return st, [None if res[i] < 0 else bool(res[i]) for i in range(n)], p1
+
+
+def join(a, b):
+    p = C.c_void_p()
+    check(lib().dk_join(a.ptr, b.ptr, C.byref(p)))
+    return State(p.value)
+
+
+def first_nonzero(s):
+    v = C.c_uint64()
+    check(lib().dk_first_nonzero(s.ptr, C.byref(v)))
+    return v.value
+
+
+def reduced_density(s, keep):
+    a = len(keep)
+    out = np.empty(2 << (2 * a))
+    check(lib().dk_reduced_density(s.ptr, mask(keep), _dp(out)))
+    return out.view(np.complex128).reshape(1 << a, 1 << a)
+
+
+def reduced_density_elements(s, keep, rows, cols):
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    c = np.ascontiguousarray(cols, dtype=np.uint64)
+    out = np.empty(2 * r.size)
+    u = C.POINTER(C.c_uint64)
+    check(lib().dk_reduced_density_elements(
+        s.ptr, mask(keep), r.size, r.ctypes.data_as(u), c.ctypes.data_as(u),
+        _dp(out)))
+    return out.view(np.complex128)
+
+
+def synchronize():
+    check(lib().dk_synchronize())
+
+
+def event_record(slot):
+    check(lib().dk_event_record(slot))
+
+
+def event_elapsed_ms(a, b):
+    v = C.c_float()
+    check(lib().dk_event_elapsed(a, b, C.byref(v)))
+    return v.value
+
+
+def tune(key, value):
+    check(lib().dk_tune(key.encode(), int(value)))
+
+
+def device_info():
+    n = C.c_int()
+    buf = C.create_string_buffer(64)
+    check(lib().dk_device_info(C.byref(n), buf, 64))
+    return n.value, buf.value.decode()

@@ -1,0 +1,118 @@
+// Internal declarations shared by the HIP sources of libdoki_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+
+#include "doki_hip.h"
+
+typedef uint64_t u64;
+
+// ---- error reporting -------------------------------------------------------
+
+void dk_set_error(const char *fmt, ...);
+
+#define DK_HIP(call)                                                        \
+	do {                                                                \
+		hipError_t e_ = (call);                                     \
+		if (e_ != hipSuccess) {                                     \
+			dk_set_error("%s failed: %s", #call,                \
+				     hipGetErrorString(e_));                \
+			return DK_E_DEVICE;                                 \
+		}                                                           \
+	} while (0)
+
+#define DK_LAUNCHED()                                                       \
+	do {                                                                \
+		hipError_t e_ = hipGetLastError();                          \
+		if (e_ != hipSuccess) {                                     \
+			dk_set_error("kernel launch failed: %s",            \
+				     hipGetErrorString(e_));                \
+			return DK_E_DEVICE;                                 \
+		}                                                           \
+	} while (0)
+
+// ---- device-resident objects -------------------------------------------------
+
+// Per-state device scalars, stored in front of the amplitudes.
+//   [0] norm constant (reads return raw / norm)
+enum { DK_SCAL_NORM = 0, DK_SCAL_BYTES = 256 };
+
+struct dk_state {
+	unsigned nq;
+	u64 len;        // 2^nq amplitudes
+	char *base;     // device allocation: scalars then amplitudes
+	double *scal;   // device scalars
+	double2 *amp;   // device amplitudes (raw)
+	u64 bytes;      // bytes of the allocation
+};
+
+struct dk_gate {
+	unsigned k;
+	unsigned dim;       // 2^k
+	double2 *dmat;      // device copy, row-major
+	double2 hmat[64 * 64];
+};
+
+// ---- library context ---------------------------------------------------------
+
+enum { DK_PARTIALS = 1 << 20 }; // doubles of reduction scratch
+
+struct dk_ctx {
+	int device = -1;
+	int num_cus = 0;
+	hipStream_t stream = nullptr;
+	double *partials = nullptr;   // reduction scratch (DK_PARTIALS doubles)
+	double *dec = nullptr;        // measurement decisions scratch (device)
+	double *host_pinned = nullptr;// pinned staging (DK_PINNED doubles)
+	u64 budget = 0, used = 0;
+	hipEvent_t events[64] = {};
+	std::mutex mu;
+	// tunables
+	int apply_blocks = 0;   // 0 = auto
+	int time_apply = 0;     // record events[62]/[63] around the gate kernel
+};
+
+enum { DK_PINNED = 1 << 16 };
+
+extern dk_ctx g_ctx;
+
+int dk_ensure_init();
+int dk_alloc_state(unsigned nq, dk_state **out);
+void dk_release_state(dk_state *s);
+
+// ---- kernel launchers (dk_kernels.hip) ----------------------------------------
+
+// Gate application; writes block partial sums of |out|^2 to g_ctx.partials and
+// finishes the norm of `out` on the device.
+int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
+		    const unsigned *targets, u64 cm, u64 acm);
+
+// sum_{i: bit q of i = 1} |raw_i / norm|^2 -> *dst (device scalar).
+int dk_launch_prob(const dk_state *s, unsigned q, double *dst);
+int dk_launch_prob_all(const dk_state *s, double *dst /* device, nq */);
+
+// Scaled copy: dst = src * (1 / src.norm), dst.norm = 1.
+int dk_launch_normalised_copy(const dk_state *src, dk_state *dst);
+
+// Measurement step: decide outcome from a device P1, write the norm of the
+// collapsed state, then collapse (with fused partials for the next qubit).
+int dk_launch_decide(const double *p1_src, int from_partials, int nparts,
+		     double roll, double *dec_slot, dk_state *next);
+int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
+		       dk_state *o, int next_q, int *nparts_out);
+
+int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o);
+
+int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev);
+
+int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev);
+int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
+			       const u64 *rows_dev, const u64 *cols_dev,
+			       double2 *out_dev);
+
+int dk_launch_fill_zero_state(dk_state *s);
+int dk_launch_set_norm(dk_state *s, double v);

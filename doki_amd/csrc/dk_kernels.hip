@@ -1,0 +1,1007 @@
+// gfx950 kernels of the state-vector core.
+//
+// Layout: a state of n qubits is 2^n complex128 amplitudes (double2, 16 B,
+// re then im) in one contiguous HBM allocation; index bit q is qubit q.
+// Every kernel reads a state through its normalisation constant
+// (value = raw * (1/norm)), as the reference's state_get does
+// (src/doki/qstate.h:86).  Reductions are FP64 and deterministic for a given
+// state size: fixed grids, per-block partials summed in block order by a
+// single-block finishing kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "dk_internal.h"
+
+namespace {
+
+constexpr int TPB = 256;          // threads per block (4 wave64)
+constexpr int WAVES = TPB / 64;
+
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+
+// acc + m * v (complex, FMA form)
+__device__ __forceinline__ double2 cmac(double2 acc, double2 m, double2 v)
+{
+	acc.x = fma(m.x, v.x, acc.x);
+	acc.x = fma(-m.y, v.y, acc.x);
+	acc.y = fma(m.x, v.y, acc.y);
+	acc.y = fma(m.y, v.x, acc.y);
+	return acc;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+// a * conj(b)
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b)
+{
+	return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+}
+
+__device__ __forceinline__ double2 cscale(double2 v, double s)
+{
+	return make_double2(v.x * s, v.y * s);
+}
+
+__device__ __forceinline__ double cabs2(double2 v)
+{
+	return fma(v.x, v.x, v.y * v.y);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o);
+	return v;
+}
+
+// Sum over the block; the result is valid in thread 0.  `red` holds WAVES
+// doubles of LDS.  Ends with the block synchronised.
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+	v = wave_sum(v);
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	double t = 0.0;
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int w = 0; w < WAVES; ++w)
+			t += red[w];
+	}
+	__syncthreads();
+	return t;
+}
+
+// Insert a zero bit at position p.
+__device__ __forceinline__ u64 ins0(u64 x, unsigned p)
+{
+	const u64 lo = (1ull << p) - 1;
+	return ((x & ~lo) << 1) | (x & lo);
+}
+
+// ---------------------------------------------------------------------------
+// One-qubit gates.  The 2x2 matrix travels by value in the kernel arguments.
+// ---------------------------------------------------------------------------
+struct Mat2 {
+	double2 m00, m01, m10, m11;
+};
+
+// Pair form: thread handles amplitude pairs (i0, i0 | 2^t).  For t >= 6 the
+// 64 lanes of a wave read two fully contiguous 1 KiB runs.
+template <int U>
+__global__ __launch_bounds__(TPB) void k_apply1_pair(
+	const double2 *in, double2 *out, const double *__restrict__ nrm,
+	Mat2 M, unsigned t, u64 cm, u64 acm, u64 npairs,
+	double *__restrict__ partials)
+{
+	__shared__ double red[WAVES];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 bit = 1ull << t, lo = bit - 1;
+	const u64 stride = (u64)gridDim.x * TPB * U;
+	double acc = 0.0;
+	for (u64 p0 = (u64)blockIdx.x * TPB * U + threadIdx.x; p0 < npairs;
+	     p0 += stride) {
+		double2 a[U], b[U];
+		u64 i0[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const u64 p = p0 + (u64)u * TPB;
+			i0[u] = ((p & ~lo) << 1) | (p & lo);
+			if (p < npairs) {
+				a[u] = in[i0[u]];
+				b[u] = in[i0[u] | bit];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			if (p0 + (u64)u * TPB >= npairs)
+				continue;
+			const double2 x = cscale(a[u], inv), y = cscale(b[u], inv);
+			double2 o0 = x, o1 = y;
+			if ((i0[u] & cm) == cm && (i0[u] & acm) == 0) {
+				o0 = cmac(cmac(cz(), M.m00, x), M.m01, y);
+				o1 = cmac(cmac(cz(), M.m10, x), M.m11, y);
+			}
+			out[i0[u]] = o0;
+			out[i0[u] | bit] = o1;
+			acc += cabs2(o0) + cabs2(o1);
+		}
+	}
+	const double tot = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = tot;
+}
+
+// Lane-exchange form for t < 6: lanes read consecutive amplitudes and fetch
+// the partner amplitude from lane ^ 2^t.  Requires len to be a multiple of
+// the grid stride (every lane of a wave runs the same trip count).
+template <int U>
+__global__ __launch_bounds__(TPB) void k_apply1_lane(
+	const double2 *in, double2 *out, const double *__restrict__ nrm,
+	Mat2 M, unsigned t, u64 cm, u64 acm, u64 len,
+	double *__restrict__ partials)
+{
+	__shared__ double red[WAVES];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const int sh = 1 << t;
+	const u64 stride = (u64)gridDim.x * TPB * U;
+	double acc = 0.0;
+	for (u64 i0 = (u64)blockIdx.x * TPB * U + threadIdx.x; i0 < len;
+	     i0 += stride) {
+		double2 a[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			a[u] = in[i0 + (u64)u * TPB];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const u64 i = i0 + (u64)u * TPB;
+			const double2 x = cscale(a[u], inv);
+			double2 y;
+			y.x = __shfl_xor(x.x, sh);
+			y.y = __shfl_xor(x.y, sh);
+			const bool hi = (i >> t) & 1;
+			const double2 v0 = hi ? y : x, v1 = hi ? x : y;
+			double2 o = x;
+			if ((i & cm) == cm && (i & acm) == 0) {
+				const double2 r0 = hi ? M.m10 : M.m00;
+				const double2 r1 = hi ? M.m11 : M.m01;
+				o = cmac(cmac(cz(), r0, v0), r1, v1);
+			}
+			out[i] = o;
+			acc += cabs2(o);
+		}
+	}
+	const double tot = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// k-qubit gates (k = 1..6) through an LDS tile.  A tile is the 2^P amplitudes
+// (P = min(12, n)) whose index bits vary over the positions pos[0..P) — the
+// gate's targets plus the lowest non-target bits — with the other bits fixed
+// by the tile number.  Loads and stores walk the tile in slot order, so
+// consecutive lanes touch consecutive addresses whenever the low positions
+// are contiguous; the gate then works on groups inside LDS.
+// ---------------------------------------------------------------------------
+constexpr int TILE_BITS = 12;
+constexpr int TILE = 1 << TILE_BITS;
+
+struct TileArgs {
+	u64 dep_lo[64];            // slot bits 0..5 -> index bits
+	u64 dep_hi[64];            // slot bits 6..11 -> index bits
+	unsigned short colslot[64];// gate column j -> slot offset of its targets
+	unsigned char pos[TILE_BITS]; // tile positions, ascending
+	unsigned char tslot[8];    // slot positions of the targets, ascending
+	int npos;
+};
+
+template <int K>
+__global__ __launch_bounds__(TPB) void k_apply_tile(
+	const double2 *in, double2 *out, const double *__restrict__ nrm,
+	const double2 *__restrict__ M, TileArgs ta, u64 cm, u64 acm,
+	u64 ntiles, double *__restrict__ partials)
+{
+	constexpr int D = 1 << K;
+	constexpr int R = D < 16 ? D : 16;   // rows per work item
+	constexpr int NRB = D / R;           // row blocks per group
+	constexpr int OPT = TILE / TPB;      // outputs per thread (16)
+	constexpr int IPT = OPT / R;         // work items per thread
+	__shared__ double2 lds[TILE];
+	__shared__ double red[WAVES];
+
+	const int tsz = 1 << ta.npos;
+	const int G = tsz >> K;
+	const int nwork = G * NRB;
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	double acc = 0.0;
+
+	for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		u64 base = tile;
+		for (int b = 0; b < ta.npos; ++b)
+			base = ins0(base, ta.pos[b]);
+		for (int s = threadIdx.x; s < tsz; s += TPB) {
+			const u64 gi = base | ta.dep_lo[s & 63] | ta.dep_hi[s >> 6];
+			lds[s] = cscale(in[gi], inv);
+		}
+		__syncthreads();
+
+		double2 res[IPT][R];
+		int gsl[IPT], rbs[IPT];
+#pragma unroll
+		for (int it = 0; it < IPT; ++it) {
+			const int w = threadIdx.x + it * TPB;
+			gsl[it] = -1;
+			if (w >= nwork)
+				continue;
+			const int g = w % G;
+			const int rb = NRB == 1 ? 0 : w / G;
+			rbs[it] = rb;
+			int gs = g;
+#pragma unroll
+			for (int b = 0; b < K; ++b)
+				gs = (int)ins0((u64)gs, ta.tslot[b]);
+			gsl[it] = gs;
+			const u64 gi = base | ta.dep_lo[gs & 63] | ta.dep_hi[gs >> 6];
+			if ((gi & cm) == cm && (gi & acm) == 0) {
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					res[it][r] = cz();
+				for (int j = 0; j < D; ++j) {
+					const double2 x = lds[gs | ta.colslot[j]];
+					const double2 *mrow = M + (size_t)(rb * R) * D + j;
+#pragma unroll
+					for (int r = 0; r < R; ++r)
+						res[it][r] = cmac(res[it][r], mrow[(size_t)r * D], x);
+				}
+			} else {
+#pragma unroll
+				for (int r = 0; r < R; ++r)
+					res[it][r] = lds[gs | ta.colslot[rb * R + r]];
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int it = 0; it < IPT; ++it) {
+			if (gsl[it] < 0)
+				continue;
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				lds[gsl[it] | ta.colslot[rbs[it] * R + r]] = res[it][r];
+		}
+		__syncthreads();
+		for (int s = threadIdx.x; s < tsz; s += TPB) {
+			const u64 gi = base | ta.dep_lo[s & 63] | ta.dep_hi[s >> 6];
+			const double2 o = lds[s];
+			out[gi] = o;
+			acc += cabs2(o);
+		}
+		__syncthreads();
+	}
+	const double tot = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Single-block finishing kernels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_finish_sum(
+	const double *__restrict__ partials, int count, double *dst, int take_sqrt)
+{
+	__shared__ double red[WAVES];
+	double s = 0.0;
+	for (int i = threadIdx.x; i < count; i += TPB)
+		s += partials[i];
+	s = block_sum(s, red);
+	if (threadIdx.x == 0)
+		dst[0] = take_sqrt ? sqrt(s) : s;
+}
+
+// Sum `cols` interleaved columns: dst[c] = sum_b partials[b * cols + c].
+__global__ __launch_bounds__(64) void k_finish_cols(
+	const double *__restrict__ partials, int rows, int cols, double *dst)
+{
+	const int c = threadIdx.x;
+	if (c >= cols)
+		return;
+	double s = 0.0;
+	for (int b = 0; b < rows; ++b)
+		s += partials[(size_t)b * cols + c];
+	dst[c] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Probabilities.
+// ---------------------------------------------------------------------------
+template <int U>
+__global__ __launch_bounds__(TPB) void k_prob(
+	const double2 *__restrict__ in, const double *__restrict__ nrm,
+	unsigned q, u64 half, double *__restrict__ partials)
+{
+	__shared__ double red[WAVES];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 bit = 1ull << q, lo = bit - 1;
+	const u64 stride = (u64)gridDim.x * TPB * U;
+	double acc = 0.0;
+	for (u64 j0 = (u64)blockIdx.x * TPB * U + threadIdx.x; j0 < half;
+	     j0 += stride) {
+		double2 a[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const u64 j = j0 + (u64)u * TPB;
+			if (j < half)
+				a[u] = in[((j & ~lo) << 1) | bit | (j & lo)];
+			else
+				a[u] = cz();
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			acc += cabs2(cscale(a[u], inv));
+	}
+	const double tot = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = tot;
+}
+
+// All single-qubit probabilities in one pass.  A wave reads chunks of 64*U
+// consecutive amplitudes: index bits 0..5 are the lane, bits 6..6+LU-1 the
+// unroll slot, higher bits the chunk number.  Lane and unroll bits are
+// resolved at the end / per chunk in registers; chunk bits add the chunk sum.
+constexpr int PQ = 40; // qubits covered by the one-pass kernel
+
+template <int LU>
+__global__ __launch_bounds__(TPB) void k_prob_all(
+	const double2 *__restrict__ in, const double *__restrict__ nrm,
+	unsigned nq, u64 len, double *__restrict__ partials)
+{
+	constexpr int U = 1 << LU;
+	constexpr int NH = PQ - 6 - LU;
+	__shared__ double red[WAVES][PQ];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const int lane = threadIdx.x & 63;
+	const u64 wid = (u64)blockIdx.x * WAVES + (threadIdx.x >> 6);
+	const u64 nw = (u64)gridDim.x * WAVES;
+	const u64 nch = len >> (6 + LU);
+	double tot = 0.0, accu[LU], acch[NH];
+#pragma unroll
+	for (int b = 0; b < LU; ++b)
+		accu[b] = 0.0;
+#pragma unroll
+	for (int b = 0; b < NH; ++b)
+		acch[b] = 0.0;
+	for (u64 c = wid; c < nch; c += nw) {
+		double x[U];
+		const double2 *p = in + (c << (6 + LU)) + lane;
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			x[u] = cabs2(cscale(p[(u64)u * 64], inv));
+		double s = 0.0;
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			s += x[u];
+#pragma unroll
+		for (int b = 0; b < LU; ++b) {
+			double t = 0.0;
+#pragma unroll
+			for (int u = 0; u < U; ++u)
+				if ((u >> b) & 1)
+					t += x[u];
+			accu[b] += t;
+		}
+		tot += s;
+#pragma unroll
+		for (int b = 0; b < NH; ++b)
+			acch[b] += ((c >> b) & 1) ? s : 0.0;
+	}
+	// per-qubit sums of this lane, reduced over the wave then the block
+	const int w = threadIdx.x >> 6;
+#pragma unroll
+	for (int q = 0; q < PQ; ++q) {
+		double v;
+		if (q < 6)
+			v = ((lane >> q) & 1) ? tot : 0.0;
+		else if (q < 6 + LU)
+			v = accu[q - 6];
+		else
+			v = acch[q - 6 - LU];
+		v = wave_sum(v);
+		if (lane == 0)
+			red[w][q] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x < PQ) {
+		double s = 0.0;
+#pragma unroll
+		for (int ww = 0; ww < WAVES; ++ww)
+			s += red[ww][threadIdx.x];
+		partials[(size_t)blockIdx.x * PQ + threadIdx.x] = s;
+	}
+	(void)nq;
+}
+
+// ---------------------------------------------------------------------------
+// Measurement: decision and collapse.
+// ---------------------------------------------------------------------------
+// dec[0] = outcome (0/1), dec[1] = P1.  When next_norm is given it receives
+// sqrt(P) with P = P1 (outcome 1) or 1 - P1 (outcome 0).
+__global__ __launch_bounds__(TPB) void k_decide(
+	const double *__restrict__ partials, int count, double roll, double *dec,
+	double *next_norm)
+{
+	__shared__ double red[WAVES];
+	double s = 0.0;
+	for (int i = threadIdx.x; i < count; i += TPB)
+		s += partials[i];
+	s = block_sum(s, red);
+	if (threadIdx.x == 0) {
+		const double p1 = s;
+		const bool v = p1 > roll;
+		const double p = v ? p1 : 1.0 - p1;
+		dec[0] = v ? 1.0 : 0.0;
+		dec[1] = p1;
+		if (next_norm)
+			next_norm[DK_SCAL_NORM] = sqrt(p);
+	}
+}
+
+// o[j] = s[j with bit q inserted = outcome] / s.norm; when next_q >= 0 also
+// the block partials of sum_{j: bit next_q set} |o_j / o.norm|^2.
+template <int U>
+__global__ __launch_bounds__(TPB) void k_collapse(
+	const double2 *__restrict__ in, const double *__restrict__ nrm_in,
+	unsigned q, const double *__restrict__ dec, double2 *__restrict__ out,
+	const double *__restrict__ nrm_out, int next_q, u64 olen,
+	double *__restrict__ partials)
+{
+	__shared__ double red[WAVES];
+	const double inv = 1.0 / nrm_in[DK_SCAL_NORM];
+	const double inv_o = 1.0 / nrm_out[DK_SCAL_NORM];
+	const u64 bit = 1ull << q, lo = bit - 1;
+	const u64 off = dec[0] != 0.0 ? bit : 0;
+	const u64 stride = (u64)gridDim.x * TPB * U;
+	double acc = 0.0;
+	for (u64 j0 = (u64)blockIdx.x * TPB * U + threadIdx.x; j0 < olen;
+	     j0 += stride) {
+		double2 a[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const u64 j = j0 + (u64)u * TPB;
+			if (j < olen)
+				a[u] = in[((j & ~lo) << 1) | off | (j & lo)];
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const u64 j = j0 + (u64)u * TPB;
+			if (j >= olen)
+				continue;
+			const double2 o = cscale(a[u], inv);
+			out[j] = o;
+			if (next_q >= 0 && ((j >> next_q) & 1))
+				acc += cabs2(cscale(o, inv_o));
+		}
+	}
+	const double tot = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Copies, join, fills.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_normalised_copy(
+	const double2 *__restrict__ in, const double *__restrict__ nrm,
+	double2 *__restrict__ out, double *out_scal, u64 len)
+{
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 stride = (u64)gridDim.x * TPB;
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len; i += stride)
+		out[i] = cscale(in[i], inv);
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		out_scal[DK_SCAL_NORM] = 1.0;
+}
+
+__global__ __launch_bounds__(TPB) void k_join(
+	const double2 *__restrict__ a, const double *__restrict__ na,
+	const double2 *__restrict__ b, const double *__restrict__ nb,
+	unsigned bbits, u64 olen, double2 *__restrict__ o, double *o_scal)
+{
+	const double ia = 1.0 / na[DK_SCAL_NORM], ib = 1.0 / nb[DK_SCAL_NORM];
+	const u64 bmask = (1ull << bbits) - 1;
+	const u64 stride = (u64)gridDim.x * TPB;
+	for (u64 idx = (u64)blockIdx.x * TPB + threadIdx.x; idx < olen;
+	     idx += stride) {
+		const double2 x = cscale(a[idx >> bbits], ia);
+		const double2 y = cscale(b[idx & bmask], ib);
+		o[idx] = cmul(x, y);
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		o_scal[DK_SCAL_NORM] = 1.0;
+}
+
+__global__ __launch_bounds__(TPB) void k_fill_zero_state(
+	double2 *__restrict__ amp, double *scal, u64 len)
+{
+	const u64 stride = (u64)gridDim.x * TPB;
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len; i += stride)
+		amp[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		scal[DK_SCAL_NORM] = 1.0;
+}
+
+__global__ void k_set_scalar(double *dst, double v)
+{
+	if (threadIdx.x == 0)
+		dst[0] = v;
+}
+
+__global__ void k_set_u64(unsigned long long *dst, u64 v)
+{
+	if (threadIdx.x == 0)
+		dst[0] = v;
+}
+
+// Smallest index >= start holding a nonzero raw amplitude, via atomicMin.
+__global__ __launch_bounds__(TPB) void k_first_nonzero(
+	const double2 *__restrict__ in, u64 start, u64 len,
+	unsigned long long *dst)
+{
+	__shared__ unsigned long long red[WAVES];
+	u64 best = len;
+	const u64 stride = (u64)gridDim.x * TPB;
+	for (u64 i = start + (u64)blockIdx.x * TPB + threadIdx.x; i < len;
+	     i += stride) {
+		const double2 v = in[i];
+		if (v.x != 0.0 || v.y != 0.0) {
+			best = i;
+			break;
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		const u64 other = __shfl_xor((unsigned long long)best, o);
+		best = other < best ? other : best;
+	}
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = best;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		u64 m = red[0];
+		for (int w = 1; w < WAVES; ++w)
+			m = red[w] < m ? red[w] : m;
+		if (m < len)
+			atomicMin(dst, (unsigned long long)m);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Reduced density matrix: rho[i][j] = sum_t A[i][t] conj(A[j][t]) with
+// A[i][t] = amp[keep(i) | traced(t)] / norm.  16x16 output tiles, t staged
+// through LDS 16 at a time; each block covers one t-chunk and writes its
+// partial tile; k_density_sum adds the chunks in order.
+// ---------------------------------------------------------------------------
+struct DensArgs {
+	u64 keep_lo[64];    // kept index bits 0..5 -> state index bits
+	u64 keep_hi[64];    // kept index bits 6..11 -> state index bits
+	unsigned char kpos[TILE_BITS]; // kept positions, ascending
+	int a;              // kept qubits
+	int m;              // traced qubits
+};
+
+__device__ __forceinline__ u64 traced_index(u64 t, const DensArgs &da)
+{
+	for (int b = 0; b < da.a; ++b)
+		t = ins0(t, da.kpos[b]);
+	return t;
+}
+
+__device__ __forceinline__ u64 keep_index(u64 i, const DensArgs &da)
+{
+	return da.keep_lo[i & 63] | da.keep_hi[(i >> 6) & 63];
+}
+
+__global__ __launch_bounds__(TPB) void k_density_tile(
+	const double2 *__restrict__ in, const double *__restrict__ nrm,
+	DensArgs da, u64 tchunk, int tiles_per_row, double2 *__restrict__ part)
+{
+	__shared__ double2 ai[16][17];
+	__shared__ double2 aj[16][17];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 dim = 1ull << da.a;
+	const int ti = blockIdx.x / tiles_per_row, tj = blockIdx.x % tiles_per_row;
+	const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+	const u64 gi = (u64)ti * 16 + r, gj = (u64)tj * 16 + c;
+	const u64 t0 = (u64)blockIdx.y * tchunk;
+	const u64 T = 1ull << da.m;
+	const u64 t1 = t0 + tchunk < T ? t0 + tchunk : T;
+	double2 acc = cz();
+	for (u64 tb = t0; tb < t1; tb += 16) {
+		// stage A[ti*16 + r][tb + c] and A[tj*16 + r][tb + c]
+		const u64 t = tb + c;
+		const u64 ri = (u64)ti * 16 + r, rj = (u64)tj * 16 + r;
+		double2 vi = cz(), vj = cz();
+		if (t < t1) {
+			const u64 tix = traced_index(t, da);
+			if (ri < dim)
+				vi = cscale(in[keep_index(ri, da) | tix], inv);
+			if (rj < dim)
+				vj = cscale(in[keep_index(rj, da) | tix], inv);
+		}
+		ai[r][c] = vi;
+		aj[r][c] = vj;
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < 16; ++k) {
+			const double2 x = ai[r][k], y = aj[c][k];
+			acc.x = fma(x.x, y.x, acc.x);
+			acc.x = fma(x.y, y.y, acc.x);
+			acc.y = fma(x.y, y.x, acc.y);
+			acc.y = fma(-x.x, y.y, acc.y);
+		}
+		__syncthreads();
+	}
+	if (gi < dim && gj < dim)
+		part[(size_t)blockIdx.y * dim * dim + gi * dim + gj] = acc;
+}
+
+__global__ __launch_bounds__(TPB) void k_density_sum(
+	const double2 *__restrict__ part, int nchunks, u64 count,
+	double2 *__restrict__ rho)
+{
+	const u64 stride = (u64)gridDim.x * TPB;
+	for (u64 e = (u64)blockIdx.x * TPB + threadIdx.x; e < count; e += stride) {
+		double2 s = cz();
+		for (int k = 0; k < nchunks; ++k) {
+			const double2 v = part[(size_t)k * count + e];
+			s.x += v.x;
+			s.y += v.y;
+		}
+		rho[e] = s;
+	}
+}
+
+// One block per requested element.
+__global__ __launch_bounds__(TPB) void k_density_elements(
+	const double2 *__restrict__ in, const double *__restrict__ nrm,
+	DensArgs da, const u64 *__restrict__ rows, const u64 *__restrict__ cols,
+	double2 *__restrict__ out)
+{
+	__shared__ double red[WAVES];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 ki = keep_index(rows[blockIdx.x], da);
+	const u64 kj = keep_index(cols[blockIdx.x], da);
+	const u64 T = 1ull << da.m;
+	double2 acc = cz();
+	for (u64 t = threadIdx.x; t < T; t += TPB) {
+		const u64 tix = traced_index(t, da);
+		acc = cmac(acc, cscale(in[ki | tix], inv),
+			   make_double2(in[kj | tix].x * inv, -in[kj | tix].y * inv));
+	}
+	const double re = block_sum(acc.x, red);
+	const double im = block_sum(acc.y, red);
+	if (threadIdx.x == 0)
+		out[blockIdx.x] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers
+// ---------------------------------------------------------------------------
+unsigned grid_for(u64 units_per_block_total, u64 per_block)
+{
+	u64 b = (units_per_block_total + per_block - 1) / per_block;
+	u64 cap = g_ctx.apply_blocks > 0 ? (u64)g_ctx.apply_blocks
+					 : (u64)std::max(1, g_ctx.num_cus) * 8;
+	// keep the grid a power of two so grid strides divide 2^n sizes
+	u64 p = 1;
+	while (p * 2 <= cap)
+		p *= 2;
+	b = std::min(b, p);
+	return (unsigned)std::max<u64>(1, b);
+}
+
+} // namespace
+
+// ===========================================================================
+int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
+		    const unsigned *targets, u64 cm, u64 acm)
+{
+	hipStream_t st = g_ctx.stream;
+	double *parts = g_ctx.partials;
+	unsigned nblocks = 0;
+	const unsigned n = in->nq;
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(g_ctx.events[62], st));
+	if (g->k == 1) {
+		Mat2 M{g->hmat[0], g->hmat[1], g->hmat[2], g->hmat[3]};
+		const unsigned t = targets[0];
+		constexpr int U = 4;
+		if (t < 6 && in->len >= (u64)TPB * U) {
+			nblocks = grid_for(in->len, (u64)TPB * U);
+			hipLaunchKernelGGL(k_apply1_lane<U>, dim3(nblocks), dim3(TPB), 0,
+					   st, in->amp, out->amp, in->scal, M, t, cm,
+					   acm, in->len, parts);
+		} else {
+			const u64 np = in->len >> 1;
+			nblocks = grid_for(np, (u64)TPB * U);
+			hipLaunchKernelGGL(k_apply1_pair<U>, dim3(nblocks), dim3(TPB), 0,
+					   st, in->amp, out->amp, in->scal, M, t, cm,
+					   acm, np, parts);
+		}
+	} else {
+		const unsigned k = g->k;
+		TileArgs ta;
+		memset(&ta, 0, sizeof(ta));
+		const int npos = (int)std::min<unsigned>(TILE_BITS, n);
+		// positions: targets plus the lowest non-target bits
+		bool is_t[64] = {false};
+		for (unsigned b = 0; b < k; ++b)
+			is_t[targets[b]] = true;
+		int cnt = 0, extra = npos - (int)k;
+		for (unsigned p = 0; p < n && cnt < npos; ++p) {
+			if (is_t[p]) {
+				ta.pos[cnt++] = (unsigned char)p;
+			} else if (extra > 0) {
+				ta.pos[cnt++] = (unsigned char)p;
+				--extra;
+			}
+		}
+		ta.npos = npos;
+		int slot_of[64];
+		for (int b = 0; b < npos; ++b)
+			slot_of[ta.pos[b]] = b;
+		for (int s = 0; s < 64; ++s) {
+			u64 lo = 0, hi = 0;
+			for (int b = 0; b < 6; ++b) {
+				if (!((s >> b) & 1))
+					continue;
+				if (b < npos)
+					lo |= 1ull << ta.pos[b];
+				if (6 + b < npos)
+					hi |= 1ull << ta.pos[6 + b];
+			}
+			ta.dep_lo[s] = lo;
+			ta.dep_hi[s] = hi;
+		}
+		for (unsigned j = 0; j < (1u << k); ++j) {
+			unsigned off = 0;
+			for (unsigned b = 0; b < k; ++b)
+				if ((j >> b) & 1)
+					off |= 1u << slot_of[targets[b]];
+			ta.colslot[j] = (unsigned short)off;
+		}
+		int ts[8];
+		for (unsigned b = 0; b < k; ++b)
+			ts[b] = slot_of[targets[b]];
+		std::sort(ts, ts + k);
+		for (unsigned b = 0; b < k; ++b)
+			ta.tslot[b] = (unsigned char)ts[b];
+		const u64 ntiles = in->len >> npos;
+		nblocks = grid_for(ntiles, 1);
+#define DK_TILE_CASE(KK)                                                    \
+	case KK:                                                            \
+		hipLaunchKernelGGL(k_apply_tile<KK>, dim3(nblocks), dim3(TPB), 0, \
+				   st, in->amp, out->amp, in->scal, g->dmat, ta, \
+				   cm, acm, ntiles, parts);                 \
+		break;
+		switch (k) {
+			DK_TILE_CASE(1)
+			DK_TILE_CASE(2)
+			DK_TILE_CASE(3)
+			DK_TILE_CASE(4)
+			DK_TILE_CASE(5)
+			DK_TILE_CASE(6)
+		default:
+			dk_set_error("gate of %u qubits is not supported", k);
+			return DK_E_ARG;
+		}
+#undef DK_TILE_CASE
+	}
+	DK_LAUNCHED();
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(g_ctx.events[63], st));
+	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, st, parts,
+			   (int)nblocks, out->scal + DK_SCAL_NORM, 1);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
+{
+	constexpr int U = 4;
+	const u64 half = s->len >> 1;
+	const unsigned nb = grid_for(half, (u64)TPB * U);
+	hipLaunchKernelGGL(k_prob<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->amp, s->scal, q, half, g_ctx.partials);
+	DK_LAUNCHED();
+	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, g_ctx.stream,
+			   g_ctx.partials, (int)nb, dst, 0);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_prob_all(const dk_state *s, double *dst)
+{
+	constexpr int LU = 3;
+	if (s->nq < 6 + LU || s->nq > PQ) {
+		for (unsigned q = 0; q < s->nq; ++q) {
+			int rc = dk_launch_prob(s, q, dst + q);
+			if (rc)
+				return rc;
+		}
+		return DK_OK;
+	}
+	const u64 nch = s->len >> (6 + LU);
+	unsigned nb = grid_for(nch, WAVES);
+	hipLaunchKernelGGL(k_prob_all<LU>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->amp, s->scal, s->nq, s->len, g_ctx.partials);
+	DK_LAUNCHED();
+	hipLaunchKernelGGL(k_finish_cols, dim3(1), dim3(64), 0, g_ctx.stream,
+			   g_ctx.partials, (int)nb, PQ, dst);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
+{
+	const unsigned nb = grid_for(src->len, TPB * 4);
+	hipLaunchKernelGGL(k_normalised_copy, dim3(nb), dim3(TPB), 0,
+			   g_ctx.stream, src->amp, src->scal, dst->amp, dst->scal,
+			   src->len);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+// The outcome's P1 is either a finished device scalar (from_partials = 0,
+// p1_src points at it, nparts = 1) or a set of block partials to be summed.
+int dk_launch_decide(const double *p1_src, int from_partials, int nparts,
+		     double roll, double *dec_slot, dk_state *next)
+{
+	(void)from_partials;
+	hipLaunchKernelGGL(k_decide, dim3(1), dim3(TPB), 0, g_ctx.stream, p1_src,
+			   nparts, roll, dec_slot, next ? next->scal : nullptr);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
+		       dk_state *o, int next_q, int *nparts_out)
+{
+	constexpr int U = 4;
+	const u64 olen = s->len >> 1;
+	const unsigned nb = grid_for(olen, (u64)TPB * U);
+	hipLaunchKernelGGL(k_collapse<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->amp, s->scal, q, dec_slot, o->amp, o->scal, next_q,
+			   olen, g_ctx.partials);
+	DK_LAUNCHED();
+	*nparts_out = (int)nb;
+	return DK_OK;
+}
+
+int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out)
+{
+	constexpr int U = 4;
+	const u64 half = s->len >> 1;
+	const unsigned nb = grid_for(half, (u64)TPB * U);
+	hipLaunchKernelGGL(k_prob<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->amp, s->scal, q, half, g_ctx.partials);
+	DK_LAUNCHED();
+	*nparts_out = (int)nb;
+	return DK_OK;
+}
+
+int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
+{
+	const unsigned nb = grid_for(o->len, TPB * 4);
+	hipLaunchKernelGGL(k_join, dim3(nb), dim3(TPB), 0, g_ctx.stream, a->amp,
+			   a->scal, b->amp, b->scal, b->nq, o->len, o->amp,
+			   o->scal);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev)
+{
+	hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, g_ctx.stream,
+			   (unsigned long long *)dst_dev, s->len);
+	DK_LAUNCHED();
+	if (start >= s->len)
+		return DK_OK;
+	const unsigned nb = grid_for(s->len - start, TPB * 4);
+	hipLaunchKernelGGL(k_first_nonzero, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->amp, start, s->len, (unsigned long long *)dst_dev);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+static void dens_args(const dk_state *s, u64 keep_mask, DensArgs &da)
+{
+	memset(&da, 0, sizeof(da));
+	int a = 0;
+	for (unsigned p = 0; p < s->nq; ++p)
+		if ((keep_mask >> p) & 1)
+			da.kpos[a++] = (unsigned char)p;
+	da.a = a;
+	da.m = (int)s->nq - a;
+	for (int v = 0; v < 64; ++v) {
+		u64 lo = 0, hi = 0;
+		for (int b = 0; b < 6; ++b) {
+			if (!((v >> b) & 1))
+				continue;
+			if (b < a)
+				lo |= 1ull << da.kpos[b];
+			if (6 + b < a)
+				hi |= 1ull << da.kpos[6 + b];
+		}
+		da.keep_lo[v] = lo;
+		da.keep_hi[v] = hi;
+	}
+}
+
+int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
+{
+	DensArgs da;
+	dens_args(s, keep_mask, da);
+	const u64 dim = 1ull << da.a;
+	const u64 T = 1ull << da.m;
+	const int tpr = (int)((dim + 15) / 16);
+	const u64 ntiles = (u64)tpr * tpr;
+	// t-chunks: enough blocks to fill the device, partial buffer <= 256 MiB
+	u64 want = std::max<u64>(1, (u64)std::max(1, g_ctx.num_cus) * 8 / ntiles);
+	u64 maxc = std::max<u64>(1, (256ull << 20) / (dim * dim * 16));
+	u64 nch = std::min<u64>(std::min<u64>(want, maxc), std::max<u64>(1, T / 16));
+	u64 p = 1;
+	while (p * 2 <= nch)
+		p *= 2;
+	nch = std::min<u64>(p, 65535);
+	const u64 tchunk = T / nch;
+	double2 *part = nullptr;
+	DK_HIP(hipMallocAsync((void **)&part, nch * dim * dim * sizeof(double2),
+			      g_ctx.stream));
+	hipLaunchKernelGGL(k_density_tile, dim3((unsigned)ntiles, (unsigned)nch),
+			   dim3(TPB), 0, g_ctx.stream, s->amp, s->scal, da, tchunk,
+			   tpr, part);
+	DK_LAUNCHED();
+	const unsigned nb = grid_for(dim * dim, TPB);
+	hipLaunchKernelGGL(k_density_sum, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   part, (int)nch, dim * dim, rho_dev);
+	DK_LAUNCHED();
+	DK_HIP(hipFreeAsync(part, g_ctx.stream));
+	return DK_OK;
+}
+
+int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
+			       const u64 *rows_dev, const u64 *cols_dev,
+			       double2 *out_dev)
+{
+	DensArgs da;
+	dens_args(s, keep_mask, da);
+	if (count == 0)
+		return DK_OK;
+	hipLaunchKernelGGL(k_density_elements, dim3((unsigned)count), dim3(TPB),
+			   0, g_ctx.stream, s->amp, s->scal, da, rows_dev, cols_dev,
+			   out_dev);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_fill_zero_state(dk_state *s)
+{
+	const unsigned nb = grid_for(s->len, TPB * 4);
+	hipLaunchKernelGGL(k_fill_zero_state, dim3(nb), dim3(TPB), 0,
+			   g_ctx.stream, s->amp, s->scal, s->len);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_set_norm(dk_state *s, double v)
+{
+	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, g_ctx.stream,
+			   s->scal + DK_SCAL_NORM, v);
+	DK_LAUNCHED();
+	return DK_OK;
+}

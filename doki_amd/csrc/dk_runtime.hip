@@ -1,0 +1,776 @@
+// C ABI of libdoki_hip.so: device context, state lifetime and the entry
+// points declared in include/doki_hip.h.  Kernels live in dk_kernels.hip.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dk_internal.h"
+
+dk_ctx g_ctx;
+static thread_local char t_err[512];
+static bool g_closed = false;
+
+void dk_set_error(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(t_err, sizeof(t_err), fmt, ap);
+	va_end(ap);
+}
+
+extern "C" const char *dk_last_error(void)
+{
+	return t_err;
+}
+
+// Device slots inside g_ctx.dec
+enum {
+	DEC_SLOTS = 64,            // measurement decisions: 2 doubles each
+	DEC_SCALAR = 2 * DEC_SLOTS,// one device scalar result
+	DEC_U64 = DEC_SCALAR + 1,  // one u64 result
+	DEC_PROBS = DEC_SCALAR + 2,// up to 64 probabilities
+	DEC_TOTAL = DEC_PROBS + 64
+};
+
+extern "C" int dk_init(int device)
+{
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	if (g_ctx.stream)
+		return DK_OK;
+	int count = 0;
+	hipError_t e = hipGetDeviceCount(&count);
+	if (e != hipSuccess || count <= 0) {
+		dk_set_error("no HIP device available: %s",
+			     e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+		return DK_E_DEVICE;
+	}
+	if (device < 0 || device >= count) {
+		dk_set_error("device %d out of range (%d devices)", device, count);
+		return DK_E_ARG;
+	}
+	DK_HIP(hipSetDevice(device));
+	hipDeviceProp_t prop;
+	DK_HIP(hipGetDeviceProperties(&prop, device));
+	if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+		dk_set_error("libdoki_hip is built for gfx950 only; device %d is %s",
+			     device, prop.gcnArchName);
+		return DK_E_DEVICE;
+	}
+	g_ctx.device = device;
+	g_ctx.num_cus = prop.multiProcessorCount;
+	hipStream_t st;
+	DK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+	DK_HIP(hipMalloc((void **)&g_ctx.partials, DK_PARTIALS * sizeof(double)));
+	DK_HIP(hipMalloc((void **)&g_ctx.dec, DEC_TOTAL * sizeof(double)));
+	DK_HIP(hipHostMalloc((void **)&g_ctx.host_pinned,
+			     DK_PINNED * sizeof(double), hipHostMallocDefault));
+	for (auto &ev : g_ctx.events)
+		DK_HIP(hipEventCreate(&ev));
+	hipMemPool_t pool;
+	DK_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+	uint64_t thr = UINT64_MAX;
+	DK_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+	size_t fr = 0, tot = 0;
+	DK_HIP(hipMemGetInfo(&fr, &tot));
+	g_ctx.budget = (u64)((double)fr * 0.92);
+	g_ctx.used = 0;
+	g_ctx.stream = st;
+	return DK_OK;
+}
+
+int dk_ensure_init()
+{
+	if (g_ctx.stream)
+		return DK_OK;
+	const char *dev = getenv("DOKI_HIP_DEVICE");
+	return dk_init(dev ? atoi(dev) : 0);
+}
+
+#define DK_INIT()                                                           \
+	do {                                                                \
+		int rc_ = dk_ensure_init();                                 \
+		if (rc_)                                                    \
+			return rc_;                                         \
+	} while (0)
+
+extern "C" int dk_synchronize(void)
+{
+	DK_INIT();
+	DK_HIP(hipStreamSynchronize(g_ctx.stream));
+	return DK_OK;
+}
+
+// Marks the library closed: later frees only drop host bookkeeping.  The
+// extension calls it when the interpreter finalises, so that any state freed
+// after that point never reaches a HIP runtime that may be tearing down.
+extern "C" void dk_shutdown(void)
+{
+	if (g_ctx.stream)
+		(void)hipStreamSynchronize(g_ctx.stream);
+	g_closed = true;
+}
+
+extern "C" int dk_memory_budget(uint64_t *budget, uint64_t *used)
+{
+	DK_INIT();
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	if (budget)
+		*budget = g_ctx.budget;
+	if (used)
+		*used = g_ctx.used;
+	return DK_OK;
+}
+
+extern "C" int dk_set_memory_budget(uint64_t budget)
+{
+	DK_INIT();
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	g_ctx.budget = budget;
+	return DK_OK;
+}
+
+// ---- states -----------------------------------------------------------------
+
+int dk_alloc_state(unsigned nq, dk_state **out)
+{
+	*out = nullptr;
+	if (nq < 1 || nq > DK_MAX_QUBITS) {
+		dk_set_error("Number of qubits exceeds maximum");
+		return DK_E_QUBITS;
+	}
+	DK_INIT();
+	if (nq > 44) {
+		dk_set_error("Failed to allocate state chunk: 2^%u amplitudes "
+			     "exceed device memory", nq);
+		return DK_E_BUDGET;
+	}
+	const u64 len = 1ull << nq;
+	const u64 bytes = DK_SCAL_BYTES + len * sizeof(double2);
+	{
+		std::lock_guard<std::mutex> lk(g_ctx.mu);
+		if (g_ctx.used + bytes > g_ctx.budget) {
+			dk_set_error("Failed to allocate state chunk: %llu bytes "
+				     "requested, %llu of %llu budget bytes in use",
+				     (unsigned long long)bytes,
+				     (unsigned long long)g_ctx.used,
+				     (unsigned long long)g_ctx.budget);
+			return DK_E_BUDGET;
+		}
+		g_ctx.used += bytes;
+	}
+	char *base = nullptr;
+	hipError_t e = hipMallocAsync((void **)&base, bytes, g_ctx.stream);
+	if (e != hipSuccess) {
+		(void)hipGetLastError();
+		// give cached pool memory back and retry once
+		hipMemPool_t pool;
+		if (hipStreamSynchronize(g_ctx.stream) == hipSuccess &&
+		    hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
+			(void)hipMemPoolTrimTo(pool, 0);
+		e = hipMallocAsync((void **)&base, bytes, g_ctx.stream);
+	}
+	if (e != hipSuccess) {
+		(void)hipGetLastError();
+		std::lock_guard<std::mutex> lk(g_ctx.mu);
+		g_ctx.used -= bytes;
+		dk_set_error("Failed to allocate state chunk: %s",
+			     hipGetErrorString(e));
+		return DK_E_ALLOC;
+	}
+	dk_state *s = new dk_state;
+	s->nq = nq;
+	s->len = len;
+	s->base = base;
+	s->scal = (double *)base;
+	s->amp = (double2 *)(base + DK_SCAL_BYTES);
+	s->bytes = bytes;
+	*out = s;
+	return DK_OK;
+}
+
+void dk_release_state(dk_state *s)
+{
+	if (!s)
+		return;
+	if (!g_closed && g_ctx.stream) {
+		(void)hipFreeAsync(s->base, g_ctx.stream);
+		std::lock_guard<std::mutex> lk(g_ctx.mu);
+		g_ctx.used -= s->bytes;
+	}
+	delete s;
+}
+
+extern "C" void dk_state_free(dk_state *s)
+{
+	dk_release_state(s);
+}
+
+extern "C" int dk_state_new(unsigned num_qubits, int zero_state, dk_state **out)
+{
+	dk_state *s;
+	int rc = dk_alloc_state(num_qubits, &s);
+	if (rc)
+		return rc;
+	rc = zero_state ? dk_launch_fill_zero_state(s) : dk_launch_set_norm(s, 1.0);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" int dk_state_from_host(unsigned num_qubits, const double *values,
+				  dk_state **out)
+{
+	if (!values) {
+		dk_set_error("values is NULL");
+		return DK_E_ARG;
+	}
+	dk_state *s;
+	int rc = dk_alloc_state(num_qubits, &s);
+	if (rc)
+		return rc;
+	hipError_t e = hipMemcpyAsync(s->amp, values, s->len * sizeof(double2),
+				      hipMemcpyHostToDevice, g_ctx.stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(g_ctx.stream);
+	if (e != hipSuccess) {
+		dk_set_error("copy to device failed: %s", hipGetErrorString(e));
+		dk_release_state(s);
+		return DK_E_DEVICE;
+	}
+	rc = dk_launch_set_norm(s, 1.0);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" int dk_state_clone(const dk_state *src, dk_state **out)
+{
+	if (!src) {
+		dk_set_error("NULL state");
+		return DK_E_ARG;
+	}
+	dk_state *s;
+	int rc = dk_alloc_state(src->nq, &s);
+	if (rc)
+		return rc;
+	rc = dk_launch_normalised_copy(src, s);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" unsigned dk_state_num_qubits(const dk_state *s)
+{
+	return s ? s->nq : 0;
+}
+
+extern "C" uint64_t dk_state_mem_size(const dk_state *s)
+{
+	return s ? 64ull + 16ull * s->len : 0;
+}
+
+extern "C" uint64_t dk_state_device_bytes(const dk_state *s)
+{
+	return s ? s->bytes : 0;
+}
+
+// Copies `count` doubles from device to host through the pinned buffer.
+static int copy_to_host(double *dst, const void *src, size_t count)
+{
+	const char *p = (const char *)src;
+	while (count > 0) {
+		const size_t n = std::min<size_t>(count, DK_PINNED);
+		DK_HIP(hipMemcpyAsync(g_ctx.host_pinned, p, n * sizeof(double),
+				      hipMemcpyDeviceToHost, g_ctx.stream));
+		DK_HIP(hipStreamSynchronize(g_ctx.stream));
+		memcpy(dst, g_ctx.host_pinned, n * sizeof(double));
+		dst += n;
+		p += n * sizeof(double);
+		count -= n;
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_state_norm(const dk_state *s, double *norm)
+{
+	DK_INIT();
+	if (!s || !norm) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	return copy_to_host(norm, s->scal + DK_SCAL_NORM, 1);
+}
+
+// ---- gates -------------------------------------------------------------------
+
+extern "C" int dk_gate_new(unsigned num_qubits, const double *matrix,
+			   dk_gate **out)
+{
+	*out = nullptr;
+	if (num_qubits < 1 || num_qubits > DK_MAX_GATE_QUBITS || !matrix) {
+		dk_set_error("gates act on 1..%d qubits", DK_MAX_GATE_QUBITS);
+		return DK_E_ARG;
+	}
+	DK_INIT();
+	dk_gate *g = new dk_gate;
+	g->k = num_qubits;
+	g->dim = 1u << num_qubits;
+	g->dmat = nullptr;
+	const size_t n = (size_t)g->dim * g->dim;
+	memcpy(g->hmat, matrix, n * sizeof(double2));
+	if (num_qubits > 1) {
+		hipError_t e = hipMalloc((void **)&g->dmat, n * sizeof(double2));
+		if (e == hipSuccess)
+			e = hipMemcpy(g->dmat, g->hmat, n * sizeof(double2),
+				      hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			dk_set_error("gate upload failed: %s", hipGetErrorString(e));
+			if (g->dmat)
+				(void)hipFree(g->dmat);
+			delete g;
+			return DK_E_DEVICE;
+		}
+	}
+	*out = g;
+	return DK_OK;
+}
+
+extern "C" void dk_gate_free(dk_gate *g)
+{
+	if (!g)
+		return;
+	if (g->dmat && !g_closed) {
+		// the gate may still be read by queued kernels
+		(void)hipStreamSynchronize(g_ctx.stream);
+		(void)hipFree(g->dmat);
+	}
+	delete g;
+}
+
+// ---- operations ----------------------------------------------------------------
+
+extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
+			const unsigned *targets, uint64_t ctrl_mask,
+			uint64_t actrl_mask, int in_place, dk_state **out)
+{
+	DK_INIT();
+	if (!in || !gate || !targets || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	const unsigned n = in->nq;
+	const u64 valid = n >= 64 ? ~0ull : ((1ull << n) - 1);
+	u64 tmask = 0;
+	for (unsigned b = 0; b < gate->k; ++b) {
+		if (targets[b] >= n) {
+			dk_set_error("Target qubit out of range");
+			return DK_E_ARG;
+		}
+		if ((tmask >> targets[b]) & 1) {
+			dk_set_error("Repeated target qubit");
+			return DK_E_ARG;
+		}
+		tmask |= 1ull << targets[b];
+	}
+	if ((ctrl_mask & ~valid) || (actrl_mask & ~valid)) {
+		dk_set_error("Control qubit out of range");
+		return DK_E_ARG;
+	}
+	if (ctrl_mask & actrl_mask) {
+		dk_set_error("A control cannot also be an anticontrol");
+		return DK_E_ARG;
+	}
+	if (tmask & (ctrl_mask | actrl_mask)) {
+		dk_set_error("A target cannot also be a control or an anticontrol");
+		return DK_E_ARG;
+	}
+	dk_state *o = in;
+	if (!in_place) {
+		int rc = dk_alloc_state(n, &o);
+		if (rc)
+			return rc;
+	}
+	int rc = dk_launch_apply(in, o, gate, targets, ctrl_mask, actrl_mask);
+	if (rc) {
+		if (o != in)
+			dk_release_state(o);
+		return rc;
+	}
+	*out = o;
+	return DK_OK;
+}
+
+extern "C" int dk_prob(const dk_state *s, unsigned q, double *p)
+{
+	DK_INIT();
+	if (!s || !p) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	if (q >= s->nq) {
+		dk_set_error("Qubit id out of range");
+		return DK_E_ARG;
+	}
+	double *dst = g_ctx.dec + DEC_SCALAR;
+	int rc = dk_launch_prob(s, q, dst);
+	if (rc)
+		return rc;
+	return copy_to_host(p, dst, 1);
+}
+
+extern "C" int dk_prob_all(const dk_state *s, double *p)
+{
+	DK_INIT();
+	if (!s || !p) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	double *dst = g_ctx.dec + DEC_PROBS;
+	int rc = dk_launch_prob_all(s, dst);
+	if (rc)
+		return rc;
+	return copy_to_host(p, dst, s->nq);
+}
+
+int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out);
+
+extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
+			  size_t num_rolls, int8_t *results, double *p1_out,
+			  dk_state **out)
+{
+	DK_INIT();
+	if (!s || !results || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	*out = nullptr;
+	const unsigned n = s->nq;
+	// qubits to measure, most significant first (bits >= n are ignored)
+	std::vector<unsigned> qs;
+	for (unsigned i = 0; i < n; ++i) {
+		const unsigned q = n - 1 - i;
+		results[i] = -1;
+		if (p1_out)
+			p1_out[i] = 0.0;
+		if ((mask >> q) & 1)
+			qs.push_back(q);
+	}
+	for (size_t k = 0; k < qs.size(); ++k) {
+		if (k >= num_rolls || !rolls || !(rolls[k] >= 0.0 && rolls[k] < 1.0)) {
+			dk_set_error("roll not in interval [0, 1)!");
+			return DK_E_ROLL;
+		}
+	}
+	if (qs.empty())
+		return dk_state_clone(s, out);
+	if (qs.size() > DEC_SLOTS) {
+		dk_set_error("too many measured qubits");
+		return DK_E_ARG;
+	}
+	const dk_state *cur = s;
+	dk_state *owned = nullptr; // cur when it is a temporary
+	dk_state *next = nullptr;
+	int rc = DK_OK;
+	int nparts = 0;
+	// P1 of the first measured qubit
+	rc = dk_launch_prob_partials(cur, qs[0], &nparts);
+	if (!rc && cur->nq > 1)
+		rc = dk_alloc_state(cur->nq - 1, &next);
+	if (!rc)
+		rc = dk_launch_decide(g_ctx.partials, 1, nparts, rolls[0],
+				      g_ctx.dec + 0, next);
+	for (size_t k = 0; !rc && k < qs.size(); ++k) {
+		if (cur->nq == 1)
+			break; // last qubit: nothing left to collapse into
+		const int next_q = k + 1 < qs.size() ? (int)qs[k + 1] : -1;
+		rc = dk_launch_collapse(cur, qs[k], g_ctx.dec + 2 * k, next,
+					next_q, &nparts);
+		if (rc)
+			break;
+		if (owned)
+			dk_release_state(owned);
+		owned = next;
+		cur = next;
+		next = nullptr;
+		if (next_q >= 0) {
+			if (cur->nq > 1)
+				rc = dk_alloc_state(cur->nq - 1, &next);
+			if (!rc)
+				rc = dk_launch_decide(g_ctx.partials, 1, nparts,
+						      rolls[k + 1],
+						      g_ctx.dec + 2 * (k + 1), next);
+		}
+	}
+	if (rc) {
+		if (next)
+			dk_release_state(next);
+		if (owned)
+			dk_release_state(owned);
+		return rc;
+	}
+	std::vector<double> dec(2 * qs.size());
+	rc = copy_to_host(dec.data(), g_ctx.dec, dec.size());
+	if (rc) {
+		if (owned)
+			dk_release_state(owned);
+		return rc;
+	}
+	for (size_t k = 0; k < qs.size(); ++k) {
+		const unsigned slot = n - 1 - qs[k];
+		results[slot] = dec[2 * k] != 0.0 ? 1 : 0;
+		if (p1_out)
+			p1_out[slot] = dec[2 * k + 1];
+		const double p = dec[2 * k] != 0.0 ? dec[2 * k + 1]
+						   : 1.0 - dec[2 * k + 1];
+		const bool leaves_qubits = (n - 1 - k) > 0;
+		if (leaves_qubits && !(sqrt(p) != 0.0)) {
+			if (owned)
+				dk_release_state(owned);
+			dk_set_error("New normalization constant is 0. Please report "
+				     "this error with the steps to reproduce it.");
+			return DK_E_ZERO_NORM;
+		}
+	}
+	if (n - qs.size() > 0) {
+		*out = owned;
+	} else {
+		if (owned)
+			dk_release_state(owned);
+		*out = nullptr;
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_join(const dk_state *most, const dk_state *least,
+		       dk_state **out)
+{
+	DK_INIT();
+	if (!most || !least || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	dk_state *o;
+	int rc = dk_alloc_state(most->nq + least->nq, &o);
+	if (rc)
+		return rc;
+	rc = dk_launch_join(most, least, o);
+	if (rc) {
+		dk_release_state(o);
+		return rc;
+	}
+	*out = o;
+	return DK_OK;
+}
+
+extern "C" int dk_read_raw(const dk_state *s, uint64_t first, uint64_t count,
+			   double *values, double *norm)
+{
+	DK_INIT();
+	if (!s) {
+		dk_set_error("NULL state");
+		return DK_E_ARG;
+	}
+	if (first > s->len || count > s->len - first) {
+		dk_set_error("Out of bounds");
+		return DK_E_ARG;
+	}
+	int rc = DK_OK;
+	if (count > 0 && values)
+		rc = copy_to_host(values, s->amp + first, 2 * count);
+	if (!rc && norm)
+		rc = copy_to_host(norm, s->scal + DK_SCAL_NORM, 1);
+	return rc;
+}
+
+extern "C" int dk_read(const dk_state *s, uint64_t first, uint64_t count,
+		       double *values)
+{
+	double nrm = 1.0;
+	int rc = dk_read_raw(s, first, count, values, &nrm);
+	if (rc)
+		return rc;
+	const double inv = 1.0 / nrm;
+	for (uint64_t i = 0; i < 2 * count; ++i)
+		values[i] *= inv;
+	return DK_OK;
+}
+
+extern "C" int dk_first_nonzero(const dk_state *s, uint64_t *index)
+{
+	DK_INIT();
+	if (!s || !index) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	// most states have a nonzero amplitude near the start: look there first
+	const u64 head = std::min<u64>(s->len, 2048);
+	std::vector<double> buf(2 * head);
+	int rc = copy_to_host(buf.data(), s->amp, 2 * head);
+	if (rc)
+		return rc;
+	for (u64 i = 0; i < head; ++i)
+		if (buf[2 * i] != 0.0 || buf[2 * i + 1] != 0.0) {
+			*index = i;
+			return DK_OK;
+		}
+	u64 *dst = (u64 *)(g_ctx.dec + DEC_U64);
+	rc = dk_launch_first_nonzero(s, head, dst);
+	if (rc)
+		return rc;
+	double tmp;
+	rc = copy_to_host(&tmp, dst, 1);
+	if (rc)
+		return rc;
+	memcpy(index, &tmp, sizeof(tmp));
+	return DK_OK;
+}
+
+static int popcount64(u64 x)
+{
+	return __builtin_popcountll(x);
+}
+
+extern "C" int dk_reduced_density(const dk_state *s, uint64_t keep_mask,
+				  double *rho)
+{
+	DK_INIT();
+	if (!s || !rho) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	const u64 valid = (1ull << s->nq) - 1;
+	if (keep_mask & ~valid) {
+		dk_set_error("kept qubit out of range");
+		return DK_E_ARG;
+	}
+	const int a = popcount64(keep_mask);
+	if (a > 12) {
+		dk_set_error("reduced density of %d qubits is too large to "
+			     "materialise (limit 12)", a);
+		return DK_E_ARG;
+	}
+	const u64 dim = 1ull << a;
+	double2 *dev = nullptr;
+	DK_HIP(hipMallocAsync((void **)&dev, dim * dim * sizeof(double2),
+			      g_ctx.stream));
+	int rc = dk_launch_density(s, keep_mask, dev);
+	if (!rc)
+		rc = copy_to_host(rho, dev, 2 * dim * dim);
+	(void)hipFreeAsync(dev, g_ctx.stream);
+	return rc;
+}
+
+extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask,
+					   size_t count, const uint64_t *rows,
+					   const uint64_t *cols, double *out)
+{
+	DK_INIT();
+	if (!s || (count && (!rows || !cols || !out))) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	const u64 valid = (1ull << s->nq) - 1;
+	if (keep_mask & ~valid) {
+		dk_set_error("kept qubit out of range");
+		return DK_E_ARG;
+	}
+	const u64 dim = 1ull << popcount64(keep_mask);
+	for (size_t e = 0; e < count; ++e)
+		if (rows[e] >= dim || cols[e] >= dim) {
+			dk_set_error("Out of bounds");
+			return DK_E_ARG;
+		}
+	if (count == 0)
+		return DK_OK;
+	char *buf = nullptr;
+	const size_t bytes = count * (2 * sizeof(u64) + sizeof(double2));
+	DK_HIP(hipMallocAsync((void **)&buf, bytes, g_ctx.stream));
+	u64 *drows = (u64 *)buf, *dcols = drows + count;
+	double2 *dout = (double2 *)(dcols + count);
+	int rc = DK_OK;
+	hipError_t e = hipMemcpyAsync(drows, rows, count * sizeof(u64),
+				      hipMemcpyHostToDevice, g_ctx.stream);
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(dcols, cols, count * sizeof(u64),
+				   hipMemcpyHostToDevice, g_ctx.stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(g_ctx.stream);
+	if (e != hipSuccess) {
+		dk_set_error("copy failed: %s", hipGetErrorString(e));
+		rc = DK_E_DEVICE;
+	}
+	if (!rc)
+		rc = dk_launch_density_elements(s, keep_mask, count, drows, dcols,
+						dout);
+	if (!rc)
+		rc = copy_to_host(out, dout, 2 * count);
+	(void)hipFreeAsync(buf, g_ctx.stream);
+	return rc;
+}
+
+// ---- timing and tuning ------------------------------------------------------------
+
+extern "C" int dk_event_record(int slot)
+{
+	DK_INIT();
+	if (slot < 0 || slot >= 64) {
+		dk_set_error("event slot out of range");
+		return DK_E_ARG;
+	}
+	DK_HIP(hipEventRecord(g_ctx.events[slot], g_ctx.stream));
+	return DK_OK;
+}
+
+extern "C" int dk_event_elapsed(int a, int b, float *ms)
+{
+	DK_INIT();
+	if (a < 0 || a >= 64 || b < 0 || b >= 64 || !ms) {
+		dk_set_error("event slot out of range");
+		return DK_E_ARG;
+	}
+	DK_HIP(hipEventSynchronize(g_ctx.events[b]));
+	DK_HIP(hipEventElapsedTime(ms, g_ctx.events[a], g_ctx.events[b]));
+	return DK_OK;
+}
+
+extern "C" int dk_tune(const char *key, int64_t value)
+{
+	DK_INIT();
+	if (!key) {
+		dk_set_error("NULL key");
+		return DK_E_ARG;
+	}
+	if (!strcmp(key, "apply_blocks")) {
+		g_ctx.apply_blocks = (int)value;
+		return DK_OK;
+	}
+	if (!strcmp(key, "time_apply")) {
+		g_ctx.time_apply = value != 0;
+		return DK_OK;
+	}
+	dk_set_error("unknown tunable %s", key);
+	return DK_E_ARG;
+}
+
+extern "C" int dk_device_info(int *num_cus, char *arch, size_t arch_len)
+{
+	DK_INIT();
+	hipDeviceProp_t prop;
+	DK_HIP(hipGetDeviceProperties(&prop, g_ctx.device));
+	if (num_cus)
+		*num_cus = prop.multiProcessorCount;
+	if (arch && arch_len)
+		snprintf(arch, arch_len, "%s", prop.gcnArchName);
+	return DK_OK;
+}

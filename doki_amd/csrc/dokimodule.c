@@ -1,0 +1,1328 @@
+/*
+ * dokimodule.c — the `doki` CPython extension of the MI355X state-vector
+ * core.  Same module name, method table, positional signatures, capsule
+ * names, exception (`qsimov.doki.error`) and error texts as the reference's
+ * extension (Mowstyl/Doki 1.6.1, src/doki/doki.c:114-222), so callers switch
+ * without changes.  All state work goes through the C ABI of libdoki_hip.so
+ * (include/doki_hip.h); there is no CPU compute path for states here.
+ *
+ * Deliberate differences (stricter, never silently different):
+ *  - registry_get / registry_prob / funmatrix ids are bounds-checked and
+ *    raise doki.error instead of reading out of bounds (reference doki.c:570,
+ *    :1227 do not check);
+ *  - repeated target qubits raise doki.error (doki.c:866-888 does not check);
+ *  - a registry passed to registry_del can no longer be used (raises instead
+ *    of dangling, doki.c:400-415);
+ *  - num_threads is validated and ignored (work runs on the GPU).
+ */
+#define PY_SSIZE_T_CLEAN
+#include <Python.h>
+#include <complex.h>
+#include <math.h>
+#include <numpy/arrayobject.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "doki_hip.h"
+
+/* not in the public header: library teardown guard */
+extern void dk_shutdown(void);
+
+#define CAP_STATE "qsimov.doki.state_vector"
+#define CAP_GATE "qsimov.doki.gate"
+#define CAP_FMAT "qsimov.doki.funmatrix"
+
+/* sizeof(struct FMatrix) and sizeof(struct DMatrixForTrace) of the reference
+ * on LP64 (funmatrix.h:28-66, funmatrix.c:31-37); funmatrix_mem reports them
+ * (funmatrix.c:1550-1567). */
+#define REF_FMATRIX_BYTES 112
+#define REF_PTRACE_ARGV_BYTES 24
+
+typedef double _Complex cplx;
+
+static PyObject *DokiError;
+
+static PyObject *dk_fail(int rc, const char *fallback)
+{
+	const char *m = dk_last_error();
+	(void)rc;
+	PyErr_SetString(DokiError, (m && m[0]) ? m : fallback);
+	return NULL;
+}
+
+static int check_threads(int num_threads)
+{
+	if (num_threads <= 0 && num_threads != -1) {
+		PyErr_SetString(DokiError, "num_threads must be at least 1 (or -1 "
+					   "to let OpenMP choose)");
+		return -1;
+	}
+	return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* registries                                                               */
+/* ------------------------------------------------------------------------ */
+
+#define CHUNK 4096 /* amplitudes per read-back chunk of registry_get */
+
+typedef struct {
+	dk_state *st;       /* NULL once deleted */
+	unsigned nq;
+	int norm_known;
+	double norm;
+	int phase_known;
+	double phase;
+	uint64_t cache_first; /* read-back chunk of raw amplitudes */
+	uint64_t cache_count;
+	double *cache;
+	double *probs;       /* all single-qubit probabilities, or NULL */
+} reg_t;
+
+static void reg_free(reg_t *r)
+{
+	if (!r)
+		return;
+	if (r->st)
+		dk_state_free(r->st);
+	PyMem_Free(r->cache);
+	PyMem_Free(r->probs);
+	PyMem_Free(r);
+}
+
+static void reg_capsule_destroy(PyObject *cap)
+{
+	reg_t *r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	if (r)
+		reg_free(r);
+}
+
+static PyObject *reg_wrap(dk_state *st)
+{
+	reg_t *r = (reg_t *)PyMem_Calloc(1, sizeof(reg_t));
+	PyObject *cap;
+	if (!r) {
+		dk_state_free(st);
+		return PyErr_NoMemory();
+	}
+	r->st = st;
+	r->nq = dk_state_num_qubits(st);
+	cap = PyCapsule_New(r, CAP_STATE, reg_capsule_destroy);
+	if (!cap)
+		reg_free(r);
+	return cap;
+}
+
+/* The registry of a capsule, or NULL with doki.error set. */
+static reg_t *reg_get(PyObject *cap, const char *nullmsg)
+{
+	reg_t *r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	if (!r) {
+		PyErr_SetString(DokiError, nullmsg);
+		return NULL;
+	}
+	if (!r->st) {
+		PyErr_SetString(DokiError, "registry has been deleted");
+		return NULL;
+	}
+	return r;
+}
+
+static int reg_norm(reg_t *r)
+{
+	if (!r->norm_known) {
+		int rc = dk_state_norm(r->st, &r->norm);
+		if (rc) {
+			dk_fail(rc, "Failed to read state");
+			return -1;
+		}
+		r->norm_known = 1;
+	}
+	return 0;
+}
+
+/* Normalised amplitude id: raw / norm with the C99 complex-by-real division
+ * of state_get (qstate.h:86). */
+static int reg_value(reg_t *r, uint64_t id, cplx *out)
+{
+	uint64_t len = (uint64_t)1 << r->nq;
+	if (reg_norm(r))
+		return -1;
+	if (!r->cache || id < r->cache_first ||
+	    id >= r->cache_first + r->cache_count) {
+		uint64_t first = id - id % CHUNK;
+		uint64_t count = len - first < CHUNK ? len - first : CHUNK;
+		int rc;
+		if (!r->cache) {
+			r->cache = (double *)PyMem_Malloc(2 * CHUNK * sizeof(double));
+			if (!r->cache) {
+				PyErr_NoMemory();
+				return -1;
+			}
+		}
+		rc = dk_read_raw(r->st, first, count, r->cache, NULL);
+		if (rc) {
+			r->cache_count = 0;
+			dk_fail(rc, "Failed to read state");
+			return -1;
+		}
+		r->cache_first = first;
+		r->cache_count = count;
+	}
+	{
+		uint64_t k = id - r->cache_first;
+		cplx raw = r->cache[2 * k] + I * r->cache[2 * k + 1];
+		*out = raw / r->norm;
+	}
+	return 0;
+}
+
+/* get_global_phase, qops.c:46-70: arg of the first nonzero normalised
+ * amplitude when its imaginary part is nonzero, else 0; cached. */
+static int reg_phase(reg_t *r, double *phase)
+{
+	if (!r->phase_known) {
+		uint64_t idx = 0, len = (uint64_t)1 << r->nq;
+		double ph = 0.0;
+		int rc = dk_first_nonzero(r->st, &idx);
+		if (rc) {
+			dk_fail(rc, "Failed to read state");
+			return -1;
+		}
+		if (idx < len) {
+			cplx v;
+			if (reg_value(r, idx, &v))
+				return -1;
+			if (cimag(v) != 0.)
+				ph = carg(v);
+		}
+		r->phase = ph;
+		r->phase_known = 1;
+	}
+	*phase = r->phase;
+	return 0;
+}
+
+static PyObject *doki_registry_new(PyObject *self, PyObject *args)
+{
+	unsigned int nq;
+	int verbose;
+	dk_state *st;
+	int rc;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Ip", &nq, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_new(num_qubits, verbose)");
+		return NULL;
+	}
+	if (nq == 0) {
+		PyErr_SetString(DokiError, "num_qubits can't be zero");
+		return NULL;
+	}
+	if (nq > DK_MAX_QUBITS) {
+		PyErr_SetString(DokiError, "Number of qubits exceeds maximum");
+		return NULL;
+	}
+	rc = dk_state_new(nq, 1, &st);
+	if (rc)
+		return dk_fail(rc, "Unknown error when creating state");
+	return reg_wrap(st);
+}
+
+static PyObject *doki_registry_clone(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int nthreads, verbose, rc;
+	reg_t *r;
+	dk_state *st;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Oip", &cap, &nthreads, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: registry_clone(registry, num_threads, verbose)");
+		return NULL;
+	}
+	if (check_threads(nthreads))
+		return NULL;
+	r = reg_get(cap, "NULL pointer to source registry");
+	if (!r)
+		return NULL;
+	rc = dk_state_clone(r->st, &st);
+	if (rc)
+		return dk_fail(rc, "Unknown error when cloning state");
+	return reg_wrap(st);
+}
+
+static PyObject *doki_registry_del(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	reg_t *r;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_del(registry, verbose)");
+		return NULL;
+	}
+	r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	if (r) {
+		if (r->st)
+			dk_state_free(r->st);
+		r->st = NULL;
+	} else {
+		PyErr_Clear();
+	}
+	Py_RETURN_NONE;
+}
+
+static int parse_complex(PyObject *o, cplx *out)
+{
+	if (PyComplex_Check(o)) {
+		*out = PyComplex_RealAsDouble(o) + I * PyComplex_ImagAsDouble(o);
+	} else if (PyFloat_Check(o)) {
+		*out = PyFloat_AsDouble(o) + I * 0.0;
+	} else if (PyLong_Check(o)) {
+		*out = (double)PyLong_AsLong(o) + I * 0.0;
+	} else {
+		return -1;
+	}
+	return 0;
+}
+
+static PyObject *doki_registry_new_data(PyObject *self, PyObject *args)
+{
+	PyObject *vals;
+	unsigned int nq;
+	short verbose;
+	uint64_t len, i;
+	double *buf = NULL;
+	dk_state *st;
+	int rc;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "IOh", &nq, &vals, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: registry_new_data(num_qubits, values, verbose)");
+		return NULL;
+	}
+	if (nq == 0) {
+		PyErr_SetString(DokiError, "num_qubits can't be zero");
+		return NULL;
+	}
+	if (nq > DK_MAX_QUBITS) {
+		PyErr_SetString(DokiError, "Number of qubits exceeds maximum");
+		return NULL;
+	}
+	len = (uint64_t)1 << nq;
+	if (PyArray_Check(vals)) {
+		PyArrayObject *arr = (PyArrayObject *)vals, *c;
+		if (!PyArray_ISNUMBER(arr)) {
+			PyErr_SetString(DokiError, "values have to be numbers");
+			return NULL;
+		}
+		if ((uint64_t)PyArray_SIZE(arr) != len) {
+			PyErr_SetString(DokiError, "Wrong array size for the specified "
+						   "number of qubits");
+			return NULL;
+		}
+		c = (PyArrayObject *)PyArray_FROMANY(vals, NPY_CDOUBLE, 0, 0,
+						     NPY_ARRAY_C_CONTIGUOUS |
+							     NPY_ARRAY_ALIGNED);
+		if (!c)
+			return NULL;
+		rc = dk_state_from_host(nq, (const double *)PyArray_DATA(c), &st);
+		Py_DECREF(c);
+	} else if (PyList_Check(vals)) {
+		if ((uint64_t)PyList_GET_SIZE(vals) != len) {
+			PyErr_SetString(DokiError, "Wrong list size for the specified "
+						   "number of qubits\n");
+			return NULL;
+		}
+		buf = (double *)PyMem_Malloc(2 * len * sizeof(double));
+		if (!buf)
+			return PyErr_NoMemory();
+		for (i = 0; i < len; i++) {
+			PyObject *o = PyList_GET_ITEM(vals, i);
+			double re = PyComplex_RealAsDouble(o);
+			double im = PyComplex_ImagAsDouble(o);
+			if (PyErr_Occurred()) {
+				PyMem_Free(buf);
+				PyErr_Clear();
+				PyErr_SetString(DokiError, "values have to be numbers");
+				return NULL;
+			}
+			buf[2 * i] = re;
+			buf[2 * i + 1] = im;
+		}
+		rc = dk_state_from_host(nq, buf, &st);
+		PyMem_Free(buf);
+	} else {
+		PyErr_SetString(DokiError, "values has to be either a python list "
+					   "or a numpy array");
+		return NULL;
+	}
+	if (rc)
+		return dk_fail(rc, "Unknown error when creating state");
+	return reg_wrap(st);
+}
+
+static PyObject *doki_registry_get(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned long long id;
+	int canonical, verbose;
+	reg_t *r;
+	cplx v;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OKpp", &cap, &id, &canonical, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: registry_get(registry, id, canonical, verbose)");
+		return NULL;
+	}
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	if (id >= ((uint64_t)1 << r->nq)) {
+		PyErr_SetString(DokiError, "Out of bounds");
+		return NULL;
+	}
+	if (reg_value(r, id, &v))
+		return NULL;
+	if (canonical) {
+		double ph;
+		cplx rot;
+		if (reg_phase(r, &ph))
+			return NULL;
+		rot = cos(ph) + I * -sin(ph);
+		v = v * rot;
+	}
+	return PyComplex_FromDoubles(creal(v), cimag(v));
+}
+
+static PyObject *doki_registry_mem(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	reg_t *r;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_mem(state, verbose)");
+		return NULL;
+	}
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	return PyLong_FromUnsignedLongLong(dk_state_mem_size(r->st));
+}
+
+/* ------------------------------------------------------------------------ */
+/* gates                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+	unsigned k;
+	unsigned dim;
+	double *mat;     /* dim*dim interleaved, row-major */
+	dk_gate *dev;    /* created on first use */
+} gate_t;
+
+static void gate_capsule_destroy(PyObject *cap)
+{
+	gate_t *g = (gate_t *)PyCapsule_GetPointer(cap, CAP_GATE);
+	if (!g)
+		return;
+	if (g->dev)
+		dk_gate_free(g->dev);
+	PyMem_Free(g->mat);
+	PyMem_Free(g);
+}
+
+static PyObject *doki_gate_new(PyObject *self, PyObject *args)
+{
+	PyObject *list, *cap;
+	unsigned int nq;
+	int verbose;
+	gate_t *g;
+	Py_ssize_t i, j, dim;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "IOp", &nq, &list, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: gate_new(num_qubits, gate, verbose)");
+		return NULL;
+	}
+	if (nq == 0) {
+		PyErr_SetString(DokiError, "num_qubits can't be zero");
+		return NULL;
+	}
+	if (!PyList_Check(list)) {
+		PyErr_SetString(DokiError, "gate must be a list of lists (matrix)");
+		return NULL;
+	}
+	if (nq > DK_MAX_GATE_QUBITS) {
+		PyErr_SetString(DokiError, "gates act on at most 6 qubits");
+		return NULL;
+	}
+	dim = (Py_ssize_t)1 << nq;
+	if (PyList_Size(list) != dim) {
+		PyErr_SetString(DokiError,
+				"Wrong matrix size for specified number of qubits");
+		return NULL;
+	}
+	g = (gate_t *)PyMem_Calloc(1, sizeof(gate_t));
+	if (!g)
+		return PyErr_NoMemory();
+	g->k = nq;
+	g->dim = (unsigned)dim;
+	g->mat = (double *)PyMem_Malloc(2 * dim * dim * sizeof(double));
+	if (!g->mat) {
+		PyMem_Free(g);
+		return PyErr_NoMemory();
+	}
+	for (i = 0; i < dim; i++) {
+		PyObject *row = PyList_GetItem(list, i);
+		if (!PyList_Check(row) || PyList_Size(row) != dim) {
+			PyErr_SetString(DokiError,
+					"rows must be lists of size 2^num_qubits");
+			goto fail;
+		}
+		for (j = 0; j < dim; j++) {
+			cplx v;
+			if (parse_complex(PyList_GetItem(row, j), &v)) {
+				PyErr_SetString(DokiError,
+						"matrix elements must be complex numbers");
+				goto fail;
+			}
+			g->mat[2 * (i * dim + j)] = creal(v);
+			g->mat[2 * (i * dim + j) + 1] = cimag(v);
+		}
+	}
+	cap = PyCapsule_New(g, CAP_GATE, gate_capsule_destroy);
+	if (!cap)
+		goto fail;
+	return cap;
+fail:
+	PyMem_Free(g->mat);
+	PyMem_Free(g);
+	return NULL;
+}
+
+static PyObject *doki_gate_get(PyObject *self, PyObject *args)
+{
+	PyObject *cap, *res;
+	int verbose;
+	gate_t *g;
+	unsigned i, j;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: gate_get(gate, verbose)");
+		return NULL;
+	}
+	g = (gate_t *)PyCapsule_GetPointer(cap, CAP_GATE);
+	if (!g) {
+		PyErr_SetString(DokiError, "NULL pointer to gate");
+		return NULL;
+	}
+	res = PyList_New(g->dim);
+	if (!res)
+		return NULL;
+	for (i = 0; i < g->dim; i++) {
+		PyObject *row = PyList_New(g->dim);
+		if (!row) {
+			Py_DECREF(res);
+			return NULL;
+		}
+		for (j = 0; j < g->dim; j++)
+			PyList_SET_ITEM(row, j,
+					PyComplex_FromDoubles(
+						g->mat[2 * (i * g->dim + j)],
+						g->mat[2 * (i * g->dim + j) + 1]));
+		PyList_SET_ITEM(res, i, row);
+	}
+	return res;
+}
+
+static dk_gate *gate_device(gate_t *g)
+{
+	if (!g->dev) {
+		int rc = dk_gate_new(g->k, g->mat, &g->dev);
+		if (rc) {
+			dk_fail(rc, "Failed to apply gate");
+			return NULL;
+		}
+	}
+	return g->dev;
+}
+
+/* ------------------------------------------------------------------------ */
+/* registry_apply: argument checks in the reference's order and wording     */
+/* (doki.c:722-941), then one device launch.                                */
+/* ------------------------------------------------------------------------ */
+
+static int pop_qubits(PyObject *set, unsigned count, unsigned *out,
+		      const char *typemsg)
+{
+	PyObject *aux = PySet_New(set);
+	unsigned i;
+	if (!aux)
+		return -1;
+	for (i = 0; i < count; i++) {
+		PyObject *v = PySet_Pop(aux);
+		if (!v || !PyLong_Check(v)) {
+			Py_XDECREF(v);
+			Py_DECREF(aux);
+			PyErr_SetString(DokiError, typemsg);
+			return -1;
+		}
+		out[i] = (unsigned)PyLong_AsLong(v);
+		Py_DECREF(v);
+	}
+	Py_DECREF(aux);
+	return 0;
+}
+
+static PyObject *doki_registry_apply(PyObject *self, PyObject *args)
+{
+	PyObject *scap, *gcap, *tlist, *cset, *aset, *aux;
+	int nthreads, verbose, rc;
+	reg_t *r;
+	gate_t *g;
+	dk_gate *dg;
+	dk_state *out;
+	unsigned nt, nc = 0, na = 0, i;
+	unsigned targets[DK_MAX_GATE_QUBITS];
+	unsigned *ctrl = NULL, *actrl = NULL;
+	uint64_t cm = 0, am = 0;
+	(void)self;
+
+	if (!PyArg_ParseTuple(args, "OOOOOip", &scap, &gcap, &tlist, &cset, &aset,
+			      &nthreads, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: registry_apply(registry, gate, target_list, "
+				"control_set, anticontrol_set, num_threads, verbose)");
+		return NULL;
+	}
+	if (check_threads(nthreads))
+		return NULL;
+	r = reg_get(scap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	g = (gate_t *)PyCapsule_GetPointer(gcap, CAP_GATE);
+	if (!g) {
+		PyErr_SetString(DokiError, "NULL pointer to gate");
+		return NULL;
+	}
+	if (!PyList_Check(tlist)) {
+		PyErr_SetString(DokiError, "target_list must be a list");
+		return NULL;
+	}
+	nt = (unsigned)PyList_Size(tlist);
+	if (nt != g->k) {
+		PyErr_SetString(DokiError,
+				"Wrong number of targets specified for that gate");
+		return NULL;
+	}
+	if (PySet_Check(cset))
+		nc = (unsigned)PySet_Size(cset);
+	else if (cset != Py_None) {
+		PyErr_SetString(DokiError, "control_set must be a set or None");
+		return NULL;
+	}
+	if (PySet_Check(aset))
+		na = (unsigned)PySet_Size(aset);
+	else if (aset != Py_None) {
+		PyErr_SetString(DokiError, "anticontrol_set must be a set or None");
+		return NULL;
+	}
+	if (nc) {
+		ctrl = (unsigned *)PyMem_Malloc(nc * sizeof(unsigned));
+		if (!ctrl)
+			return PyErr_NoMemory();
+		if (pop_qubits(cset, nc, ctrl,
+			       "control_set must be a set qubit ids (unsigned integers)"))
+			goto fail;
+		for (i = 0; i < nc; i++) {
+			if (ctrl[i] >= r->nq) {
+				PyErr_SetString(DokiError, "Control qubit out of range");
+				goto fail;
+			}
+			cm |= (uint64_t)1 << ctrl[i];
+		}
+	}
+	if (na) {
+		aux = PySet_New(aset);
+		if (!aux)
+			goto fail;
+		actrl = (unsigned *)PyMem_Malloc(na * sizeof(unsigned));
+		if (!actrl) {
+			Py_DECREF(aux);
+			PyErr_NoMemory();
+			goto fail;
+		}
+		for (i = 0; i < na; i++) {
+			PyObject *v = PySet_Pop(aux);
+			if (!v || !PyLong_Check(v)) {
+				Py_XDECREF(v);
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError,
+						"anticontrol_set must be a set "
+						"qubit ids (unsigned integers)");
+				goto fail;
+			}
+			/* PySet_Contains(None, ...) is -1, i.e. true (Q9) */
+			if (PySet_Contains(cset, v)) {
+				Py_DECREF(v);
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError,
+						"A control cannot also be an anticontrol");
+				goto fail;
+			}
+			actrl[i] = (unsigned)PyLong_AsLong(v);
+			Py_DECREF(v);
+			if (actrl[i] >= r->nq) {
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError,
+						"Anticontrol qubit out of range");
+				goto fail;
+			}
+			am |= (uint64_t)1 << actrl[i];
+		}
+		Py_DECREF(aux);
+	}
+	for (i = 0; i < nt; i++) {
+		PyObject *v = PyList_GetItem(tlist, i);
+		unsigned j;
+		if (!PyLong_Check(v)) {
+			PyErr_SetString(DokiError, "target_list must be a list of "
+						   "qubit ids (unsigned integers)");
+			goto fail;
+		}
+		if ((nc > 0 && PySet_Contains(cset, v)) ||
+		    (na > 0 && PySet_Contains(aset, v))) {
+			PyErr_SetString(DokiError, "A target cannot also be a control "
+						   "or an anticontrol");
+			goto fail;
+		}
+		targets[i] = (unsigned)PyLong_AsLong(v);
+		if (targets[i] >= r->nq) {
+			PyErr_SetString(DokiError, "Target qubit out of range");
+			goto fail;
+		}
+		for (j = 0; j < i; j++)
+			if (targets[j] == targets[i]) {
+				PyErr_SetString(DokiError, "Repeated target qubit");
+				goto fail;
+			}
+	}
+	PyMem_Free(ctrl);
+	PyMem_Free(actrl);
+	ctrl = actrl = NULL;
+	dg = gate_device(g);
+	if (!dg)
+		return NULL;
+	rc = dk_apply(r->st, dg, targets, cm, am, 0, &out);
+	if (rc)
+		return dk_fail(rc, "Failed to apply gate");
+	return reg_wrap(out);
+fail:
+	PyMem_Free(ctrl);
+	PyMem_Free(actrl);
+	return NULL;
+}
+
+static PyObject *doki_registry_join(PyObject *self, PyObject *args)
+{
+	PyObject *c1, *c2;
+	int nthreads, verbose, rc;
+	reg_t *a, *b;
+	dk_state *out;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OOip", &c1, &c2, &nthreads, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_join(most_registry, "
+					   "least_registry, num_threads, verbose)");
+		return NULL;
+	}
+	if (check_threads(nthreads))
+		return NULL;
+	a = reg_get(c1, "NULL pointer to registry1");
+	if (!a)
+		return NULL;
+	b = reg_get(c2, "NULL pointer to registry2");
+	if (!b)
+		return NULL;
+	if (a->nq + b->nq > DK_MAX_QUBITS) {
+		PyErr_SetString(DokiError, "Number of qubits exceeds maximum");
+		return NULL;
+	}
+	rc = dk_join(a->st, b->st, &out);
+	if (rc)
+		return dk_fail(rc, "Unknown error when joining states");
+	return reg_wrap(out);
+}
+
+static PyObject *doki_registry_measure(PyObject *self, PyObject *args)
+{
+	PyObject *cap, *rolls_obj, *lst, *newcap, *res;
+	unsigned long long mask;
+	int nthreads, verbose, rc;
+	reg_t *r;
+	unsigned n, i, m = 0;
+	double *rolls = NULL;
+	int8_t results[64];
+	dk_state *out = NULL;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OKOip", &cap, &mask, &rolls_obj, &nthreads,
+			      &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_measure(registry, mask, "
+					   "roll_list, num_threads, verbose)");
+		return NULL;
+	}
+	if (check_threads(nthreads))
+		return NULL;
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	if (!PyList_Check(rolls_obj)) {
+		PyErr_SetString(DokiError,
+				"roll_list must be a list of real numbers in [0, 1)!");
+		return NULL;
+	}
+	n = r->nq;
+	for (i = 0; i < n; i++)
+		if ((mask >> i) & 1)
+			m++;
+	rolls = (double *)PyMem_Malloc((m ? m : 1) * sizeof(double));
+	if (!rolls)
+		return PyErr_NoMemory();
+	for (i = 0; i < m; i++) {
+		PyObject *o = PyList_GetItem(rolls_obj, i);
+		double v = o ? PyFloat_AsDouble(o) : -1.0;
+		if (PyErr_Occurred())
+			PyErr_Clear();
+		if (!(v >= 0 && v < 1)) {
+			PyMem_Free(rolls);
+			PyErr_SetString(DokiError, "roll not in interval [0, 1)!");
+			return NULL;
+		}
+		rolls[i] = v;
+	}
+	rc = dk_measure(r->st, (uint64_t)mask, rolls, m, results, NULL, &out);
+	PyMem_Free(rolls);
+	if (rc)
+		return dk_fail(rc, "Unknown error while collapsing state");
+	lst = PyList_New(n);
+	if (!lst) {
+		if (out)
+			dk_state_free(out);
+		return NULL;
+	}
+	for (i = 0; i < n; i++) {
+		PyObject *v = results[i] < 0 ? Py_None
+					     : (results[i] ? Py_True : Py_False);
+		Py_INCREF(v);
+		PyList_SET_ITEM(lst, i, v);
+	}
+	if (out) {
+		newcap = reg_wrap(out);
+		if (!newcap) {
+			Py_DECREF(lst);
+			return NULL;
+		}
+	} else {
+		newcap = Py_None;
+		Py_INCREF(newcap);
+	}
+	res = PyTuple_Pack(2, newcap, lst);
+	Py_DECREF(newcap);
+	Py_DECREF(lst);
+	return res;
+}
+
+static PyObject *doki_registry_prob(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned int id;
+	int nthreads, verbose, rc;
+	reg_t *r;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OIip", &cap, &id, &nthreads, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_prob(registry, "
+					   "qubit_id, num_threads, verbose)");
+		return NULL;
+	}
+	if (check_threads(nthreads))
+		return NULL;
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	if (id >= r->nq) {
+		PyErr_SetString(DokiError, "Qubit id out of range");
+		return NULL;
+	}
+	/* States never change after creation: one pass yields every qubit's
+	 * probability, later calls are answered from the cache. */
+	if (!r->probs) {
+		double *p = (double *)PyMem_Malloc(r->nq * sizeof(double));
+		if (!p)
+			return PyErr_NoMemory();
+		rc = dk_prob_all(r->st, p);
+		if (rc) {
+			PyMem_Free(p);
+			return dk_fail(rc, "Failed to compute probability");
+		}
+		r->probs = p;
+	}
+	return PyFloat_FromDouble(r->probs[id]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* funmatrix: density matrices of registries and their partial traces       */
+/* ------------------------------------------------------------------------ */
+
+enum { FM_DENSITY = 1, FM_PTRACE = 2 };
+
+typedef struct fm_t {
+	int kind;
+	int64_t r, c;
+	size_t mem;           /* bytes the reference would report */
+	/* FM_DENSITY: reduced density of the kept qubits of a registry */
+	PyObject *state_cap;  /* owned reference */
+	uint64_t keep_mask;
+	unsigned keep_count;
+	unsigned char keep_pos[64];
+	double *cache;        /* materialised matrix, or NULL */
+	/* FM_PTRACE: generic partial trace of a parent funmatrix */
+	PyObject *parent_cap; /* owned reference */
+	int e;
+} fm_t;
+
+#define FM_MATERIALISE_MAX_QUBITS 10
+
+static void fm_capsule_destroy(PyObject *cap)
+{
+	fm_t *f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f)
+		return;
+	Py_XDECREF(f->state_cap);
+	Py_XDECREF(f->parent_cap);
+	PyMem_Free(f->cache);
+	PyMem_Free(f);
+}
+
+static PyObject *fm_wrap(fm_t *f)
+{
+	PyObject *cap = PyCapsule_New(f, CAP_FMAT, fm_capsule_destroy);
+	if (!cap) {
+		Py_XDECREF(f->state_cap);
+		Py_XDECREF(f->parent_cap);
+		PyMem_Free(f);
+	}
+	return cap;
+}
+
+static PyObject *doki_registry_density(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	reg_t *r;
+	fm_t *f;
+	unsigned q;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_density(state, verbose)");
+		return NULL;
+	}
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	f = (fm_t *)PyMem_Calloc(1, sizeof(fm_t));
+	if (!f)
+		return PyErr_NoMemory();
+	f->kind = FM_DENSITY;
+	f->r = f->c = (int64_t)1 << r->nq;
+	f->mem = REF_FMATRIX_BYTES + (size_t)dk_state_mem_size(r->st);
+	f->state_cap = cap;
+	Py_INCREF(cap);
+	f->keep_count = r->nq;
+	for (q = 0; q < r->nq; q++)
+		f->keep_pos[q] = (unsigned char)q;
+	f->keep_mask = r->nq >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << r->nq) - 1);
+	return fm_wrap(f);
+}
+
+/* Element (i, j) following getitem's rules (funmatrix.c:238-340): 0 ok,
+ * 7 out of bounds, 8 NaN, -1 Python error set. */
+static int fm_get(fm_t *f, int64_t i, int64_t j, cplx *out);
+
+static int64_t ins_bit(int value, int64_t pos, int bit)
+{
+	int64_t step = (int64_t)1 << bit;
+	return pos % step + (pos / step) * (step << 1) + value * step;
+}
+
+static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
+{
+	reg_t *r = reg_get(f->state_cap, "NULL pointer to registry");
+	int rc;
+	if (!r)
+		return -1;
+	if (f->keep_count <= FM_MATERIALISE_MAX_QUBITS) {
+		if (!f->cache) {
+			size_t n = (size_t)f->r * (size_t)f->r;
+			f->cache = (double *)PyMem_Malloc(2 * n * sizeof(double));
+			if (!f->cache) {
+				PyErr_NoMemory();
+				return -1;
+			}
+			rc = dk_reduced_density(r->st, f->keep_mask, f->cache);
+			if (rc) {
+				PyMem_Free(f->cache);
+				f->cache = NULL;
+				dk_fail(rc, "[DENSITY] Unknown error");
+				return -1;
+			}
+		}
+		{
+			size_t k = (size_t)i * (size_t)f->r + (size_t)j;
+			*out = f->cache[2 * k] + I * f->cache[2 * k + 1];
+		}
+		return 0;
+	} else {
+		uint64_t ri = (uint64_t)i, cj = (uint64_t)j;
+		double v[2];
+		rc = dk_reduced_density_elements(r->st, f->keep_mask, 1, &ri, &cj, v);
+		if (rc) {
+			dk_fail(rc, "[DENSITY] Unknown error");
+			return -1;
+		}
+		*out = v[0] + I * v[1];
+		return 0;
+	}
+}
+
+static int fm_get(fm_t *f, int64_t i, int64_t j, cplx *out)
+{
+	*out = NAN;
+	if (!(i >= 0 && j >= 0 && i < f->r && j < f->c))
+		return 7;
+	if (f->kind == FM_DENSITY) {
+		int rc = fm_density_get(f, i, j, out);
+		if (rc)
+			return rc;
+	} else {
+		/* _PartialTFunct, funmatrix.c:920-944 */
+		fm_t *p = (fm_t *)PyCapsule_GetPointer(f->parent_cap, CAP_FMAT);
+		cplx a = 0, b = 0;
+		int r1, r2 = 0;
+		if (!p) {
+			PyErr_Clear();
+			return 8;
+		}
+		r1 = fm_get(p, ins_bit(0, i, f->e), ins_bit(0, j, f->e), &a);
+		if (r1 < 0)
+			return -1;
+		if (r1 == 0) {
+			r2 = fm_get(p, ins_bit(1, i, f->e), ins_bit(1, j, f->e), &b);
+			if (r2 < 0)
+				return -1;
+		}
+		*out = (r1 == 0 && r2 == 0) ? a + b : a;
+	}
+	if (isnan(creal(*out)) || isnan(cimag(*out)))
+		return 8;
+	return 0;
+}
+
+static const char *get_err_text(int code, int trace)
+{
+	switch (code) {
+	case 7:
+		return trace ? "[TRACE] Element out of bounds" : "[GET] Element out of bounds";
+	case 8:
+		return trace ? "[TRACE] f returned NAN" : "[GET] f returned NAN";
+	default:
+		return trace ? "[TRACE] Unknown error code" : "[GET] Unknown error code";
+	}
+}
+
+static PyObject *doki_funmatrix_get(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned long long i, j;
+	int verbose, rc;
+	fm_t *f;
+	cplx v;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OKKp", &cap, &i, &j, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: funmatrix_get(funmatrix, i, j, verbose)");
+		return NULL;
+	}
+	f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f) {
+		PyErr_SetString(DokiError, "NULL pointer to matrix");
+		return NULL;
+	}
+	if ((int64_t)i < 0 || (int64_t)j < 0 || (int64_t)i >= f->r ||
+	    (int64_t)j >= f->c) {
+		PyErr_SetString(DokiError, "Out of bounds");
+		return NULL;
+	}
+	rc = fm_get(f, (int64_t)i, (int64_t)j, &v);
+	if (rc < 0)
+		return NULL;
+	if (rc) {
+		PyErr_SetString(DokiError, get_err_text(rc, 0));
+		return NULL;
+	}
+	if (isnan(creal(v)) || isnan(cimag(v))) {
+		PyErr_SetString(DokiError, "[GET] Unexpected NAN value");
+		return NULL;
+	}
+	return PyComplex_FromDoubles(creal(v), cimag(v));
+}
+
+static PyObject *doki_funmatrix_shape(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	fm_t *f;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_shape(funmatrix, verbose)");
+		return NULL;
+	}
+	f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f) {
+		PyErr_SetString(DokiError, "NULL pointer to matrix");
+		return NULL;
+	}
+	return Py_BuildValue("(KK)", (unsigned long long)f->r,
+			     (unsigned long long)f->c);
+}
+
+static PyObject *doki_funmatrix_partialtrace(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned int id;
+	int verbose, e;
+	fm_t *f, *o;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OIp", &cap, &id, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: funmatrix_partialtrace(funmatrix, id, verbose)");
+		return NULL;
+	}
+	f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f) {
+		PyErr_SetString(DokiError, "[PTRACE] The matrix is NULL");
+		return NULL;
+	}
+	if (f->r != f->c) {
+		PyErr_SetString(DokiError, "[PTRACE] The matrix is not a square matrix");
+		return NULL;
+	}
+	e = (int)id;
+	if (e < 0) {
+		PyErr_SetString(DokiError, "[PTRACE] elem id has to be >= 0");
+		return NULL;
+	}
+	o = (fm_t *)PyMem_Calloc(1, sizeof(fm_t));
+	if (!o)
+		return PyErr_NoMemory();
+	o->r = f->r >> 1;
+	o->c = f->c >> 1;
+	o->mem = REF_FMATRIX_BYTES + REF_PTRACE_ARGV_BYTES + f->mem;
+	if (f->kind == FM_DENSITY && (unsigned)e < f->keep_count) {
+		/* trace out the e-th remaining qubit: same node kind, one fewer
+		 * kept qubit (higher ones shift down, funmatrix.c:905-918) */
+		unsigned k, w = 0;
+		o->kind = FM_DENSITY;
+		o->state_cap = f->state_cap;
+		Py_INCREF(o->state_cap);
+		for (k = 0; k < f->keep_count; k++)
+			if ((int)k != e)
+				o->keep_pos[w++] = f->keep_pos[k];
+		o->keep_count = w;
+		o->keep_mask = f->keep_mask & ~((uint64_t)1 << f->keep_pos[e]);
+	} else {
+		o->kind = FM_PTRACE;
+		o->parent_cap = cap;
+		Py_INCREF(cap);
+		o->e = e;
+	}
+	return fm_wrap(o);
+}
+
+static PyObject *doki_funmatrix_trace(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	fm_t *f;
+	int64_t i, n;
+	cplx acc = 0, v;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_trace(funmatrix, verbose)");
+		return NULL;
+	}
+	f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f) {
+		PyErr_SetString(DokiError, "NULL pointer to matrix");
+		return NULL;
+	}
+	n = f->r <= f->c ? f->r : f->c;
+	for (i = 0; i < n; i++) {
+		int rc = fm_get(f, i, i, &v);
+		if (rc < 0)
+			return NULL;
+		if (rc) {
+			PyErr_SetString(DokiError, get_err_text(rc, 1));
+			return NULL;
+		}
+		if (isnan(creal(v)) || isnan(cimag(v))) {
+			PyErr_SetString(DokiError, "[TRACE] Unexpected NAN value");
+			return NULL;
+		}
+		acc = acc + v;
+	}
+	return PyComplex_FromDoubles(creal(acc), cimag(acc));
+}
+
+static PyObject *doki_funmatrix_mem(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	fm_t *f;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_mem(fmatrix, verbose)");
+		return NULL;
+	}
+	f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f) {
+		PyErr_SetString(DokiError, "NULL pointer to FMatrix");
+		return NULL;
+	}
+	return PyLong_FromSize_t(f->mem);
+}
+
+/* The lazy funmatrix algebra of the reference (doki.c:1297-1497,
+ * 1581-2055, 2217-2436) is not part of the state-vector core; these entries
+ * exist so that the method table matches, and say so when called. */
+#define FM_UNSUPPORTED(name)                                                 \
+	static PyObject *doki_##name(PyObject *self, PyObject *args)         \
+	{                                                                    \
+		(void)self;                                                  \
+		(void)args;                                                  \
+		PyErr_SetString(DokiError, #name ": the lazy funmatrix algebra " \
+					   "is not provided by the device core"); \
+		return NULL;                                                 \
+	}
+
+FM_UNSUPPORTED(funmatrix_create)
+FM_UNSUPPORTED(funmatrix_identity)
+FM_UNSUPPORTED(funmatrix_statezero)
+FM_UNSUPPORTED(funmatrix_densityzero)
+FM_UNSUPPORTED(funmatrix_hadamard)
+FM_UNSUPPORTED(funmatrix_addcontrol)
+FM_UNSUPPORTED(funmatrix_add)
+FM_UNSUPPORTED(funmatrix_sub)
+FM_UNSUPPORTED(funmatrix_scalar_mul)
+FM_UNSUPPORTED(funmatrix_scalar_div)
+FM_UNSUPPORTED(funmatrix_matmul)
+FM_UNSUPPORTED(funmatrix_ewmul)
+FM_UNSUPPORTED(funmatrix_kron)
+FM_UNSUPPORTED(funmatrix_eyekron)
+FM_UNSUPPORTED(funmatrix_transpose)
+FM_UNSUPPORTED(funmatrix_dagger)
+FM_UNSUPPORTED(funmatrix_projection)
+FM_UNSUPPORTED(funmatrix_apply)
+
+/* ------------------------------------------------------------------------ */
+
+static PyMethodDef DokiMethods[] = {
+	{"gate_new", doki_gate_new, METH_VARARGS, "Create new gate"},
+	{"gate_get", doki_gate_get, METH_VARARGS, "Get matrix associated to gate"},
+	{"registry_new", doki_registry_new, METH_VARARGS, "Create new registry"},
+	{"registry_new_data", doki_registry_new_data, METH_VARARGS,
+	 "Create new registry initialized with the specified values"},
+	{"registry_clone", doki_registry_clone, METH_VARARGS, "Clone a registry"},
+	{"registry_del", doki_registry_del, METH_VARARGS, "Destroy a registry"},
+	{"registry_get", doki_registry_get, METH_VARARGS, "Get value from registry"},
+	{"registry_apply", doki_registry_apply, METH_VARARGS, "Apply a gate"},
+	{"registry_join", doki_registry_join, METH_VARARGS, "Merges two registries"},
+	{"registry_measure", doki_registry_measure, METH_VARARGS,
+	 "Measures and collapses specified qubits"},
+	{"registry_prob", doki_registry_prob, METH_VARARGS,
+	 "Get the chances of obtaining 1 when measuring a certain qubit"},
+	{"registry_density", doki_registry_density, METH_VARARGS,
+	 "Get the density matrix"},
+	{"registry_mem", doki_registry_mem, METH_VARARGS,
+	 "Get the memory allocated by this registry in bytes"},
+	{"funmatrix_create", doki_funmatrix_create, METH_VARARGS,
+	 "Create a functional matrix from a matrix"},
+	{"funmatrix_identity", doki_funmatrix_identity, METH_VARARGS,
+	 "Create an identity functional matrix"},
+	{"funmatrix_statezero", doki_funmatrix_statezero, METH_VARARGS,
+	 "Create a functional matrix of the state vector |0>"},
+	{"funmatrix_densityzero", doki_funmatrix_densityzero, METH_VARARGS,
+	 "Create a functional matrix of the density matrix |0><0|"},
+	{"funmatrix_hadamard", doki_funmatrix_hadamard, METH_VARARGS,
+	 "Create a functional matrix of a Hadamard gate"},
+	{"funmatrix_addcontrol", doki_funmatrix_addcontrol, METH_VARARGS,
+	 "Add a control qubit to a gate"},
+	{"funmatrix_get", doki_funmatrix_get, METH_VARARGS,
+	 "Get a value from a functional matrix"},
+	{"funmatrix_add", doki_funmatrix_add, METH_VARARGS, "Matrix addition"},
+	{"funmatrix_sub", doki_funmatrix_sub, METH_VARARGS, "Matrix subtraction"},
+	{"funmatrix_scalar_mul", doki_funmatrix_scalar_mul, METH_VARARGS,
+	 "Scalar product"},
+	{"funmatrix_scalar_div", doki_funmatrix_scalar_div, METH_VARARGS,
+	 "Scalar division"},
+	{"funmatrix_matmul", doki_funmatrix_matmul, METH_VARARGS,
+	 "Matrix product"},
+	{"funmatrix_ewmul", doki_funmatrix_ewmul, METH_VARARGS,
+	 "Entity-wise product"},
+	{"funmatrix_kron", doki_funmatrix_kron, METH_VARARGS, "Kronecker product"},
+	{"funmatrix_eyekron", doki_funmatrix_eyekron, METH_VARARGS,
+	 "Kronecker product of I(2^left), U and I(2^right)"},
+	{"funmatrix_transpose", doki_funmatrix_transpose, METH_VARARGS,
+	 "Transpose"},
+	{"funmatrix_dagger", doki_funmatrix_dagger, METH_VARARGS,
+	 "Conjugate transpose"},
+	{"funmatrix_projection", doki_funmatrix_projection, METH_VARARGS,
+	 "Projection over a column vector"},
+	{"funmatrix_shape", doki_funmatrix_shape, METH_VARARGS,
+	 "Get a tuple with the shape of the matrix"},
+	{"funmatrix_partialtrace", doki_funmatrix_partialtrace, METH_VARARGS,
+	 "Get the partial trace of a functional matrix"},
+	{"funmatrix_trace", doki_funmatrix_trace, METH_VARARGS,
+	 "Get the trace of a functional matrix"},
+	{"funmatrix_apply", doki_funmatrix_apply, METH_VARARGS,
+	 "Apply a gate to a functional state vector"},
+	{"funmatrix_mem", doki_funmatrix_mem, METH_VARARGS,
+	 "Get the memory allocated by this FMatrix in bytes"},
+	{NULL, NULL, 0, NULL}};
+
+static struct PyModuleDef dokimodule = {PyModuleDef_HEAD_INIT, "doki",
+					"MI355X state-vector core (drop-in doki)",
+					-1, DokiMethods};
+
+static void doki_atexit(void)
+{
+	dk_shutdown();
+}
+
+PyMODINIT_FUNC PyInit_doki(void)
+{
+	PyObject *m;
+	import_array();
+	m = PyModule_Create(&dokimodule);
+	if (!m)
+		return NULL;
+	DokiError = PyErr_NewException("qsimov.doki.error", NULL, NULL);
+	Py_XINCREF(DokiError);
+	if (PyModule_AddObject(m, "error", DokiError) < 0) {
+		Py_XDECREF(DokiError);
+		Py_CLEAR(DokiError);
+		Py_DECREF(m);
+		return NULL;
+	}
+	Py_AtExit(doki_atexit);
+	return m;
+}

@@ -1,0 +1,173 @@
+/*
+ * doki_hip.h — C ABI of libdoki_hip.so, the MI355X (gfx950) state-vector core.
+ *
+ * Plain C: opaque handles, pointers and sizes only.  States live in device
+ * memory (HBM) for their whole life; only scalars (probabilities, outcomes,
+ * norms) and explicit read-outs cross to the host.  All work is enqueued on
+ * the library's HIP stream; calls that return host values synchronise it.
+ *
+ * Each entry point replaces one function of the reference's internal C API
+ * (Mowstyl/Doki 1.6.1, src/doki/qops.h, qstate.h) or one step of its Python
+ * wrappers (src/doki/doki.c); the citation is given per declaration.
+ *
+ * Value semantics follow the reference exactly: a state holds raw amplitudes
+ * and a normalisation constant, and every read returns raw / norm
+ * (qstate.h:84-86).  Amplitude index bit q is qubit q.  Complex numbers are
+ * interleaved (re, im) doubles.
+ *
+ * Return codes (int): 0 ok; the nonzero values mirror the reference's exit
+ * codes where one exists (qstate.c:23-88, qops.c:176-261).
+ */
+#ifndef DOKI_HIP_H_
+#define DOKI_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DK_OK 0
+#define DK_E_ALLOC 2        /* device allocation failed (qstate.c:44-82 code 2) */
+#define DK_E_QUBITS 3       /* qubit count outside 1..62 (qstate.c:29-31) */
+#define DK_E_ARG 5          /* invalid argument (ids, sizes, null handles) */
+#define DK_E_ROLL 6         /* measurement roll missing or outside [0, 1) */
+#define DK_E_ZERO_NORM 7    /* post-collapse normalisation is 0 */
+#define DK_E_DEVICE 8       /* HIP runtime / device failure */
+#define DK_E_BUDGET 9       /* state would exceed the device memory budget */
+
+#define DK_MAX_QUBITS 62    /* MAX_NUM_QUBITS = log2_64(INTMAX_MAX), platform.h:270 */
+#define DK_MAX_GATE_QUBITS 6
+
+typedef struct dk_state dk_state;
+typedef struct dk_gate dk_gate;
+
+/* Library set-up on a HIP device (first call wins; later calls are no-ops).
+ * No reference counterpart: the reference has no device. */
+int dk_init(int device);
+
+/* Message describing the last nonzero return code on this thread.
+ * Replaces the PyErr_SetString texts of doki.c at the C level. */
+const char *dk_last_error(void);
+
+/* Wait for all queued work.  No reference counterpart (the CPU path is
+ * synchronous). */
+int dk_synchronize(void);
+
+/* Device-memory budget in bytes for live states (default: 90% of free HBM at
+ * init); allocations beyond it fail with DK_E_BUDGET.  `used` may be NULL. */
+int dk_memory_budget(uint64_t *budget, uint64_t *used);
+int dk_set_memory_budget(uint64_t budget);
+
+/* ---- states: qstate.c / qstate.h ---------------------------------------- */
+
+/* |0...0> when zero_state != 0, else uninitialised; norm 1.
+ * Replaces state_init, qstate.c:23-88 (via registry_new, doki.c:302-341). */
+int dk_state_new(unsigned num_qubits, int zero_state, dk_state **out);
+
+/* State holding `values` (2^n interleaved complex) as given, norm 1 — not
+ * normalised.  Replaces registry_new_data + custom_state_init_*,
+ * doki.c:597-720. */
+int dk_state_from_host(unsigned num_qubits, const double *values,
+		       dk_state **out);
+
+/* Normalised copy (raw / norm, norm 1).  Replaces state_clone,
+ * qstate.c:90-105 (registry_clone, doki.c:343-398). */
+int dk_state_clone(const dk_state *src, dk_state **out);
+
+/* Releases the state's device memory.  Replaces state_clear,
+ * qstate.c:107-121 (registry_del / capsule destructor, doki.c:258-270). */
+void dk_state_free(dk_state *s);
+
+unsigned dk_state_num_qubits(const dk_state *s);
+
+/* Bytes the reference would report: 64 + 16 * 2^n on LP64.  Replaces
+ * state_mem_size, qstate.c:123-135 (registry_mem, doki.c:1272-1295). */
+uint64_t dk_state_mem_size(const dk_state *s);
+
+/* Bytes of device memory this state holds. */
+uint64_t dk_state_device_bytes(const dk_state *s);
+
+/* The state's normalisation constant (reads return raw / norm). */
+int dk_state_norm(const dk_state *s, double *norm);
+
+/* ---- gates: qgate.h ----------------------------------------------------- */
+
+/* A k-qubit gate (1 <= k <= 6): 2^k x 2^k complex matrix, row-major,
+ * interleaved; row = output basis state.  Copied to the device once.
+ * Replaces struct qgate, qgate.h:25-32, built by gate_new, doki.c:417-509. */
+int dk_gate_new(unsigned num_qubits, const double *matrix, dk_gate **out);
+void dk_gate_free(dk_gate *g);
+
+/* ---- operations: qops.c ------------------------------------------------- */
+
+/* out = gate applied to in: for every index whose control bits (ctrl_mask)
+ * are all 1 and anticontrol bits (actrl_mask) all 0,
+ *   out_i = sum_j M[row(i)][j] * in(i with target bits <- j),
+ * row(i) = sum_b bit(i, targets[b]) << b; other indices copy in_i.
+ * out's norm = sqrt(sum |out_i|^2).  Input values are read normalised.
+ * When in_place != 0 the result overwrites `in` and *out is set to `in`.
+ * Replaces apply_gate, qops.c:176-261 (registry_apply, doki.c:722-941). */
+int dk_apply(dk_state *in, const dk_gate *gate, const unsigned *targets,
+	     uint64_t ctrl_mask, uint64_t actrl_mask, int in_place,
+	     dk_state **out);
+
+/* P(qubit q = 1) = sum over indices with bit q set of |raw/norm|^2.
+ * Replaces probability, qops.c:72-95 (registry_prob, doki.c:1194-1228). */
+int dk_prob(const dk_state *s, unsigned q, double *p);
+
+/* All n single-qubit probabilities in one pass over the state.  Same values
+ * as n calls of dk_prob (summation order differs).  No single reference
+ * counterpart: batches n calls of probability, qops.c:72-95. */
+int dk_prob_all(const dk_state *s, double *p);
+
+/* Measurement of the qubits in `mask`, most significant first, consuming
+ * one roll per measured qubit: outcome = P1 > roll; the kept half is
+ * renormalised by sqrt(P), P = P1 or 1 - P1.  results[i] describes qubit
+ * n-1-i: -1 not measured, 0 or 1.  p1_out (may be NULL) receives P1 per
+ * results[] slot.  *out is the remaining state, or NULL when every qubit was
+ * measured.  Replaces the loop of registry_measure, doki.c:1023-1192, with
+ * measure / collapse, qops.c:124-174. */
+int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
+	       size_t num_rolls, int8_t *results, double *p1_out,
+	       dk_state **out);
+
+/* Kronecker join, `most` in the high qubits: r[i * 2^n2 + j] = a_i * b_j,
+ * norm 1.  Replaces join, qops.c:97-122 (registry_join, doki.c:943-1021). */
+int dk_join(const dk_state *most, const dk_state *least, dk_state **out);
+
+/* Index of the first amplitude whose raw value is nonzero (2^n if none).
+ * Device half of get_global_phase, qops.c:46-70. */
+int dk_first_nonzero(const dk_state *s, uint64_t *index);
+
+/* Raw stored amplitudes [first, first+count) and the norm constant, so the
+ * caller divides exactly as state_get does (qstate.h:86).  Read-out of
+ * registry_get, doki.c:546-595. */
+int dk_read_raw(const dk_state *s, uint64_t first, uint64_t count,
+		double *values, double *norm);
+
+/* Normalised values [first, first+count) (raw * (1/norm) on the device). */
+int dk_read(const dk_state *s, uint64_t first, uint64_t count,
+	    double *values);
+
+/* Reduced density matrix of the qubits in keep_mask (ascending qubit order =
+ * reduced index bit order), tracing out the rest:
+ *   rho[i][j] = sum_t psi(i, t) * conj(psi(j, t)),
+ * written row-major interleaved, dim = 2^popcount(keep_mask) <= 2^12.
+ * Replaces the lazy chain density_matrix / _densityFun (qops.c:266-293,
+ * 580-601) read through partial_trace / _PartialTFunct (funmatrix.c:905-1037)
+ * and funmatrix_get (doki.c:1499-1579). */
+int dk_reduced_density(const dk_state *s, uint64_t keep_mask, double *rho);
+
+/* Selected elements of the same reduced density matrix, for shapes too large
+ * to materialise: out[e] = rho[rows[e]][cols[e]]. */
+int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask,
+				size_t count, const uint64_t *rows,
+				const uint64_t *cols, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DOKI_HIP_H_ */

@@ -1,0 +1,316 @@
+"""Parity of the HIP path with the reference, on a gfx950 GPU.
+
+Every check goes through the C ABI of libdoki_hip.so — via the drop-in `doki`
+extension (which calls nothing else) or via the ctypes binding of the same
+entry points — and compares with either the reference's own golden vectors
+(tests/golden/) or the oracle on the same seeded inputs.
+Bar: FP64 amplitudes, probabilities and density elements within 1e-12
+max-abs; measurement outcomes identical given the same rolls.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from doki_amd import workloads as W
+from helpers import (ATOL, GOLDEN, doki_run, doki_values, load, oracle_run,
+                     reduced_density_einsum, unpack_steps)
+
+pytestmark = pytest.mark.gpu
+
+C0_FIXTURES = ["c0_n12_seed0.npz", "c0_n12_seed1.npz", "c0_n12_seed2.npz",
+               "c0_n6_seed7.npz"]
+
+
+def res_list(arr):
+    return [None if x < 0 else bool(x) for x in arr]
+
+
+def maxabs(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) if np.size(a) else 0.0
+
+
+# ---- against the reference's golden vectors ----------------------------------
+
+@pytest.mark.parametrize("name", C0_FIXTURES)
+def test_c0_golden(doki, name):
+    z = load(name)
+    n = int(z["n"])
+    reg = doki_run(doki, doki.registry_new(n, False), unpack_steps(z))
+    assert maxabs(doki_values(doki, reg, n), z["amps"]) <= ATOL
+    assert maxabs(doki_values(doki, reg, n, True), z["canon"]) <= ATOL
+    probs = [doki.registry_prob(reg, q, 1, False) for q in range(n)]
+    assert maxabs(probs, z["probs"]) <= ATOL
+    left, lst = doki.registry_measure(reg, (1 << n) - 1, z["rolls"].tolist(), 1,
+                                      False)
+    assert left is None
+    assert lst == res_list(z["measure_all"])
+    left, lst = doki.registry_measure(reg, int(z["half_mask"]),
+                                      z["rolls"][: n // 2].tolist(), 1, False)
+    assert lst == res_list(z["measure_half"])
+    assert maxabs(doki_values(doki, left, n - n // 2), z["half_state"]) <= ATOL
+
+
+def test_multi_target_golden(doki):
+    z = load("apply_multi.npz")
+    cases = json.load(open(f"{GOLDEN}/apply_multi.json"))
+    for i, c in enumerate(cases):
+        reg = doki.registry_new_data(c["n"], z[f"in{i}"], False)
+        g = doki.gate_new(len(c["targets"]), z[f"u{i}"].tolist(), False)
+        out = doki.registry_apply(reg, g, c["targets"], set(c["controls"]),
+                                  set(c["anticontrols"]), 1, False)
+        assert maxabs(doki_values(doki, out, c["n"]), z[f"out{i}"]) <= ATOL, c
+
+
+def test_measure_cases_golden(doki):
+    z = load("measure_cases.npz")
+    cases = json.load(open(f"{GOLDEN}/measure_cases.json"))
+    for i, c in enumerate(cases):
+        n = c["n"]
+        steps = [(z[f"mats{i}"][j], [int(z[f"tg{i}"][j])],
+                  {q for q in range(n) if (int(z[f"cm{i}"][j]) >> q) & 1},
+                  {q for q in range(n) if (int(z[f"am{i}"][j]) >> q) & 1})
+                 for j in range(len(z[f"mats{i}"]))]
+        reg = doki_run(doki, doki.registry_new(n, False), steps)
+        left, lst = doki.registry_measure(reg, c["mask"], c["rolls"], 1, False)
+        assert lst == res_list(z[f"res{i}"]), i
+        assert (left is not None) == c["has_state"]
+        if left is not None:
+            assert maxabs(doki_values(doki, left, c["left"]),
+                          z[f"state{i}"]) <= ATOL
+
+
+def test_misc_golden(doki):
+    z = load("misc.npz")
+    meta = json.load(open(f"{GOLDEN}/misc.json"))
+    for i in range(5):
+        reg = doki.registry_new_data(2, z[f"canon_in{i}"], False)
+        assert maxabs(doki_values(doki, reg, 2, True), z[f"canon_out{i}"]) <= ATOL
+    reg = doki.registry_new_data(4, z["unnorm_in"], False)
+    assert maxabs([doki.registry_prob(reg, q, 1, False) for q in range(4)],
+                  z["unnorm_probs"]) <= ATOL
+    assert maxabs(doki_values(doki, reg, 4), z["unnorm_get"]) <= ATOL
+    reg2 = doki_run(doki, reg, [(W.gate_h(), [2], set(), set())])
+    assert maxabs(doki_values(doki, reg2, 4), z["unnorm_after_h"]) <= ATOL
+    assert maxabs([doki.registry_prob(reg2, q, 1, False) for q in range(4)],
+                  z["unnorm_after_h_probs"]) <= ATOL
+    a = doki_run(doki, doki.registry_new(3, False), unpack_steps(z, "join_a_"))
+    b = doki_run(doki, doki.registry_new(2, False), unpack_steps(z, "join_b_"))
+    j = doki.registry_join(a, b, 1, False)
+    assert maxabs(doki_values(doki, j, 5), z["join_out"]) <= ATOL
+    d = doki_run(doki, doki.registry_new(5, False), unpack_steps(z, "dens_"))
+    rho = doki.registry_density(d, False)
+    full = np.array([[doki.funmatrix_get(rho, i, k, False) for k in range(32)]
+                     for i in range(32)])
+    assert maxabs(full, z["dens_full"]) <= ATOL
+    assert doki.funmatrix_mem(rho, False) == meta["dens_full_mem"]
+    tr = doki.funmatrix_trace(rho, False)
+    assert abs(tr - complex(*meta["dens_full_trace"])) <= ATOL
+    for ci, chain in enumerate(meta["dens_chains"]):
+        m = rho
+        for e in chain:
+            m = doki.funmatrix_partialtrace(m, e, False)
+        r, c = doki.funmatrix_shape(m, False)
+        assert [r, c] == meta["dens_shapes"][ci]
+        assert doki.funmatrix_mem(m, False) == meta["dens_mem"][ci]
+        got = np.array([[doki.funmatrix_get(m, i, k, False) for k in range(c)]
+                        for i in range(r)])
+        assert maxabs(got, z[f"dens_chain{ci}"]) <= ATOL, chain
+        t = doki.funmatrix_trace(m, False)
+        assert abs(t - complex(*meta["dens_trace"][ci])) <= ATOL
+    for n, b in meta["registry_mem"].items():
+        assert doki.registry_mem(doki.registry_new(int(n), False), False) == b
+
+
+ERRORS = json.load(open(f"{GOLDEN}/errors.json"))
+
+
+def test_error_texts_golden(doki):
+    g1 = doki.gate_new(1, W.gate_x().tolist(), False)
+    g2 = doki.gate_new(2, np.eye(4).tolist(), False)
+    r3 = doki.registry_new(3, False)
+    calls = {
+        "apply_threads": lambda: doki.registry_apply(r3, g1, [0], None, None, 0, False),
+        "apply_targets_not_list": lambda: doki.registry_apply(r3, g1, 0, None, None, 1, False),
+        "apply_wrong_ntargets": lambda: doki.registry_apply(r3, g1, [0, 1], None, None, 1, False),
+        "apply_controls_type": lambda: doki.registry_apply(r3, g1, [0], [1], None, 1, False),
+        "apply_anticontrols_type": lambda: doki.registry_apply(r3, g1, [0], None, [1], 1, False),
+        "apply_control_range": lambda: doki.registry_apply(r3, g1, [0], {5}, None, 1, False),
+        "apply_anticontrol_range": lambda: doki.registry_apply(r3, g1, [0], set(), {5}, 1, False),
+        "apply_anticontrol_none_controls": lambda: doki.registry_apply(r3, g1, [0], None, {1}, 1, False),
+        "apply_control_is_anticontrol": lambda: doki.registry_apply(r3, g1, [0], {1}, {1}, 1, False),
+        "apply_target_is_control": lambda: doki.registry_apply(r3, g1, [1], {1}, None, 1, False),
+        "apply_target_is_anticontrol": lambda: doki.registry_apply(r3, g1, [1], set(), {1}, 1, False),
+        "apply_target_range": lambda: doki.registry_apply(r3, g1, [3], None, None, 1, False),
+        "apply_target_type": lambda: doki.registry_apply(r3, g1, ["a"], None, None, 1, False),
+        "apply_2q_wrong_ntargets": lambda: doki.registry_apply(r3, g2, [0], None, None, 1, False),
+        "measure_rolls_type": lambda: doki.registry_measure(r3, 1, (0.5,), 1, False),
+        "measure_roll_range": lambda: doki.registry_measure(r3, 1, [1.0], 1, False),
+        "measure_roll_negative": lambda: doki.registry_measure(r3, 1, [-0.1], 1, False),
+        "measure_missing_roll": lambda: doki.registry_measure(r3, 3, [0.5], 1, False),
+        "measure_threads": lambda: doki.registry_measure(r3, 1, [0.5], -2, False),
+        "prob_threads": lambda: doki.registry_prob(r3, 0, 0, False),
+        "join_threads": lambda: doki.registry_join(r3, r3, 0, False),
+        "clone_threads": lambda: doki.registry_clone(r3, 0, False),
+        "funmatrix_get_oob": lambda: doki.funmatrix_get(doki.registry_density(r3, False), 8, 0, False),
+    }
+    for name, fn in calls.items():
+        with pytest.raises(doki.error) as ei:
+            fn()
+        assert str(ei.value) == ERRORS[name][1], name
+
+
+# ---- against the oracle on seeded random inputs --------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 9, 10, 11, 13, 14])
+def test_random_circuit_vs_oracle(doki, oracle, n):
+    steps, rolls = W.random_circuit(1000 + n, n, 60)
+    reg = doki_run(doki, doki.registry_new(n, False), steps)
+    st = oracle_run(oracle, oracle.new(n), steps)
+    assert maxabs(doki_values(doki, reg, n), st.values()) <= ATOL
+    probs = [doki.registry_prob(reg, q, 1, False) for q in range(n)]
+    assert maxabs(probs, [oracle.prob(st, q) for q in range(n)]) <= ATOL
+    left, lst = doki.registry_measure(reg, (1 << n) - 1, rolls, 1, False)
+    rc, oleft, olst = oracle.measure(st, (1 << n) - 1, rolls)
+    assert lst == olst and left is None and oleft is None
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("n", [6, 8, 13, 15])
+def test_k_qubit_gates_vs_oracle(capi, oracle, k, n):
+    if k > n:
+        pytest.skip("gate wider than the register")
+    rng = W.rng_for(7000 + 31 * n + k)
+    psi = W.random_normalised_state(50 + n + k, n)
+    dst = capi.from_host(psi)
+    ost = oracle.from_data(n, psi)
+    for rep in range(4):
+        targets = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        rest = [q for q in range(n) if q not in targets]
+        rng.shuffle(rest)
+        ctrl = rest[: rep % 3] if rest else []
+        actrl = rest[2: 2 + (rep % 2)] if len(rest) > 2 else []
+        u = W.haar_unitary(rng, k)
+        g = capi.Gate(u)
+        dst = capi.apply(dst, g, targets, ctrl, actrl)
+        ost = oracle.apply(ost, u, targets, sorted(ctrl), sorted(actrl))
+        assert maxabs(dst.values(), ost.values()) <= ATOL, (targets, ctrl, actrl)
+        assert abs(dst.norm - ost.norm) <= ATOL
+
+
+@pytest.mark.parametrize("t", [0, 1, 3, 5, 6, 9, 15])
+def test_one_qubit_targets_vs_oracle(capi, oracle, t):
+    n = 16
+    psi = W.random_normalised_state(t, n)
+    u = W.gate_u3(0.3 + t, 1.1, 2.5)
+    for ctrl, actrl in [((), ()), ((t ^ 1,), ()), ((), (t ^ 2,)),
+                        ((t ^ 4,), (t ^ 8,))]:
+        d = capi.apply(capi.from_host(psi), capi.Gate(u), [t], ctrl, actrl)
+        o = oracle.apply(oracle.from_data(n, psi), u, [t], list(ctrl), list(actrl))
+        assert maxabs(d.values(), o.values()) <= ATOL
+        assert abs(d.norm - o.norm) <= ATOL
+
+
+def test_in_place_equals_out_of_place(capi):
+    n = 14
+    psi = W.random_normalised_state(3, n)
+    rng = W.rng_for(4)
+    for k in (1, 2, 4, 6):
+        u = W.haar_unitary(rng, k)
+        targets = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        g = capi.Gate(u)
+        a = capi.from_host(psi)
+        b = capi.apply(a, g, targets)
+        c = capi.apply(capi.from_host(psi), g, targets, in_place=True)
+        assert np.array_equal(b.values(), c.values())
+
+
+@pytest.mark.parametrize("n", [9, 12, 17, 20])
+def test_prob_all_vs_oracle(capi, oracle, n):
+    psi = W.random_normalised_state(900 + n, n) * 1.7  # unnormalised on purpose
+    d = capi.from_host(psi)
+    o = oracle.from_data(n, psi)
+    pa = capi.prob_all(d)
+    assert maxabs(pa, [oracle.prob(o, q) for q in range(n)]) <= ATOL
+    assert maxabs(pa, [capi.prob(d, q) for q in range(n)]) <= ATOL
+
+
+@pytest.mark.parametrize("n,mask", [(10, 0b1011001101), (12, 0xFFF), (15, 0x5A5A),
+                                    (1, 1), (4, 0b110000), (6, 0)])
+def test_measure_vs_oracle(capi, oracle, n, mask):
+    steps, _ = W.random_circuit(33 + n, n, 40)
+    st = oracle_run(oracle, oracle.new(n), steps)
+    d = capi.from_host(st.values())
+    m = bin(mask & ((1 << n) - 1)).count("1")
+    rng = W.rng_for(77 + n)
+    for rep in range(5):
+        rolls = rng.random(m)
+        left, lst, p1 = capi.measure(d, mask, rolls)
+        rc, oleft, olst = oracle.measure(st, mask, rolls)
+        assert rc == 0 and lst == olst
+        assert (left is None) == (oleft is None)
+        if left is not None:
+            assert maxabs(left.values(), oleft.values()) <= ATOL
+
+
+def test_join_vs_oracle(capi, oracle):
+    for na, nb in [(1, 1), (3, 5), (9, 4), (2, 12)]:
+        a = W.random_normalised_state(na, na) * 0.5
+        b = W.random_normalised_state(nb + 40, nb) * 3.0
+        d = capi.join(capi.from_host(a), capi.from_host(b))
+        o = oracle.join(oracle.from_data(na, a), oracle.from_data(nb, b))
+        assert maxabs(d.values(), o.values()) <= ATOL
+
+
+@pytest.mark.parametrize("n,keep", [(5, [0, 1, 2, 3, 4]), (6, [1, 4]),
+                                    (14, [0, 3, 5, 8, 11, 13]), (10, []),
+                                    (9, [8]), (16, [2, 3, 4, 5, 6, 7, 8, 9, 10, 11])])
+def test_reduced_density_vs_einsum_and_oracle(capi, oracle, n, keep):
+    psi = W.random_normalised_state(400 + n, n)
+    d = capi.from_host(psi)
+    rho = capi.reduced_density(d, keep)
+    want = reduced_density_einsum(psi, keep)
+    assert maxabs(rho, want) <= ATOL
+    rows = np.arange(min(rho.shape[0], 7))
+    cols = rows[::-1].copy()
+    el = capi.reduced_density_elements(d, keep, rows, cols)
+    assert maxabs(el, rho[rows, cols]) <= ATOL
+    if n <= 10:
+        # the reference's chain of partial traces, evaluated by the oracle
+        traced = [q for q in range(n) if q not in keep]
+        chain, remaining = [], list(range(n))
+        for q in traced:
+            chain.append(remaining.index(q))
+            remaining.remove(q)
+        o = oracle.from_data(n, psi)
+        assert maxabs(rho, oracle.density_matrix(o, chain)) <= ATOL
+
+
+def test_first_nonzero(capi):
+    v = np.zeros(1 << 13, dtype=complex)
+    v[5000] = 0.6j
+    v[7000] = 0.8
+    assert capi.first_nonzero(capi.from_host(v)) == 5000
+    v[3] = 1e-300
+    assert capi.first_nonzero(capi.from_host(v)) == 3
+
+
+def test_canonical_after_measure_and_join(doki):
+    # tests/canonical_form_tests.py: canonical get removes the phase of the
+    # first amplitude after apply, join and measure
+    rng = W.rng_for(12)
+    n = 6
+    reg = doki.registry_new(n, False)
+    for q in range(n):
+        m = np.exp(1j * math.pi * (rng.random() * 2 - 1)) * np.eye(2)
+        reg = doki_run(doki, reg, [(m, [q], set(), set())])
+        raw = doki_values(doki, reg, n)
+        want = np.exp(-1j * np.angle(raw[0])) * raw
+        assert maxabs(doki_values(doki, reg, n, True), want) <= 1e-13
+    reg = doki_run(doki, reg, [(W.gate_h(), [q], set(), set()) for q in range(n)])
+    for i in range(n - 1):
+        reg, _ = doki.registry_measure(reg, 1, [float(rng.random())], 1, False)
+        raw = doki_values(doki, reg, n - i - 1)
+        want = np.exp(-1j * np.angle(raw[0])) * raw
+        assert maxabs(doki_values(doki, reg, n - i - 1, True), want) <= 1e-13
