@@ -82,7 +82,7 @@ def gpu_leg(args):
         kms.append(capi.event_elapsed_ms(62, 63))
     per_target = {}
     u3 = layers[-1][0][0]
-    for t in (0, 3, 6, n // 2, n - 1):
+    for t in sorted({0, 3, 6, 12, n // 2, n - 10, n - 5, n - 1}):
         ms = []
         for _ in range(5):
             reg = d.registry_apply(reg, u3, [t], set(), None, -1, False)
@@ -117,7 +117,9 @@ def cpu_leg(args, gb_per_gate_unit=True):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = args.cpu_qubits
     rng = W.rng_for(7)
-    steps = W.brickwork_layer(rng, n, 0) + W.brickwork_layer(rng, n, 1)
+    steps = []
+    for layer in range(40):
+        steps += W.brickwork_layer(rng, n, layer)
     if ref is None:
         # fall back to the single-threaded oracle port
         kind, threads = "port", 1

@@ -37,14 +37,14 @@ void dk_set_error(const char *fmt, ...);
 
 // ---- device-resident objects -------------------------------------------------
 
-// Per-state device scalars, stored in front of the amplitudes.
+// Per-state device scalars, stored after the amplitudes.
 //   [0] norm constant (reads return raw / norm)
 enum { DK_SCAL_NORM = 0, DK_SCAL_BYTES = 256 };
 
 struct dk_state {
 	unsigned nq;
 	u64 len;        // 2^nq amplitudes
-	char *base;     // device allocation: scalars then amplitudes
+	char *base;     // device allocation: amplitudes then scalars
 	double *scal;   // device scalars
 	double2 *amp;   // device amplitudes (raw)
 	u64 bytes;      // bytes of the allocation
@@ -65,7 +65,8 @@ struct dk_ctx {
 	int device = -1;
 	int num_cus = 0;
 	hipStream_t stream = nullptr;
-	double *partials = nullptr;   // reduction scratch (DK_PARTIALS doubles)
+	double *partials = nullptr;   // reduction scratch, grows on demand
+	u64 partials_cap = 0;         // doubles held by `partials`
 	double *dec = nullptr;        // measurement decisions scratch (device)
 	double *host_pinned = nullptr;// pinned staging (DK_PINNED doubles)
 	u64 budget = 0, used = 0;
@@ -90,6 +91,11 @@ void dk_release_state(dk_state *s);
 // finishes the norm of `out` on the device.
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm);
+
+// Shared partials buffer with room for `count` block partials plus the
+// chunked finishing pass; deterministic finishing sum of such partials.
+int dk_partials_reserve(u64 count, double **out);
+int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt);
 
 // sum_{i: bit q of i = 1} |raw_i / norm|^2 -> *dst (device scalar).
 int dk_launch_prob(const dk_state *s, unsigned q, double *dst);

@@ -91,8 +91,78 @@ struct Mat2 {
 	double2 m00, m01, m10, m11;
 };
 
-// Pair form: thread handles amplitude pairs (i0, i0 | 2^t).  For t >= 6 the
-// 64 lanes of a wave read two fully contiguous 1 KiB runs.
+// Streaming stores that bypass cache allocation: the output is not re-read
+// by this kernel.
+__device__ __forceinline__ void st_stream(double2 *p, double2 v)
+{
+	__builtin_nontemporal_store(v.x, &p->x);
+	__builtin_nontemporal_store(v.y, &p->y);
+}
+
+// Full-grid forms: one launch covers the state exactly once, one amplitude
+// per thread, blocks dispatched in address order so the amplitudes in flight
+// form a sliding window over HBM.  Each block leaves one partial |out|^2 sum.
+//
+// t >= 6: a block is two waves.  Wave 0 reads the lower amplitudes of 64
+// consecutive pairs (one contiguous 1 KiB run), wave 1 the partner run 2^t
+// amplitudes above; partners are exchanged through LDS.
+__global__ __launch_bounds__(128) void k_apply1_pairs(
+	const double2 *in, double2 *out, const double *__restrict__ nrm,
+	Mat2 M, unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+{
+	__shared__ double2 buf[128];
+	__shared__ double red[2];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 bit = 1ull << t, lo = bit - 1;
+	const u64 p = (u64)blockIdx.x * 64 + (threadIdx.x & 63);
+	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
+	const bool upper = threadIdx.x >= 64;
+	const u64 my = upper ? (i0 | bit) : i0;
+	const double2 v = cscale(in[my], inv);
+	buf[threadIdx.x] = v;
+	__syncthreads();
+	const double2 w = buf[threadIdx.x ^ 64];
+	const double2 x = upper ? w : v, y = upper ? v : w;
+	double2 o = v;
+	if ((i0 & cm) == cm && (i0 & acm) == 0) {
+		o = upper ? cmac(cmac(cz(), M.m10, x), M.m11, y)
+			  : cmac(cmac(cz(), M.m00, x), M.m01, y);
+	}
+	st_stream(out + my, o);
+	const double s = wave_sum(cabs2(o));
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = red[0] + red[1];
+}
+
+// t < 6: one wave per block over 64 consecutive amplitudes; the partner
+// amplitude comes from lane ^ 2^t.
+__global__ __launch_bounds__(64) void k_apply1_lanes(
+	const double2 *in, double2 *out, const double *__restrict__ nrm,
+	Mat2 M, unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+{
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
+	const double2 v = cscale(in[i], inv);
+	double2 w;
+	w.x = __shfl_xor(v.x, 1 << t);
+	w.y = __shfl_xor(v.y, 1 << t);
+	const bool upper = (i >> t) & 1;
+	const double2 x = upper ? w : v, y = upper ? v : w;
+	double2 o = v;
+	if ((i & cm) == cm && (i & acm) == 0) {
+		o = upper ? cmac(cmac(cz(), M.m10, x), M.m11, y)
+			  : cmac(cmac(cz(), M.m00, x), M.m01, y);
+	}
+	st_stream(out + i, o);
+	const double s = wave_sum(cabs2(o));
+	if (threadIdx.x == 0)
+		partials[blockIdx.x] = s;
+}
+
+// Small registers (fewer than 128 amplitudes): pairs, grid-stride, guarded.
 template <int U>
 __global__ __launch_bounds__(TPB) void k_apply1_pair(
 	const double2 *in, double2 *out, const double *__restrict__ nrm,
@@ -130,50 +200,6 @@ __global__ __launch_bounds__(TPB) void k_apply1_pair(
 			out[i0[u]] = o0;
 			out[i0[u] | bit] = o1;
 			acc += cabs2(o0) + cabs2(o1);
-		}
-	}
-	const double tot = block_sum(acc, red);
-	if (threadIdx.x == 0)
-		partials[blockIdx.x] = tot;
-}
-
-// Lane-exchange form for t < 6: lanes read consecutive amplitudes and fetch
-// the partner amplitude from lane ^ 2^t.  Requires len to be a multiple of
-// the grid stride (every lane of a wave runs the same trip count).
-template <int U>
-__global__ __launch_bounds__(TPB) void k_apply1_lane(
-	const double2 *in, double2 *out, const double *__restrict__ nrm,
-	Mat2 M, unsigned t, u64 cm, u64 acm, u64 len,
-	double *__restrict__ partials)
-{
-	__shared__ double red[WAVES];
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
-	const int sh = 1 << t;
-	const u64 stride = (u64)gridDim.x * TPB * U;
-	double acc = 0.0;
-	for (u64 i0 = (u64)blockIdx.x * TPB * U + threadIdx.x; i0 < len;
-	     i0 += stride) {
-		double2 a[U];
-#pragma unroll
-		for (int u = 0; u < U; ++u)
-			a[u] = in[i0 + (u64)u * TPB];
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const u64 i = i0 + (u64)u * TPB;
-			const double2 x = cscale(a[u], inv);
-			double2 y;
-			y.x = __shfl_xor(x.x, sh);
-			y.y = __shfl_xor(x.y, sh);
-			const bool hi = (i >> t) & 1;
-			const double2 v0 = hi ? y : x, v1 = hi ? x : y;
-			double2 o = x;
-			if ((i & cm) == cm && (i & acm) == 0) {
-				const double2 r0 = hi ? M.m10 : M.m00;
-				const double2 r1 = hi ? M.m11 : M.m01;
-				o = cmac(cmac(cz(), r0, v0), r1, v1);
-			}
-			out[i] = o;
-			acc += cabs2(o);
 		}
 	}
 	const double tot = block_sum(acc, red);
@@ -301,6 +327,25 @@ __global__ __launch_bounds__(TPB) void k_finish_sum(
 	s = block_sum(s, red);
 	if (threadIdx.x == 0)
 		dst[0] = take_sqrt ? sqrt(s) : s;
+}
+
+// First level of a long reduction: out[b] = sum of in[b*4096 .. b*4096+4095]
+// (in index order within each thread, then the block tree).
+constexpr int RCHUNK = 4096;
+__global__ __launch_bounds__(TPB) void k_reduce_chunks(
+	const double *__restrict__ in, u64 count, double *__restrict__ out)
+{
+	__shared__ double red[WAVES];
+	const u64 base = (u64)blockIdx.x * RCHUNK;
+	double s = 0.0;
+	for (int k = threadIdx.x; k < RCHUNK; k += TPB) {
+		const u64 i = base + k;
+		if (i < count)
+			s += in[i];
+	}
+	s = block_sum(s, red);
+	if (threadIdx.x == 0)
+		out[blockIdx.x] = s;
 }
 
 // Sum `cols` interleaved columns: dst[c] = sum_b partials[b * cols + c].
@@ -711,20 +756,29 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm)
 {
 	hipStream_t st = g_ctx.stream;
-	double *parts = g_ctx.partials;
-	unsigned nblocks = 0;
+	u64 nblocks = 0;
 	const unsigned n = in->nq;
+	// one partial per block of the largest grid below
+	double *parts = nullptr;
+	int rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
+	if (rc)
+		return rc;
 	if (g_ctx.time_apply)
 		DK_HIP(hipEventRecord(g_ctx.events[62], st));
 	if (g->k == 1) {
 		Mat2 M{g->hmat[0], g->hmat[1], g->hmat[2], g->hmat[3]};
 		const unsigned t = targets[0];
 		constexpr int U = 4;
-		if (t < 6 && in->len >= (u64)TPB * U) {
-			nblocks = grid_for(in->len, (u64)TPB * U);
-			hipLaunchKernelGGL(k_apply1_lane<U>, dim3(nblocks), dim3(TPB), 0,
-					   st, in->amp, out->amp, in->scal, M, t, cm,
-					   acm, in->len, parts);
+		if (in->len >= 128 && t >= 6) {
+			nblocks = in->len / 128;
+			hipLaunchKernelGGL(k_apply1_pairs, dim3((unsigned)nblocks), dim3(128),
+					   0, st, in->amp, out->amp, in->scal, M, t, cm,
+					   acm, parts);
+		} else if (in->len >= 128) {
+			nblocks = in->len / 64;
+			hipLaunchKernelGGL(k_apply1_lanes, dim3((unsigned)nblocks), dim3(64),
+					   0, st, in->amp, out->amp, in->scal, M, t, cm,
+					   acm, parts);
 		} else {
 			const u64 np = in->len >> 1;
 			nblocks = grid_for(np, (u64)TPB * U);
@@ -784,9 +838,9 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		nblocks = grid_for(ntiles, 1);
 #define DK_TILE_CASE(KK)                                                    \
 	case KK:                                                            \
-		hipLaunchKernelGGL(k_apply_tile<KK>, dim3(nblocks), dim3(TPB), 0, \
-				   st, in->amp, out->amp, in->scal, g->dmat, ta, \
-				   cm, acm, ntiles, parts);                 \
+		hipLaunchKernelGGL(k_apply_tile<KK>, dim3((unsigned)nblocks),   \
+				   dim3(TPB), 0, st, in->amp, out->amp, in->scal, \
+				   g->dmat, ta, cm, acm, ntiles, parts);    \
 		break;
 		switch (k) {
 			DK_TILE_CASE(1)
@@ -804,9 +858,45 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	DK_LAUNCHED();
 	if (g_ctx.time_apply)
 		DK_HIP(hipEventRecord(g_ctx.events[63], st));
-	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, st, parts,
-			   (int)nblocks, out->scal + DK_SCAL_NORM, 1);
+	return dk_finish_partials(parts, nblocks, out->scal + DK_SCAL_NORM, 1);
+}
+
+// Deterministic sum of `count` device partials into *dst (sqrt when asked):
+// one finishing block for short lists, otherwise 4096-wide chunks first.
+// The partials buffer holds room for the chunk sums after `count`.
+int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt)
+{
+	hipStream_t st = g_ctx.stream;
+	if (count <= 4 * RCHUNK) {
+		hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, st, parts,
+				   (int)count, dst, take_sqrt);
+		DK_LAUNCHED();
+		return DK_OK;
+	}
+	const u64 nb = (count + RCHUNK - 1) / RCHUNK;
+	double *second = parts + count;
+	hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)nb), dim3(TPB), 0, st,
+			   parts, count, second);
 	DK_LAUNCHED();
+	return dk_finish_partials(second, nb, dst, take_sqrt);
+}
+
+// Ensures the shared partials buffer holds `count` values plus the room the
+// chunked finishing pass needs after them.  Growth is stream-ordered.
+int dk_partials_reserve(u64 count, double **out)
+{
+	u64 need = count + 16;
+	for (u64 c = count; c > 4 * RCHUNK; c = (c + RCHUNK - 1) / RCHUNK)
+		need += (c + RCHUNK - 1) / RCHUNK;
+	if (need > g_ctx.partials_cap) {
+		double *p = nullptr;
+		DK_HIP(hipMallocAsync((void **)&p, need * sizeof(double), g_ctx.stream));
+		if (g_ctx.partials)
+			DK_HIP(hipFreeAsync(g_ctx.partials, g_ctx.stream));
+		g_ctx.partials = p;
+		g_ctx.partials_cap = need;
+	}
+	*out = g_ctx.partials;
 	return DK_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "dk_internal.h"
@@ -62,7 +63,9 @@ extern "C" int dk_init(int device)
 	g_ctx.num_cus = prop.multiProcessorCount;
 	hipStream_t st;
 	DK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-	DK_HIP(hipMalloc((void **)&g_ctx.partials, DK_PARTIALS * sizeof(double)));
+	DK_HIP(hipMallocAsync((void **)&g_ctx.partials, DK_PARTIALS * sizeof(double),
+			      st));
+	g_ctx.partials_cap = DK_PARTIALS;
 	DK_HIP(hipMalloc((void **)&g_ctx.dec, DEC_TOTAL * sizeof(double)));
 	DK_HIP(hipHostMalloc((void **)&g_ctx.host_pinned,
 			     DK_PINNED * sizeof(double), hipHostMallocDefault));
@@ -131,6 +134,94 @@ extern "C" int dk_set_memory_budget(uint64_t budget)
 	return DK_OK;
 }
 
+// ---- device memory ------------------------------------------------------------
+//
+// Small blocks come from the stream-ordered pool (cheap, recycled in stream
+// order).  Large blocks (state vectors of 2^22+ amplitudes) are plain
+// hipMalloc allocations kept in a size-keyed cache and handed out again in
+// stream order: they map with large pages, which high-stride gate passes over
+// multi-GiB states need to stream at full HBM rate, and reusing them avoids a
+// synchronising hipFree per gate.  Every block is used only on the library
+// stream, so a block released by one call can serve the next call at once.
+
+enum : u64 { DK_BIG_BLOCK = 64ull << 20 };
+
+static std::multimap<u64, void *> g_big_free;
+static u64 g_big_cached = 0;
+
+static void release_cached_blocks()
+{
+	if (g_big_free.empty())
+		return;
+	(void)hipStreamSynchronize(g_ctx.stream);
+	for (auto &kv : g_big_free)
+		(void)hipFree(kv.second);
+	g_big_free.clear();
+	g_big_cached = 0;
+}
+
+int dk_device_alloc(u64 bytes, void **out)
+{
+	*out = nullptr;
+	if (bytes < DK_BIG_BLOCK) {
+		hipError_t e = hipMallocAsync(out, bytes, g_ctx.stream);
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			std::lock_guard<std::mutex> lk(g_ctx.mu);
+			release_cached_blocks();
+			hipMemPool_t pool;
+			if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
+				(void)hipMemPoolTrimTo(pool, 0);
+			e = hipMallocAsync(out, bytes, g_ctx.stream);
+		}
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			dk_set_error("Failed to allocate state chunk: %s",
+				     hipGetErrorString(e));
+			return DK_E_ALLOC;
+		}
+		return DK_OK;
+	}
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	auto it = g_big_free.find(bytes);
+	if (it != g_big_free.end()) {
+		*out = it->second;
+		g_big_free.erase(it);
+		g_big_cached -= bytes;
+		return DK_OK;
+	}
+	hipError_t e = hipMalloc(out, bytes);
+	if (e != hipSuccess) {
+		(void)hipGetLastError();
+		release_cached_blocks();
+		hipMemPool_t pool;
+		(void)hipStreamSynchronize(g_ctx.stream);
+		if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
+			(void)hipMemPoolTrimTo(pool, 0);
+		e = hipMalloc(out, bytes);
+	}
+	if (e != hipSuccess) {
+		(void)hipGetLastError();
+		*out = nullptr;
+		dk_set_error("Failed to allocate state chunk: %s", hipGetErrorString(e));
+		return DK_E_ALLOC;
+	}
+	return DK_OK;
+}
+
+void dk_device_free(void *p, u64 bytes)
+{
+	if (!p || g_closed || !g_ctx.stream)
+		return;
+	if (bytes < DK_BIG_BLOCK) {
+		(void)hipFreeAsync(p, g_ctx.stream);
+		return;
+	}
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	g_big_free.emplace(bytes, p);
+	g_big_cached += bytes;
+}
+
 // ---- states -----------------------------------------------------------------
 
 int dk_alloc_state(unsigned nq, dk_state **out)
@@ -161,30 +252,19 @@ int dk_alloc_state(unsigned nq, dk_state **out)
 		g_ctx.used += bytes;
 	}
 	char *base = nullptr;
-	hipError_t e = hipMallocAsync((void **)&base, bytes, g_ctx.stream);
-	if (e != hipSuccess) {
-		(void)hipGetLastError();
-		// give cached pool memory back and retry once
-		hipMemPool_t pool;
-		if (hipStreamSynchronize(g_ctx.stream) == hipSuccess &&
-		    hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
-			(void)hipMemPoolTrimTo(pool, 0);
-		e = hipMallocAsync((void **)&base, bytes, g_ctx.stream);
-	}
-	if (e != hipSuccess) {
-		(void)hipGetLastError();
+	int rc = dk_device_alloc(bytes, (void **)&base);
+	if (rc) {
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		g_ctx.used -= bytes;
-		dk_set_error("Failed to allocate state chunk: %s",
-			     hipGetErrorString(e));
-		return DK_E_ALLOC;
+		return rc;
 	}
 	dk_state *s = new dk_state;
 	s->nq = nq;
 	s->len = len;
 	s->base = base;
-	s->scal = (double *)base;
-	s->amp = (double2 *)(base + DK_SCAL_BYTES);
+	// amplitudes first, so they start on the allocation's own alignment
+	s->amp = (double2 *)base;
+	s->scal = (double *)(base + len * sizeof(double2));
 	s->bytes = bytes;
 	*out = s;
 	return DK_OK;
@@ -195,7 +275,7 @@ void dk_release_state(dk_state *s)
 	if (!s)
 		return;
 	if (!g_closed && g_ctx.stream) {
-		(void)hipFreeAsync(s->base, g_ctx.stream);
+		dk_device_free(s->base, s->bytes);
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		g_ctx.used -= s->bytes;
 	}

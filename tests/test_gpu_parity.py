@@ -314,3 +314,17 @@ def test_canonical_after_measure_and_join(doki):
         raw = doki_values(doki, reg, n - i - 1)
         want = np.exp(-1j * np.angle(raw[0])) * raw
         assert maxabs(doki_values(doki, reg, n - i - 1, True), want) <= 1e-13
+
+
+@pytest.mark.parametrize("t", [0, 4, 7, 21])
+def test_large_register_two_level_norm(capi, oracle, t):
+    # 2^22 amplitudes: the gate kernels leave more than 16384 block partials,
+    # so the norm goes through the chunked finishing pass
+    n = 22
+    psi = W.random_normalised_state(2200 + t, n) * 0.8
+    u = W.gate_u3(1.0 + t, 0.4, 2.2)
+    for ctrl in ((), ((t + 1) % n,)):
+        d = capi.apply(capi.from_host(psi), capi.Gate(u), [t], ctrl)
+        o = oracle.apply(oracle.from_data(n, psi), u, [t], list(ctrl))
+        assert abs(d.norm - o.norm) <= ATOL
+        assert maxabs(d.values(), o.values()) <= ATOL
