@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 EXT = sysconfig.get_config_var("EXT_SUFFIX")
 
-HIP_SOURCES = ["dk_runtime.hip", "dk_kernels.hip"]
+HIP_SOURCES = ["dk_runtime.hip", "dk_kernels.hip", "dk_dense.hip",
+               "dk_circuit.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of the device code
@@ -53,7 +54,7 @@ def build_library(verbose=False):
     objs = []
     cmds = []
     for src in HIP_SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:

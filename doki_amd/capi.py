@@ -73,6 +73,17 @@ def lib():
         "dk_event_elapsed": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "dk_tune": (C.c_int, [C.c_char_p, C.c_int64]),
         "dk_device_info": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+        "dk_state_new_typed": (C.c_int, [C.c_uint, C.c_int, C.c_int, PP]),
+        "dk_state_dtype": (C.c_int, [P]),
+        "dk_state_convert": (C.c_int, [P, C.c_int, PP]),
+        "dk_circuit_new": (C.c_int, [C.c_uint, PP]),
+        "dk_circuit_free": (None, [P]),
+        "dk_circuit_add": (C.c_int, [P, C.c_uint, dp, C.POINTER(C.c_uint), u64, u64]),
+        "dk_circuit_fuse": (C.c_int, [P, C.c_uint, C.POINTER(C.c_size_t)]),
+        "dk_circuit_num_gates": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "dk_circuit_block": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_uint),
+                                       C.POINTER(C.c_uint), dp]),
+        "dk_circuit_run": (C.c_int, [P, P, C.c_int, PP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -231,6 +242,68 @@ def reduced_density_elements(s, keep, rows, cols):
         s.ptr, mask(keep), r.size, r.ctypes.data_as(u), c.ctypes.data_as(u),
         _dp(out)))
     return out.view(np.complex128)
+
+
+C128, C64 = 0, 1
+
+
+def new_typed(nq, dtype):
+    p = C.c_void_p()
+    check(lib().dk_state_new_typed(nq, 1, dtype, C.byref(p)))
+    return State(p.value)
+
+
+def convert(s, dtype):
+    p = C.c_void_p()
+    check(lib().dk_state_convert(s.ptr, dtype, C.byref(p)))
+    return State(p.value)
+
+
+def dtype_of(s):
+    return lib().dk_state_dtype(s.ptr)
+
+
+class Circuit:
+    """A gate list with optional fusion into dense blocks (C ABI dk_circuit_*)."""
+
+    def __init__(self, nq, steps=()):
+        p = C.c_void_p()
+        check(lib().dk_circuit_new(nq, C.byref(p)))
+        self.ptr = p
+        self.nq = nq
+        for m, t, c, a in steps:
+            self.add(m, t, c, a)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _lib is not None:
+            _lib.dk_circuit_free(self.ptr)
+            self.ptr = None
+
+    def add(self, matrix, targets, controls=(), anticontrols=()):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+        t = (C.c_uint * len(targets))(*targets)
+        check(lib().dk_circuit_add(self.ptr, len(targets),
+                                   _dp(m.view(np.float64).ravel()), t,
+                                   mask(controls), mask(anticontrols)))
+
+    def fuse(self, k):
+        n = C.c_size_t()
+        check(lib().dk_circuit_fuse(self.ptr, k, C.byref(n)))
+        return n.value
+
+    def block(self, i):
+        m = C.c_uint()
+        qs = (C.c_uint * 64)()
+        check(lib().dk_circuit_block(self.ptr, i, C.byref(m), qs, None))
+        dim = 1 << m.value
+        out = np.empty(2 * dim * dim)
+        check(lib().dk_circuit_block(self.ptr, i, None, None, _dp(out)))
+        return [qs[j] for j in range(m.value)], out.view(np.complex128).reshape(dim, dim)
+
+    def run(self, s, in_place=False):
+        p = C.c_void_p()
+        check(lib().dk_circuit_run(self.ptr, s.ptr, int(in_place), C.byref(p)))
+        return s if in_place else State(p.value)
 
 
 def synchronize():

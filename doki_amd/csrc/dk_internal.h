@@ -43,23 +43,32 @@ enum { DK_SCAL_NORM = 0, DK_SCAL_BYTES = 256 };
 
 struct dk_state {
 	unsigned nq;
+	int dtype;      // DK_C128 or DK_C64
 	u64 len;        // 2^nq amplitudes
 	char *base;     // device allocation: amplitudes then scalars
 	double *scal;   // device scalars
-	double2 *amp;   // device amplitudes (raw)
+	void *amp;      // device amplitudes (raw), double2 or float2
 	u64 bytes;      // bytes of the allocation
+	double2 *a128() const { return (double2 *)amp; }
+	float2 *a64() const { return (float2 *)amp; }
 };
+
+static inline size_t dk_amp_size(int dtype)
+{
+	return dtype == DK_C64 ? sizeof(float2) : sizeof(double2);
+}
 
 struct dk_gate {
 	unsigned k;
 	unsigned dim;       // 2^k
-	double2 *dmat;      // device copy, row-major
+	double2 *dmat;      // device FP64 copy, row-major (k > 1)
+	float2 *dmat32;     // device FP32 copy, row-major (k > 1)
 	double2 hmat[64 * 64];
 };
 
 // ---- library context ---------------------------------------------------------
 
-enum { DK_PARTIALS = 1 << 20 }; // doubles of reduction scratch
+enum { DK_PARTIALS = 1 << 20 }; // initial doubles of reduction scratch
 
 struct dk_ctx {
 	int device = -1;
@@ -75,6 +84,7 @@ struct dk_ctx {
 	// tunables
 	int apply_blocks = 0;   // 0 = auto
 	int time_apply = 0;     // record events[62]/[63] around the gate kernel
+	int dense_mfma = 1;     // dense k>=4 gates on MFMA (else VALU tiles)
 };
 
 enum { DK_PINNED = 1 << 16 };
@@ -82,15 +92,21 @@ enum { DK_PINNED = 1 << 16 };
 extern dk_ctx g_ctx;
 
 int dk_ensure_init();
-int dk_alloc_state(unsigned nq, dk_state **out);
+int dk_alloc_state(unsigned nq, int dtype, dk_state **out);
 void dk_release_state(dk_state *s);
+int dk_device_alloc(u64 bytes, void **out);
+void dk_device_free(void *p, u64 bytes);
 
-// ---- kernel launchers (dk_kernels.hip) ----------------------------------------
+// ---- kernel launchers (dk_kernels.hip, dk_dense.hip) --------------------------
 
-// Gate application; writes block partial sums of |out|^2 to g_ctx.partials and
-// finishes the norm of `out` on the device.
+// Gate application; block partial sums of |out|^2 finish into out's norm on
+// the device.
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm);
+
+// Dense k-qubit gate (4 <= k <= 6, no controls, n >= 12) on MFMA.
+int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
+			  const unsigned *targets, double **parts, u64 *nparts);
 
 // Shared partials buffer with room for `count` block partials plus the
 // chunked finishing pass; deterministic finishing sum of such partials.
@@ -100,14 +116,15 @@ int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt);
 // sum_{i: bit q of i = 1} |raw_i / norm|^2 -> *dst (device scalar).
 int dk_launch_prob(const dk_state *s, unsigned q, double *dst);
 int dk_launch_prob_all(const dk_state *s, double *dst /* device, nq */);
+int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out);
 
-// Scaled copy: dst = src * (1 / src.norm), dst.norm = 1.
+// Scaled copy (and type conversion): dst = src * (1 / src.norm), norm 1.
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst);
 
-// Measurement step: decide outcome from a device P1, write the norm of the
-// collapsed state, then collapse (with fused partials for the next qubit).
-int dk_launch_decide(const double *p1_src, int from_partials, int nparts,
-		     double roll, double *dec_slot, dk_state *next);
+// Measurement step: decide outcome from device partials, write the norm of
+// the collapsed state; collapse (with fused partials for the next qubit).
+int dk_launch_decide(const double *parts, int nparts, double roll,
+		     double *dec_slot, dk_state *next);
 int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 		       dk_state *o, int next_q, int *nparts_out);
 
@@ -122,3 +139,4 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 
 int dk_launch_fill_zero_state(dk_state *s);
 int dk_launch_set_norm(dk_state *s, double v);
+int dk_launch_widen(const dk_state *s, u64 first, u64 count, double2 *dst);

@@ -1,135 +1,73 @@
-// gfx950 kernels of the state-vector core.
+// gfx950 kernels of the state-vector core (everything except the MFMA dense
+// gate kernel, which lives in dk_dense.hip).
 //
-// Layout: a state of n qubits is 2^n complex128 amplitudes (double2, 16 B,
-// re then im) in one contiguous HBM allocation; index bit q is qubit q.
-// Every kernel reads a state through its normalisation constant
-// (value = raw * (1/norm)), as the reference's state_get does
+// Layout: a state of n qubits is 2^n amplitudes — complex128 (double2) or
+// complex64 (float2), re then im — in one contiguous HBM allocation; index
+// bit q is qubit q.  Every kernel reads a state through its normalisation
+// constant (value = raw * (1/norm)), as the reference's state_get does
 // (src/doki/qstate.h:86).  Reductions are FP64 and deterministic for a given
-// state size: fixed grids, per-block partials summed in block order by a
-// single-block finishing kernel.
+// state size: fixed grids, per-block partials summed in block order.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
-#include "dk_internal.h"
+#include "dk_device.h"
+
+using namespace dkd;
 
 namespace {
 
-constexpr int TPB = 256;          // threads per block (4 wave64)
-constexpr int WAVES = TPB / 64;
-
-__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
-
-// acc + m * v (complex, FMA form)
-__device__ __forceinline__ double2 cmac(double2 acc, double2 m, double2 v)
-{
-	acc.x = fma(m.x, v.x, acc.x);
-	acc.x = fma(-m.y, v.y, acc.x);
-	acc.y = fma(m.x, v.y, acc.y);
-	acc.y = fma(m.y, v.x, acc.y);
-	return acc;
-}
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
-{
-	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
-}
-
-// a * conj(b)
-__device__ __forceinline__ double2 cmulc(double2 a, double2 b)
-{
-	return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
-}
-
-__device__ __forceinline__ double2 cscale(double2 v, double s)
-{
-	return make_double2(v.x * s, v.y * s);
-}
-
-__device__ __forceinline__ double cabs2(double2 v)
-{
-	return fma(v.x, v.x, v.y * v.y);
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		v += __shfl_xor(v, o);
-	return v;
-}
-
-// Sum over the block; the result is valid in thread 0.  `red` holds WAVES
-// doubles of LDS.  Ends with the block synchronised.
-__device__ __forceinline__ double block_sum(double v, double *red)
-{
-	v = wave_sum(v);
-	if ((threadIdx.x & 63) == 0)
-		red[threadIdx.x >> 6] = v;
-	__syncthreads();
-	double t = 0.0;
-	if (threadIdx.x == 0) {
-#pragma unroll
-		for (int w = 0; w < WAVES; ++w)
-			t += red[w];
-	}
-	__syncthreads();
-	return t;
-}
-
-// Insert a zero bit at position p.
-__device__ __forceinline__ u64 ins0(u64 x, unsigned p)
-{
-	const u64 lo = (1ull << p) - 1;
-	return ((x & ~lo) << 1) | (x & lo);
-}
-
-// ---------------------------------------------------------------------------
-// One-qubit gates.  The 2x2 matrix travels by value in the kernel arguments.
-// ---------------------------------------------------------------------------
-struct Mat2 {
-	double2 m00, m01, m10, m11;
+template <typename T> struct Mat2 {
+	T m00, m01, m10, m11;
 };
 
-// Streaming stores that bypass cache allocation: the output is not re-read
-// by this kernel.
-__device__ __forceinline__ void st_stream(double2 *p, double2 v)
+template <typename T> Mat2<T> mat2_of(const dk_gate *g)
 {
-	__builtin_nontemporal_store(v.x, &p->x);
-	__builtin_nontemporal_store(v.y, &p->y);
+	Mat2<T> M;
+	T *dst[4] = {&M.m00, &M.m01, &M.m10, &M.m11};
+	for (int i = 0; i < 4; ++i) {
+		dst[i]->x = (decltype(dst[i]->x))g->hmat[i].x;
+		dst[i]->y = (decltype(dst[i]->y))g->hmat[i].y;
+	}
+	return M;
 }
 
-// Full-grid forms: one launch covers the state exactly once, one amplitude
-// per thread, blocks dispatched in address order so the amplitudes in flight
-// form a sliding window over HBM.  Each block leaves one partial |out|^2 sum.
+// ---------------------------------------------------------------------------
+// One-qubit gates.  Full-grid forms: one launch covers the state exactly
+// once, one amplitude per thread, blocks dispatched in address order so the
+// amplitudes in flight form a sliding window over HBM.  Each block leaves one
+// partial |out|^2 sum.
 //
 // t >= 6: a block is two waves.  Wave 0 reads the lower amplitudes of 64
-// consecutive pairs (one contiguous 1 KiB run), wave 1 the partner run 2^t
+// consecutive pairs (one contiguous run), wave 1 the partner run 2^t
 // amplitudes above; partners are exchanged through LDS.
+// ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(128) void k_apply1_pairs(
-	const double2 *in, double2 *out, const double *__restrict__ nrm,
-	Mat2 M, unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
 {
-	__shared__ double2 buf[128];
+	typedef typename real_of<T>::type R;
+	__shared__ T buf[128];
 	__shared__ double red[2];
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	const u64 bit = 1ull << t, lo = bit - 1;
 	const u64 p = (u64)blockIdx.x * 64 + (threadIdx.x & 63);
 	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
 	const bool upper = threadIdx.x >= 64;
 	const u64 my = upper ? (i0 | bit) : i0;
-	const double2 v = cscale(in[my], inv);
+	const T v = cscale(in[my], inv);
 	buf[threadIdx.x] = v;
 	__syncthreads();
-	const double2 w = buf[threadIdx.x ^ 64];
-	const double2 x = upper ? w : v, y = upper ? v : w;
-	double2 o = v;
+	const T w = buf[threadIdx.x ^ 64];
+	const T x = upper ? w : v, y = upper ? v : w;
+	T o = v;
 	if ((i0 & cm) == cm && (i0 & acm) == 0) {
-		o = upper ? cmac(cmac(cz(), M.m10, x), M.m11, y)
-			  : cmac(cmac(cz(), M.m00, x), M.m01, y);
+		o = upper ? cmac(cmac(czero<T>(), M.m10, x), M.m11, y)
+			  : cmac(cmac(czero<T>(), M.m00, x), M.m01, y);
 	}
 	st_stream(out + my, o);
-	const double s = wave_sum(cabs2(o));
+	const double s = wave_sum(cabs2d(o));
 	if ((threadIdx.x & 63) == 0)
 		red[threadIdx.x >> 6] = s;
 	__syncthreads();
@@ -139,68 +77,54 @@ __global__ __launch_bounds__(128) void k_apply1_pairs(
 
 // t < 6: one wave per block over 64 consecutive amplitudes; the partner
 // amplitude comes from lane ^ 2^t.
+template <typename T>
 __global__ __launch_bounds__(64) void k_apply1_lanes(
-	const double2 *in, double2 *out, const double *__restrict__ nrm,
-	Mat2 M, unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
 {
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	typedef typename real_of<T>::type R;
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
-	const double2 v = cscale(in[i], inv);
-	double2 w;
+	const T v = cscale(in[i], inv);
+	T w;
 	w.x = __shfl_xor(v.x, 1 << t);
 	w.y = __shfl_xor(v.y, 1 << t);
 	const bool upper = (i >> t) & 1;
-	const double2 x = upper ? w : v, y = upper ? v : w;
-	double2 o = v;
+	const T x = upper ? w : v, y = upper ? v : w;
+	T o = v;
 	if ((i & cm) == cm && (i & acm) == 0) {
-		o = upper ? cmac(cmac(cz(), M.m10, x), M.m11, y)
-			  : cmac(cmac(cz(), M.m00, x), M.m01, y);
+		o = upper ? cmac(cmac(czero<T>(), M.m10, x), M.m11, y)
+			  : cmac(cmac(czero<T>(), M.m00, x), M.m01, y);
 	}
 	st_stream(out + i, o);
-	const double s = wave_sum(cabs2(o));
+	const double s = wave_sum(cabs2d(o));
 	if (threadIdx.x == 0)
 		partials[blockIdx.x] = s;
 }
 
 // Small registers (fewer than 128 amplitudes): pairs, grid-stride, guarded.
-template <int U>
-__global__ __launch_bounds__(TPB) void k_apply1_pair(
-	const double2 *in, double2 *out, const double *__restrict__ nrm,
-	Mat2 M, unsigned t, u64 cm, u64 acm, u64 npairs,
-	double *__restrict__ partials)
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_apply1_small(
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, u64 npairs, double *__restrict__ partials)
 {
+	typedef typename real_of<T>::type R;
 	__shared__ double red[WAVES];
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	const u64 bit = 1ull << t, lo = bit - 1;
-	const u64 stride = (u64)gridDim.x * TPB * U;
 	double acc = 0.0;
-	for (u64 p0 = (u64)blockIdx.x * TPB * U + threadIdx.x; p0 < npairs;
-	     p0 += stride) {
-		double2 a[U], b[U];
-		u64 i0[U];
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const u64 p = p0 + (u64)u * TPB;
-			i0[u] = ((p & ~lo) << 1) | (p & lo);
-			if (p < npairs) {
-				a[u] = in[i0[u]];
-				b[u] = in[i0[u] | bit];
-			}
+	for (u64 p = (u64)blockIdx.x * TPB + threadIdx.x; p < npairs;
+	     p += (u64)gridDim.x * TPB) {
+		const u64 i0 = ((p & ~lo) << 1) | (p & lo);
+		const T x = cscale(in[i0], inv), y = cscale(in[i0 | bit], inv);
+		T o0 = x, o1 = y;
+		if ((i0 & cm) == cm && (i0 & acm) == 0) {
+			o0 = cmac(cmac(czero<T>(), M.m00, x), M.m01, y);
+			o1 = cmac(cmac(czero<T>(), M.m10, x), M.m11, y);
 		}
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			if (p0 + (u64)u * TPB >= npairs)
-				continue;
-			const double2 x = cscale(a[u], inv), y = cscale(b[u], inv);
-			double2 o0 = x, o1 = y;
-			if ((i0[u] & cm) == cm && (i0[u] & acm) == 0) {
-				o0 = cmac(cmac(cz(), M.m00, x), M.m01, y);
-				o1 = cmac(cmac(cz(), M.m10, x), M.m11, y);
-			}
-			out[i0[u]] = o0;
-			out[i0[u] | bit] = o1;
-			acc += cabs2(o0) + cabs2(o1);
-		}
+		out[i0] = o0;
+		out[i0 | bit] = o1;
+		acc += cabs2d(o0) + cabs2d(o1);
 	}
 	const double tot = block_sum(acc, red);
 	if (threadIdx.x == 0)
@@ -208,87 +132,67 @@ __global__ __launch_bounds__(TPB) void k_apply1_pair(
 }
 
 // ---------------------------------------------------------------------------
-// k-qubit gates (k = 1..6) through an LDS tile.  A tile is the 2^P amplitudes
-// (P = min(12, n)) whose index bits vary over the positions pos[0..P) — the
-// gate's targets plus the lowest non-target bits — with the other bits fixed
-// by the tile number.  Loads and stores walk the tile in slot order, so
-// consecutive lanes touch consecutive addresses whenever the low positions
-// are contiguous; the gate then works on groups inside LDS.
+// k-qubit gates (k = 1..6, with controls) through an LDS tile on the vector
+// ALUs.  Loads and stores walk the tile in slot order, so consecutive lanes
+// touch consecutive addresses whenever the low positions are contiguous.
 // ---------------------------------------------------------------------------
-constexpr int TILE_BITS = 12;
-constexpr int TILE = 1 << TILE_BITS;
-
-struct TileArgs {
-	u64 dep_lo[64];            // slot bits 0..5 -> index bits
-	u64 dep_hi[64];            // slot bits 6..11 -> index bits
-	unsigned short colslot[64];// gate column j -> slot offset of its targets
-	unsigned char pos[TILE_BITS]; // tile positions, ascending
-	unsigned char tslot[8];    // slot positions of the targets, ascending
-	int npos;
-};
-
-template <int K>
+template <typename T, int K>
 __global__ __launch_bounds__(TPB) void k_apply_tile(
-	const double2 *in, double2 *out, const double *__restrict__ nrm,
-	const double2 *__restrict__ M, TileArgs ta, u64 cm, u64 acm,
-	u64 ntiles, double *__restrict__ partials)
+	const T *in, T *out, const double *__restrict__ nrm,
+	const T *__restrict__ M, TileArgs ta, u64 cm, u64 acm, u64 ntiles,
+	double *__restrict__ partials)
 {
+	typedef typename real_of<T>::type R;
 	constexpr int D = 1 << K;
-	constexpr int R = D < 16 ? D : 16;   // rows per work item
-	constexpr int NRB = D / R;           // row blocks per group
+	constexpr int RR = D < 16 ? D : 16;  // rows per work item
+	constexpr int NRB = D / RR;          // row blocks per group
 	constexpr int OPT = TILE / TPB;      // outputs per thread (16)
-	constexpr int IPT = OPT / R;         // work items per thread
-	__shared__ double2 lds[TILE];
+	constexpr int IPT = OPT / RR;        // work items per thread
+	__shared__ T lds[TILE];
 	__shared__ double red[WAVES];
 
 	const int tsz = 1 << ta.npos;
 	const int G = tsz >> K;
 	const int nwork = G * NRB;
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	double acc = 0.0;
 
 	for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		u64 base = tile;
-		for (int b = 0; b < ta.npos; ++b)
-			base = ins0(base, ta.pos[b]);
-		for (int s = threadIdx.x; s < tsz; s += TPB) {
-			const u64 gi = base | ta.dep_lo[s & 63] | ta.dep_hi[s >> 6];
-			lds[s] = cscale(in[gi], inv);
-		}
+		const u64 base = tile_base(tile, ta);
+		for (int s = threadIdx.x; s < tsz; s += TPB)
+			lds[s] = cscale(in[tile_index(base, s, ta)], inv);
 		__syncthreads();
 
-		double2 res[IPT][R];
+		T res[IPT][RR];
 		int gsl[IPT], rbs[IPT];
 #pragma unroll
 		for (int it = 0; it < IPT; ++it) {
 			const int w = threadIdx.x + it * TPB;
 			gsl[it] = -1;
+			rbs[it] = 0;
 			if (w >= nwork)
 				continue;
 			const int g = w % G;
 			const int rb = NRB == 1 ? 0 : w / G;
 			rbs[it] = rb;
-			int gs = g;
-#pragma unroll
-			for (int b = 0; b < K; ++b)
-				gs = (int)ins0((u64)gs, ta.tslot[b]);
+			const int gs = group_slot<K>(g, ta);
 			gsl[it] = gs;
-			const u64 gi = base | ta.dep_lo[gs & 63] | ta.dep_hi[gs >> 6];
+			const u64 gi = tile_index(base, gs, ta);
 			if ((gi & cm) == cm && (gi & acm) == 0) {
 #pragma unroll
-				for (int r = 0; r < R; ++r)
-					res[it][r] = cz();
+				for (int r = 0; r < RR; ++r)
+					res[it][r] = czero<T>();
 				for (int j = 0; j < D; ++j) {
-					const double2 x = lds[gs | ta.colslot[j]];
-					const double2 *mrow = M + (size_t)(rb * R) * D + j;
+					const T x = lds[gs | ta.colslot[j]];
+					const T *mrow = M + (size_t)(rb * RR) * D + j;
 #pragma unroll
-					for (int r = 0; r < R; ++r)
+					for (int r = 0; r < RR; ++r)
 						res[it][r] = cmac(res[it][r], mrow[(size_t)r * D], x);
 				}
 			} else {
 #pragma unroll
-				for (int r = 0; r < R; ++r)
-					res[it][r] = lds[gs | ta.colslot[rb * R + r]];
+				for (int r = 0; r < RR; ++r)
+					res[it][r] = lds[gs | ta.colslot[rb * RR + r]];
 			}
 		}
 		__syncthreads();
@@ -297,15 +201,14 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 			if (gsl[it] < 0)
 				continue;
 #pragma unroll
-			for (int r = 0; r < R; ++r)
-				lds[gsl[it] | ta.colslot[rbs[it] * R + r]] = res[it][r];
+			for (int r = 0; r < RR; ++r)
+				lds[gsl[it] | ta.colslot[rbs[it] * RR + r]] = res[it][r];
 		}
 		__syncthreads();
 		for (int s = threadIdx.x; s < tsz; s += TPB) {
-			const u64 gi = base | ta.dep_lo[s & 63] | ta.dep_hi[s >> 6];
-			const double2 o = lds[s];
-			out[gi] = o;
-			acc += cabs2(o);
+			const T o = lds[s];
+			out[tile_index(base, s, ta)] = o;
+			acc += cabs2d(o);
 		}
 		__syncthreads();
 	}
@@ -315,7 +218,7 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 }
 
 // ---------------------------------------------------------------------------
-// Single-block finishing kernels.
+// Reductions.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_finish_sum(
 	const double *__restrict__ partials, int count, double *dst, int take_sqrt)
@@ -329,8 +232,7 @@ __global__ __launch_bounds__(TPB) void k_finish_sum(
 		dst[0] = take_sqrt ? sqrt(s) : s;
 }
 
-// First level of a long reduction: out[b] = sum of in[b*4096 .. b*4096+4095]
-// (in index order within each thread, then the block tree).
+// First level of a long reduction: out[b] = sum of in[b*4096 .. b*4096+4095].
 constexpr int RCHUNK = 4096;
 __global__ __launch_bounds__(TPB) void k_reduce_chunks(
 	const double *__restrict__ in, u64 count, double *__restrict__ out)
@@ -348,46 +250,38 @@ __global__ __launch_bounds__(TPB) void k_reduce_chunks(
 		out[blockIdx.x] = s;
 }
 
-// Sum `cols` interleaved columns: dst[c] = sum_b partials[b * cols + c].
-__global__ __launch_bounds__(64) void k_finish_cols(
+// One block per column: dst[c] = sum_b partials[b * cols + c].
+__global__ __launch_bounds__(TPB) void k_finish_cols(
 	const double *__restrict__ partials, int rows, int cols, double *dst)
 {
-	const int c = threadIdx.x;
-	if (c >= cols)
-		return;
+	__shared__ double red[WAVES];
+	const int c = blockIdx.x;
 	double s = 0.0;
-	for (int b = 0; b < rows; ++b)
+	for (int b = threadIdx.x; b < rows; b += TPB)
 		s += partials[(size_t)b * cols + c];
-	dst[c] = s;
+	s = block_sum(s, red);
+	if (threadIdx.x == 0)
+		dst[c] = s;
 }
 
 // ---------------------------------------------------------------------------
 // Probabilities.
 // ---------------------------------------------------------------------------
-template <int U>
+// Half-index form: block of 64 threads per 64 indices with bit q set.
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_prob(
-	const double2 *__restrict__ in, const double *__restrict__ nrm,
-	unsigned q, u64 half, double *__restrict__ partials)
+	const T *__restrict__ in, const double *__restrict__ nrm, unsigned q,
+	u64 half, double *__restrict__ partials)
 {
 	__shared__ double red[WAVES];
 	const double inv = 1.0 / nrm[DK_SCAL_NORM];
 	const u64 bit = 1ull << q, lo = bit - 1;
-	const u64 stride = (u64)gridDim.x * TPB * U;
 	double acc = 0.0;
-	for (u64 j0 = (u64)blockIdx.x * TPB * U + threadIdx.x; j0 < half;
-	     j0 += stride) {
-		double2 a[U];
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const u64 j = j0 + (u64)u * TPB;
-			if (j < half)
-				a[u] = in[((j & ~lo) << 1) | bit | (j & lo)];
-			else
-				a[u] = cz();
-		}
-#pragma unroll
-		for (int u = 0; u < U; ++u)
-			acc += cabs2(cscale(a[u], inv));
+	for (u64 j = (u64)blockIdx.x * TPB + threadIdx.x; j < half;
+	     j += (u64)gridDim.x * TPB) {
+		const T a = in[((j & ~lo) << 1) | bit | (j & lo)];
+		const double x = a.x * inv, y = a.y * inv;
+		acc += fma(x, x, y * y);
 	}
 	const double tot = block_sum(acc, red);
 	if (threadIdx.x == 0)
@@ -396,14 +290,13 @@ __global__ __launch_bounds__(TPB) void k_prob(
 
 // All single-qubit probabilities in one pass.  A wave reads chunks of 64*U
 // consecutive amplitudes: index bits 0..5 are the lane, bits 6..6+LU-1 the
-// unroll slot, higher bits the chunk number.  Lane and unroll bits are
-// resolved at the end / per chunk in registers; chunk bits add the chunk sum.
+// unroll slot, higher bits the chunk number.
 constexpr int PQ = 40; // qubits covered by the one-pass kernel
 
-template <int LU>
+template <typename T, int LU>
 __global__ __launch_bounds__(TPB) void k_prob_all(
-	const double2 *__restrict__ in, const double *__restrict__ nrm,
-	unsigned nq, u64 len, double *__restrict__ partials)
+	const T *__restrict__ in, const double *__restrict__ nrm, u64 len,
+	double *__restrict__ partials)
 {
 	constexpr int U = 1 << LU;
 	constexpr int NH = PQ - 6 - LU;
@@ -422,10 +315,13 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 		acch[b] = 0.0;
 	for (u64 c = wid; c < nch; c += nw) {
 		double x[U];
-		const double2 *p = in + (c << (6 + LU)) + lane;
+		const T *p = in + (c << (6 + LU)) + lane;
 #pragma unroll
-		for (int u = 0; u < U; ++u)
-			x[u] = cabs2(cscale(p[(u64)u * 64], inv));
+		for (int u = 0; u < U; ++u) {
+			const T a = p[(u64)u * 64];
+			const double re = a.x * inv, im = a.y * inv;
+			x[u] = fma(re, re, im * im);
+		}
 		double s = 0.0;
 #pragma unroll
 		for (int u = 0; u < U; ++u)
@@ -444,7 +340,6 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 		for (int b = 0; b < NH; ++b)
 			acch[b] += ((c >> b) & 1) ? s : 0.0;
 	}
-	// per-qubit sums of this lane, reduced over the wave then the block
 	const int w = threadIdx.x >> 6;
 #pragma unroll
 	for (int q = 0; q < PQ; ++q) {
@@ -467,7 +362,6 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 			s += red[ww][threadIdx.x];
 		partials[(size_t)blockIdx.x * PQ + threadIdx.x] = s;
 	}
-	(void)nq;
 }
 
 // ---------------------------------------------------------------------------
@@ -497,38 +391,27 @@ __global__ __launch_bounds__(TPB) void k_decide(
 
 // o[j] = s[j with bit q inserted = outcome] / s.norm; when next_q >= 0 also
 // the block partials of sum_{j: bit next_q set} |o_j / o.norm|^2.
-template <int U>
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_collapse(
-	const double2 *__restrict__ in, const double *__restrict__ nrm_in,
-	unsigned q, const double *__restrict__ dec, double2 *__restrict__ out,
+	const T *__restrict__ in, const double *__restrict__ nrm_in, unsigned q,
+	const double *__restrict__ dec, T *__restrict__ out,
 	const double *__restrict__ nrm_out, int next_q, u64 olen,
 	double *__restrict__ partials)
 {
+	typedef typename real_of<T>::type R;
 	__shared__ double red[WAVES];
-	const double inv = 1.0 / nrm_in[DK_SCAL_NORM];
+	const R inv = (R)(1.0 / nrm_in[DK_SCAL_NORM]);
 	const double inv_o = 1.0 / nrm_out[DK_SCAL_NORM];
 	const u64 bit = 1ull << q, lo = bit - 1;
 	const u64 off = dec[0] != 0.0 ? bit : 0;
-	const u64 stride = (u64)gridDim.x * TPB * U;
 	double acc = 0.0;
-	for (u64 j0 = (u64)blockIdx.x * TPB * U + threadIdx.x; j0 < olen;
-	     j0 += stride) {
-		double2 a[U];
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const u64 j = j0 + (u64)u * TPB;
-			if (j < olen)
-				a[u] = in[((j & ~lo) << 1) | off | (j & lo)];
-		}
-#pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const u64 j = j0 + (u64)u * TPB;
-			if (j >= olen)
-				continue;
-			const double2 o = cscale(a[u], inv);
-			out[j] = o;
-			if (next_q >= 0 && ((j >> next_q) & 1))
-				acc += cabs2(cscale(o, inv_o));
+	for (u64 j = (u64)blockIdx.x * TPB + threadIdx.x; j < olen;
+	     j += (u64)gridDim.x * TPB) {
+		const T o = cscale(in[((j & ~lo) << 1) | off | (j & lo)], inv);
+		out[j] = o;
+		if (next_q >= 0 && ((j >> next_q) & 1)) {
+			const double x = o.x * inv_o, y = o.y * inv_o;
+			acc += fma(x, x, y * y);
 		}
 	}
 	const double tot = block_sum(acc, red);
@@ -537,44 +420,65 @@ __global__ __launch_bounds__(TPB) void k_collapse(
 }
 
 // ---------------------------------------------------------------------------
-// Copies, join, fills.
+// Copies, join, fills, read-out.
 // ---------------------------------------------------------------------------
+template <typename TI, typename TO>
 __global__ __launch_bounds__(TPB) void k_normalised_copy(
-	const double2 *__restrict__ in, const double *__restrict__ nrm,
-	double2 *__restrict__ out, double *out_scal, u64 len)
+	const TI *__restrict__ in, const double *__restrict__ nrm,
+	TO *__restrict__ out, double *out_scal, u64 len)
 {
 	const double inv = 1.0 / nrm[DK_SCAL_NORM];
-	const u64 stride = (u64)gridDim.x * TPB;
-	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len; i += stride)
-		out[i] = cscale(in[i], inv);
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len;
+	     i += (u64)gridDim.x * TPB) {
+		const TI a = in[i];
+		TO o;
+		o.x = (decltype(o.x))(a.x * inv);
+		o.y = (decltype(o.y))(a.y * inv);
+		out[i] = o;
+	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		out_scal[DK_SCAL_NORM] = 1.0;
 }
 
-__global__ __launch_bounds__(TPB) void k_join(
-	const double2 *__restrict__ a, const double *__restrict__ na,
-	const double2 *__restrict__ b, const double *__restrict__ nb,
-	unsigned bbits, u64 olen, double2 *__restrict__ o, double *o_scal)
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_widen(const T *__restrict__ in,
+					       u64 count, double2 *__restrict__ out)
 {
-	const double ia = 1.0 / na[DK_SCAL_NORM], ib = 1.0 / nb[DK_SCAL_NORM];
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < count;
+	     i += (u64)gridDim.x * TPB)
+		out[i] = make_double2(in[i].x, in[i].y);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_join(
+	const T *__restrict__ a, const double *__restrict__ na,
+	const T *__restrict__ b, const double *__restrict__ nb, unsigned bbits,
+	u64 olen, T *__restrict__ o, double *o_scal)
+{
+	typedef typename real_of<T>::type R;
+	const R ia = (R)(1.0 / na[DK_SCAL_NORM]), ib = (R)(1.0 / nb[DK_SCAL_NORM]);
 	const u64 bmask = (1ull << bbits) - 1;
-	const u64 stride = (u64)gridDim.x * TPB;
 	for (u64 idx = (u64)blockIdx.x * TPB + threadIdx.x; idx < olen;
-	     idx += stride) {
-		const double2 x = cscale(a[idx >> bbits], ia);
-		const double2 y = cscale(b[idx & bmask], ib);
+	     idx += (u64)gridDim.x * TPB) {
+		const T x = cscale(a[idx >> bbits], ia);
+		const T y = cscale(b[idx & bmask], ib);
 		o[idx] = cmul(x, y);
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		o_scal[DK_SCAL_NORM] = 1.0;
 }
 
-__global__ __launch_bounds__(TPB) void k_fill_zero_state(
-	double2 *__restrict__ amp, double *scal, u64 len)
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_fill_zero_state(T *__restrict__ amp,
+							 double *scal, u64 len)
 {
-	const u64 stride = (u64)gridDim.x * TPB;
-	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len; i += stride)
-		amp[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len;
+	     i += (u64)gridDim.x * TPB) {
+		T v = czero<T>();
+		if (i == 0)
+			v.x = 1;
+		amp[i] = v;
+	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		scal[DK_SCAL_NORM] = 1.0;
 }
@@ -592,17 +496,16 @@ __global__ void k_set_u64(unsigned long long *dst, u64 v)
 }
 
 // Smallest index >= start holding a nonzero raw amplitude, via atomicMin.
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_first_nonzero(
-	const double2 *__restrict__ in, u64 start, u64 len,
-	unsigned long long *dst)
+	const T *__restrict__ in, u64 start, u64 len, unsigned long long *dst)
 {
 	__shared__ unsigned long long red[WAVES];
 	u64 best = len;
-	const u64 stride = (u64)gridDim.x * TPB;
 	for (u64 i = start + (u64)blockIdx.x * TPB + threadIdx.x; i < len;
-	     i += stride) {
-		const double2 v = in[i];
-		if (v.x != 0.0 || v.y != 0.0) {
+	     i += (u64)gridDim.x * TPB) {
+		const T v = in[i];
+		if (v.x != 0 || v.y != 0) {
 			best = i;
 			break;
 		}
@@ -626,9 +529,9 @@ __global__ __launch_bounds__(TPB) void k_first_nonzero(
 
 // ---------------------------------------------------------------------------
 // Reduced density matrix: rho[i][j] = sum_t A[i][t] conj(A[j][t]) with
-// A[i][t] = amp[keep(i) | traced(t)] / norm.  16x16 output tiles, t staged
-// through LDS 16 at a time; each block covers one t-chunk and writes its
-// partial tile; k_density_sum adds the chunks in order.
+// A[i][t] = amp[keep(i) | traced(t)] / norm, accumulated in FP64.  16x16
+// output tiles, t staged through LDS 16 at a time; each block covers one
+// t-chunk and writes its partial tile; k_density_sum adds chunks in order.
 // ---------------------------------------------------------------------------
 struct DensArgs {
 	u64 keep_lo[64];    // kept index bits 0..5 -> state index bits
@@ -650,9 +553,10 @@ __device__ __forceinline__ u64 keep_index(u64 i, const DensArgs &da)
 	return da.keep_lo[i & 63] | da.keep_hi[(i >> 6) & 63];
 }
 
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_density_tile(
-	const double2 *__restrict__ in, const double *__restrict__ nrm,
-	DensArgs da, u64 tchunk, int tiles_per_row, double2 *__restrict__ part)
+	const T *__restrict__ in, const double *__restrict__ nrm, DensArgs da,
+	u64 tchunk, int tiles_per_row, double2 *__restrict__ part)
 {
 	__shared__ double2 ai[16][17];
 	__shared__ double2 aj[16][17];
@@ -662,20 +566,23 @@ __global__ __launch_bounds__(TPB) void k_density_tile(
 	const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
 	const u64 gi = (u64)ti * 16 + r, gj = (u64)tj * 16 + c;
 	const u64 t0 = (u64)blockIdx.y * tchunk;
-	const u64 T = 1ull << da.m;
-	const u64 t1 = t0 + tchunk < T ? t0 + tchunk : T;
-	double2 acc = cz();
+	const u64 T_ = 1ull << da.m;
+	const u64 t1 = t0 + tchunk < T_ ? t0 + tchunk : T_;
+	double2 acc = make_double2(0.0, 0.0);
 	for (u64 tb = t0; tb < t1; tb += 16) {
-		// stage A[ti*16 + r][tb + c] and A[tj*16 + r][tb + c]
 		const u64 t = tb + c;
 		const u64 ri = (u64)ti * 16 + r, rj = (u64)tj * 16 + r;
-		double2 vi = cz(), vj = cz();
+		double2 vi = make_double2(0.0, 0.0), vj = vi;
 		if (t < t1) {
 			const u64 tix = traced_index(t, da);
-			if (ri < dim)
-				vi = cscale(in[keep_index(ri, da) | tix], inv);
-			if (rj < dim)
-				vj = cscale(in[keep_index(rj, da) | tix], inv);
+			if (ri < dim) {
+				const T a = in[keep_index(ri, da) | tix];
+				vi = make_double2(a.x * inv, a.y * inv);
+			}
+			if (rj < dim) {
+				const T a = in[keep_index(rj, da) | tix];
+				vj = make_double2(a.x * inv, a.y * inv);
+			}
 		}
 		ai[r][c] = vi;
 		aj[r][c] = vj;
@@ -698,9 +605,9 @@ __global__ __launch_bounds__(TPB) void k_density_sum(
 	const double2 *__restrict__ part, int nchunks, u64 count,
 	double2 *__restrict__ rho)
 {
-	const u64 stride = (u64)gridDim.x * TPB;
-	for (u64 e = (u64)blockIdx.x * TPB + threadIdx.x; e < count; e += stride) {
-		double2 s = cz();
+	for (u64 e = (u64)blockIdx.x * TPB + threadIdx.x; e < count;
+	     e += (u64)gridDim.x * TPB) {
+		double2 s = make_double2(0.0, 0.0);
 		for (int k = 0; k < nchunks; ++k) {
 			const double2 v = part[(size_t)k * count + e];
 			s.x += v.x;
@@ -711,21 +618,24 @@ __global__ __launch_bounds__(TPB) void k_density_sum(
 }
 
 // One block per requested element.
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_density_elements(
-	const double2 *__restrict__ in, const double *__restrict__ nrm,
-	DensArgs da, const u64 *__restrict__ rows, const u64 *__restrict__ cols,
+	const T *__restrict__ in, const double *__restrict__ nrm, DensArgs da,
+	const u64 *__restrict__ rows, const u64 *__restrict__ cols,
 	double2 *__restrict__ out)
 {
 	__shared__ double red[WAVES];
 	const double inv = 1.0 / nrm[DK_SCAL_NORM];
 	const u64 ki = keep_index(rows[blockIdx.x], da);
 	const u64 kj = keep_index(cols[blockIdx.x], da);
-	const u64 T = 1ull << da.m;
-	double2 acc = cz();
-	for (u64 t = threadIdx.x; t < T; t += TPB) {
+	const u64 T_ = 1ull << da.m;
+	double2 acc = make_double2(0.0, 0.0);
+	for (u64 t = threadIdx.x; t < T_; t += TPB) {
 		const u64 tix = traced_index(t, da);
-		acc = cmac(acc, cscale(in[ki | tix], inv),
-			   make_double2(in[kj | tix].x * inv, -in[kj | tix].y * inv));
+		const T a = in[ki | tix], b = in[kj | tix];
+		const double2 x = make_double2(a.x * inv, a.y * inv);
+		const double2 y = make_double2(b.x * inv, -b.y * inv);
+		acc = cmac(acc, x, y);
 	}
 	const double re = block_sum(acc.x, red);
 	const double im = block_sum(acc.y, red);
@@ -736,9 +646,9 @@ __global__ __launch_bounds__(TPB) void k_density_elements(
 // ---------------------------------------------------------------------------
 // Host-side launch helpers
 // ---------------------------------------------------------------------------
-unsigned grid_for(u64 units_per_block_total, u64 per_block)
+unsigned grid_for(u64 units, u64 per_block)
 {
-	u64 b = (units_per_block_total + per_block - 1) / per_block;
+	u64 b = (units + per_block - 1) / per_block;
 	u64 cap = g_ctx.apply_blocks > 0 ? (u64)g_ctx.apply_blocks
 					 : (u64)std::max(1, g_ctx.num_cus) * 8;
 	// keep the grid a power of two so grid strides divide 2^n sizes
@@ -749,101 +659,49 @@ unsigned grid_for(u64 units_per_block_total, u64 per_block)
 	return (unsigned)std::max<u64>(1, b);
 }
 
-} // namespace
-
-// ===========================================================================
-int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
-		    const unsigned *targets, u64 cm, u64 acm)
+template <typename T>
+int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
+		       const unsigned *targets, u64 cm, u64 acm, double *parts,
+		       u64 *nblocks_out)
 {
 	hipStream_t st = g_ctx.stream;
+	const T *src = (const T *)in->amp;
+	T *dst = (T *)out->amp;
 	u64 nblocks = 0;
-	const unsigned n = in->nq;
-	// one partial per block of the largest grid below
-	double *parts = nullptr;
-	int rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
-	if (rc)
-		return rc;
-	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(g_ctx.events[62], st));
 	if (g->k == 1) {
-		Mat2 M{g->hmat[0], g->hmat[1], g->hmat[2], g->hmat[3]};
+		const Mat2<T> M = mat2_of<T>(g);
 		const unsigned t = targets[0];
-		constexpr int U = 4;
 		if (in->len >= 128 && t >= 6) {
 			nblocks = in->len / 128;
-			hipLaunchKernelGGL(k_apply1_pairs, dim3((unsigned)nblocks), dim3(128),
-					   0, st, in->amp, out->amp, in->scal, M, t, cm,
+			hipLaunchKernelGGL(k_apply1_pairs<T>, dim3((unsigned)nblocks),
+					   dim3(128), 0, st, src, dst, in->scal, M, t, cm,
 					   acm, parts);
 		} else if (in->len >= 128) {
 			nblocks = in->len / 64;
-			hipLaunchKernelGGL(k_apply1_lanes, dim3((unsigned)nblocks), dim3(64),
-					   0, st, in->amp, out->amp, in->scal, M, t, cm,
+			hipLaunchKernelGGL(k_apply1_lanes<T>, dim3((unsigned)nblocks),
+					   dim3(64), 0, st, src, dst, in->scal, M, t, cm,
 					   acm, parts);
 		} else {
-			const u64 np = in->len >> 1;
-			nblocks = grid_for(np, (u64)TPB * U);
-			hipLaunchKernelGGL(k_apply1_pair<U>, dim3(nblocks), dim3(TPB), 0,
-					   st, in->amp, out->amp, in->scal, M, t, cm,
-					   acm, np, parts);
+			nblocks = 1;
+			hipLaunchKernelGGL(k_apply1_small<T>, dim3(1), dim3(TPB), 0, st,
+					   src, dst, in->scal, M, t, cm, acm, in->len >> 1,
+					   parts);
 		}
 	} else {
 		const unsigned k = g->k;
 		TileArgs ta;
-		memset(&ta, 0, sizeof(ta));
-		const int npos = (int)std::min<unsigned>(TILE_BITS, n);
-		// positions: targets plus the lowest non-target bits
-		bool is_t[64] = {false};
-		for (unsigned b = 0; b < k; ++b)
-			is_t[targets[b]] = true;
-		int cnt = 0, extra = npos - (int)k;
-		for (unsigned p = 0; p < n && cnt < npos; ++p) {
-			if (is_t[p]) {
-				ta.pos[cnt++] = (unsigned char)p;
-			} else if (extra > 0) {
-				ta.pos[cnt++] = (unsigned char)p;
-				--extra;
-			}
-		}
-		ta.npos = npos;
-		int slot_of[64];
-		for (int b = 0; b < npos; ++b)
-			slot_of[ta.pos[b]] = b;
-		for (int s = 0; s < 64; ++s) {
-			u64 lo = 0, hi = 0;
-			for (int b = 0; b < 6; ++b) {
-				if (!((s >> b) & 1))
-					continue;
-				if (b < npos)
-					lo |= 1ull << ta.pos[b];
-				if (6 + b < npos)
-					hi |= 1ull << ta.pos[6 + b];
-			}
-			ta.dep_lo[s] = lo;
-			ta.dep_hi[s] = hi;
-		}
-		for (unsigned j = 0; j < (1u << k); ++j) {
-			unsigned off = 0;
-			for (unsigned b = 0; b < k; ++b)
-				if ((j >> b) & 1)
-					off |= 1u << slot_of[targets[b]];
-			ta.colslot[j] = (unsigned short)off;
-		}
-		int ts[8];
-		for (unsigned b = 0; b < k; ++b)
-			ts[b] = slot_of[targets[b]];
-		std::sort(ts, ts + k);
-		for (unsigned b = 0; b < k; ++b)
-			ta.tslot[b] = (unsigned char)ts[b];
-		const u64 ntiles = in->len >> npos;
+		dk_tile_args(in->nq, k, targets, ta);
+		const u64 ntiles = in->len >> ta.npos;
 		nblocks = grid_for(ntiles, 1);
+		const T *M = std::is_same<T, double2>::value ? (const T *)g->dmat
+							      : (const T *)g->dmat32;
 #define DK_TILE_CASE(KK)                                                    \
 	case KK:                                                            \
-		hipLaunchKernelGGL(k_apply_tile<KK>, dim3((unsigned)nblocks),   \
-				   dim3(TPB), 0, st, in->amp, out->amp, in->scal, \
-				   g->dmat, ta, cm, acm, ntiles, parts);    \
+		hipLaunchKernelGGL((k_apply_tile<T, KK>), dim3((unsigned)nblocks), \
+				   dim3(TPB), 0, st, src, dst, in->scal, M, ta, \
+				   cm, acm, ntiles, parts);                 \
 		break;
 		switch (k) {
-			DK_TILE_CASE(1)
 			DK_TILE_CASE(2)
 			DK_TILE_CASE(3)
 			DK_TILE_CASE(4)
@@ -856,6 +714,86 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 #undef DK_TILE_CASE
 	}
 	DK_LAUNCHED();
+	*nblocks_out = nblocks;
+	return DK_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+
+void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta)
+{
+	memset(&ta, 0, sizeof(ta));
+	const int npos = (int)std::min<unsigned>(TILE_BITS, n);
+	bool is_t[64] = {false};
+	for (unsigned b = 0; b < k; ++b)
+		is_t[targets[b]] = true;
+	int cnt = 0, extra = npos - (int)k;
+	for (unsigned p = 0; p < n && cnt < npos; ++p) {
+		if (is_t[p]) {
+			ta.pos[cnt++] = (unsigned char)p;
+		} else if (extra > 0) {
+			ta.pos[cnt++] = (unsigned char)p;
+			--extra;
+		}
+	}
+	ta.npos = npos;
+	int slot_of[64] = {0};
+	for (int b = 0; b < npos; ++b)
+		slot_of[ta.pos[b]] = b;
+	for (int s = 0; s < 64; ++s) {
+		u64 lo = 0, hi = 0;
+		for (int b = 0; b < 6; ++b) {
+			if (!((s >> b) & 1))
+				continue;
+			if (b < npos)
+				lo |= 1ull << ta.pos[b];
+			if (6 + b < npos)
+				hi |= 1ull << ta.pos[6 + b];
+		}
+		ta.dep_lo[s] = lo;
+		ta.dep_hi[s] = hi;
+	}
+	for (unsigned j = 0; j < (1u << k); ++j) {
+		unsigned off = 0;
+		for (unsigned b = 0; b < k; ++b)
+			if ((j >> b) & 1)
+				off |= 1u << slot_of[targets[b]];
+		ta.colslot[j] = (unsigned short)off;
+	}
+	int ts[8] = {0};
+	for (unsigned b = 0; b < k; ++b)
+		ts[b] = slot_of[targets[b]];
+	std::sort(ts, ts + k);
+	for (unsigned b = 0; b < k; ++b)
+		ta.tslot[b] = (unsigned char)ts[b];
+}
+
+int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
+		    const unsigned *targets, u64 cm, u64 acm)
+{
+	hipStream_t st = g_ctx.stream;
+	double *parts = nullptr;
+	u64 nblocks = 0;
+	int rc;
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(g_ctx.events[62], st));
+	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
+	    in->nq >= (unsigned)TILE_BITS) {
+		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
+	} else {
+		rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
+		if (rc)
+			return rc;
+		rc = in->dtype == DK_C64
+			     ? launch_apply_typed<float2>(in, out, g, targets, cm, acm,
+							  parts, &nblocks)
+			     : launch_apply_typed<double2>(in, out, g, targets, cm, acm,
+							   parts, &nblocks);
+	}
+	if (rc)
+		return rc;
 	if (g_ctx.time_apply)
 		DK_HIP(hipEventRecord(g_ctx.events[63], st));
 	return dk_finish_partials(parts, nblocks, out->scal + DK_SCAL_NORM, 1);
@@ -900,16 +838,37 @@ int dk_partials_reserve(u64 count, double **out)
 	return DK_OK;
 }
 
+#define DK_BY_TYPE(s, CALL)                                                  \
+	do {                                                                 \
+		if ((s)->dtype == DK_C64) {                                  \
+			CALL(float2);                                        \
+		} else {                                                     \
+			CALL(double2);                                       \
+		}                                                            \
+	} while (0)
+
+int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out)
+{
+	const u64 half = s->len >> 1;
+	const unsigned nb = grid_for(half, (u64)TPB * 4);
+#define L(T)                                                                \
+	hipLaunchKernelGGL(k_prob<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream,  \
+			    (const T *)s->amp, s->scal, q, half, g_ctx.partials)
+	DK_BY_TYPE(s, L);
+#undef L
+	DK_LAUNCHED();
+	*nparts_out = (int)nb;
+	return DK_OK;
+}
+
 int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
 {
-	constexpr int U = 4;
-	const u64 half = s->len >> 1;
-	const unsigned nb = grid_for(half, (u64)TPB * U);
-	hipLaunchKernelGGL(k_prob<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
-			   s->amp, s->scal, q, half, g_ctx.partials);
-	DK_LAUNCHED();
+	int nb = 0;
+	int rc = dk_launch_prob_partials(s, q, &nb);
+	if (rc)
+		return rc;
 	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, g_ctx.stream,
-			   g_ctx.partials, (int)nb, dst, 0);
+			   g_ctx.partials, nb, dst, 0);
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -926,11 +885,15 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 		return DK_OK;
 	}
 	const u64 nch = s->len >> (6 + LU);
-	unsigned nb = grid_for(nch, WAVES);
-	hipLaunchKernelGGL(k_prob_all<LU>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
-			   s->amp, s->scal, s->nq, s->len, g_ctx.partials);
+	const unsigned nb = grid_for(nch, WAVES);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL((k_prob_all<T, LU>), dim3(nb), dim3(TPB), 0,        \
+			    g_ctx.stream, (const T *)s->amp, s->scal, s->len,       \
+			    g_ctx.partials)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
-	hipLaunchKernelGGL(k_finish_cols, dim3(1), dim3(64), 0, g_ctx.stream,
+	hipLaunchKernelGGL(k_finish_cols, dim3(s->nq), dim3(TPB), 0, g_ctx.stream,
 			   g_ctx.partials, (int)nb, PQ, dst);
 	DK_LAUNCHED();
 	return DK_OK;
@@ -939,20 +902,40 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
 {
 	const unsigned nb = grid_for(src->len, TPB * 4);
-	hipLaunchKernelGGL(k_normalised_copy, dim3(nb), dim3(TPB), 0,
-			   g_ctx.stream, src->amp, src->scal, dst->amp, dst->scal,
-			   src->len);
+	hipStream_t st = g_ctx.stream;
+	if (src->dtype == DK_C128 && dst->dtype == DK_C128)
+		hipLaunchKernelGGL((k_normalised_copy<double2, double2>), dim3(nb),
+				   dim3(TPB), 0, st, src->a128(), src->scal,
+				   dst->a128(), dst->scal, src->len);
+	else if (src->dtype == DK_C128)
+		hipLaunchKernelGGL((k_normalised_copy<double2, float2>), dim3(nb),
+				   dim3(TPB), 0, st, src->a128(), src->scal,
+				   dst->a64(), dst->scal, src->len);
+	else if (dst->dtype == DK_C128)
+		hipLaunchKernelGGL((k_normalised_copy<float2, double2>), dim3(nb),
+				   dim3(TPB), 0, st, src->a64(), src->scal,
+				   dst->a128(), dst->scal, src->len);
+	else
+		hipLaunchKernelGGL((k_normalised_copy<float2, float2>), dim3(nb),
+				   dim3(TPB), 0, st, src->a64(), src->scal,
+				   dst->a64(), dst->scal, src->len);
 	DK_LAUNCHED();
 	return DK_OK;
 }
 
-// The outcome's P1 is either a finished device scalar (from_partials = 0,
-// p1_src points at it, nparts = 1) or a set of block partials to be summed.
-int dk_launch_decide(const double *p1_src, int from_partials, int nparts,
-		     double roll, double *dec_slot, dk_state *next)
+int dk_launch_widen(const dk_state *s, u64 first, u64 count, double2 *dst)
 {
-	(void)from_partials;
-	hipLaunchKernelGGL(k_decide, dim3(1), dim3(TPB), 0, g_ctx.stream, p1_src,
+	const unsigned nb = grid_for(count, TPB * 4);
+	hipLaunchKernelGGL(k_widen<float2>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+			   s->a64() + first, count, dst);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_decide(const double *parts, int nparts, double roll,
+		     double *dec_slot, dk_state *next)
+{
+	hipLaunchKernelGGL(k_decide, dim3(1), dim3(TPB), 0, g_ctx.stream, parts,
 			   nparts, roll, dec_slot, next ? next->scal : nullptr);
 	DK_LAUNCHED();
 	return DK_OK;
@@ -961,24 +944,14 @@ int dk_launch_decide(const double *p1_src, int from_partials, int nparts,
 int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 		       dk_state *o, int next_q, int *nparts_out)
 {
-	constexpr int U = 4;
 	const u64 olen = s->len >> 1;
-	const unsigned nb = grid_for(olen, (u64)TPB * U);
-	hipLaunchKernelGGL(k_collapse<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
-			   s->amp, s->scal, q, dec_slot, o->amp, o->scal, next_q,
-			   olen, g_ctx.partials);
-	DK_LAUNCHED();
-	*nparts_out = (int)nb;
-	return DK_OK;
-}
-
-int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out)
-{
-	constexpr int U = 4;
-	const u64 half = s->len >> 1;
-	const unsigned nb = grid_for(half, (u64)TPB * U);
-	hipLaunchKernelGGL(k_prob<U>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
-			   s->amp, s->scal, q, half, g_ctx.partials);
+	const unsigned nb = grid_for(olen, (u64)TPB * 4);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_collapse<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream, \
+			    (const T *)s->amp, s->scal, q, dec_slot, (T *)o->amp,  \
+			    o->scal, next_q, olen, g_ctx.partials)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
 	*nparts_out = (int)nb;
 	return DK_OK;
@@ -987,9 +960,12 @@ int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out)
 int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
 {
 	const unsigned nb = grid_for(o->len, TPB * 4);
-	hipLaunchKernelGGL(k_join, dim3(nb), dim3(TPB), 0, g_ctx.stream, a->amp,
-			   a->scal, b->amp, b->scal, b->nq, o->len, o->amp,
-			   o->scal);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_join<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream,     \
+			    (const T *)a->amp, a->scal, (const T *)b->amp, b->scal, \
+			    b->nq, o->len, (T *)o->amp, o->scal)
+	DK_BY_TYPE(o, L);
+#undef L
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -1002,8 +978,12 @@ int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev)
 	if (start >= s->len)
 		return DK_OK;
 	const unsigned nb = grid_for(s->len - start, TPB * 4);
-	hipLaunchKernelGGL(k_first_nonzero, dim3(nb), dim3(TPB), 0, g_ctx.stream,
-			   s->amp, start, s->len, (unsigned long long *)dst_dev);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_first_nonzero<T>, dim3(nb), dim3(TPB), 0,         \
+			    g_ctx.stream, (const T *)s->amp, start, s->len,         \
+			    (unsigned long long *)dst_dev)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -1052,9 +1032,12 @@ int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
 	double2 *part = nullptr;
 	DK_HIP(hipMallocAsync((void **)&part, nch * dim * dim * sizeof(double2),
 			      g_ctx.stream));
-	hipLaunchKernelGGL(k_density_tile, dim3((unsigned)ntiles, (unsigned)nch),
-			   dim3(TPB), 0, g_ctx.stream, s->amp, s->scal, da, tchunk,
-			   tpr, part);
+#define L(T_)                                                                  \
+	hipLaunchKernelGGL(k_density_tile<T_>, dim3((unsigned)ntiles, (unsigned)nch), \
+			    dim3(TPB), 0, g_ctx.stream, (const T_ *)s->amp, s->scal, \
+			    da, tchunk, tpr, part)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
 	const unsigned nb = grid_for(dim * dim, TPB);
 	hipLaunchKernelGGL(k_density_sum, dim3(nb), dim3(TPB), 0, g_ctx.stream,
@@ -1072,9 +1055,12 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 	dens_args(s, keep_mask, da);
 	if (count == 0)
 		return DK_OK;
-	hipLaunchKernelGGL(k_density_elements, dim3((unsigned)count), dim3(TPB),
-			   0, g_ctx.stream, s->amp, s->scal, da, rows_dev, cols_dev,
-			   out_dev);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_density_elements<T>, dim3((unsigned)count),       \
+			    dim3(TPB), 0, g_ctx.stream, (const T *)s->amp, s->scal, \
+			    da, rows_dev, cols_dev, out_dev)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -1082,8 +1068,11 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 int dk_launch_fill_zero_state(dk_state *s)
 {
 	const unsigned nb = grid_for(s->len, TPB * 4);
-	hipLaunchKernelGGL(k_fill_zero_state, dim3(nb), dim3(TPB), 0,
-			   g_ctx.stream, s->amp, s->scal, s->len);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_fill_zero_state<T>, dim3(nb), dim3(TPB), 0,       \
+			    g_ctx.stream, (T *)s->amp, s->scal, s->len)
+	DK_BY_TYPE(s, L);
+#undef L
 	DK_LAUNCHED();
 	return DK_OK;
 }

@@ -224,12 +224,16 @@ void dk_device_free(void *p, u64 bytes)
 
 // ---- states -----------------------------------------------------------------
 
-int dk_alloc_state(unsigned nq, dk_state **out)
+int dk_alloc_state(unsigned nq, int dtype, dk_state **out)
 {
 	*out = nullptr;
 	if (nq < 1 || nq > DK_MAX_QUBITS) {
 		dk_set_error("Number of qubits exceeds maximum");
 		return DK_E_QUBITS;
+	}
+	if (dtype != DK_C128 && dtype != DK_C64) {
+		dk_set_error("unknown amplitude type %d", dtype);
+		return DK_E_ARG;
 	}
 	DK_INIT();
 	if (nq > 44) {
@@ -238,7 +242,7 @@ int dk_alloc_state(unsigned nq, dk_state **out)
 		return DK_E_BUDGET;
 	}
 	const u64 len = 1ull << nq;
-	const u64 bytes = DK_SCAL_BYTES + len * sizeof(double2);
+	const u64 bytes = DK_SCAL_BYTES + len * dk_amp_size(dtype);
 	{
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		if (g_ctx.used + bytes > g_ctx.budget) {
@@ -260,11 +264,12 @@ int dk_alloc_state(unsigned nq, dk_state **out)
 	}
 	dk_state *s = new dk_state;
 	s->nq = nq;
+	s->dtype = dtype;
 	s->len = len;
 	s->base = base;
 	// amplitudes first, so they start on the allocation's own alignment
-	s->amp = (double2 *)base;
-	s->scal = (double *)(base + len * sizeof(double2));
+	s->amp = base;
+	s->scal = (double *)(base + len * dk_amp_size(dtype));
 	s->bytes = bytes;
 	*out = s;
 	return DK_OK;
@@ -287,10 +292,50 @@ extern "C" void dk_state_free(dk_state *s)
 	dk_release_state(s);
 }
 
+extern "C" int dk_state_new_typed(unsigned num_qubits, int zero_state,
+				  int dtype, dk_state **out)
+{
+	dk_state *s;
+	int rc = dk_alloc_state(num_qubits, dtype, &s);
+	if (rc)
+		return rc;
+	rc = zero_state ? dk_launch_fill_zero_state(s) : dk_launch_set_norm(s, 1.0);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" int dk_state_dtype(const dk_state *s)
+{
+	return s ? s->dtype : -1;
+}
+
+extern "C" int dk_state_convert(const dk_state *src, int dtype, dk_state **out)
+{
+	if (!src || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	dk_state *s;
+	int rc = dk_alloc_state(src->nq, dtype, &s);
+	if (rc)
+		return rc;
+	rc = dk_launch_normalised_copy(src, s);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
 extern "C" int dk_state_new(unsigned num_qubits, int zero_state, dk_state **out)
 {
 	dk_state *s;
-	int rc = dk_alloc_state(num_qubits, &s);
+	int rc = dk_alloc_state(num_qubits, DK_C128, &s);
 	if (rc)
 		return rc;
 	rc = zero_state ? dk_launch_fill_zero_state(s) : dk_launch_set_norm(s, 1.0);
@@ -310,7 +355,7 @@ extern "C" int dk_state_from_host(unsigned num_qubits, const double *values,
 		return DK_E_ARG;
 	}
 	dk_state *s;
-	int rc = dk_alloc_state(num_qubits, &s);
+	int rc = dk_alloc_state(num_qubits, DK_C128, &s);
 	if (rc)
 		return rc;
 	hipError_t e = hipMemcpyAsync(s->amp, values, s->len * sizeof(double2),
@@ -338,7 +383,7 @@ extern "C" int dk_state_clone(const dk_state *src, dk_state **out)
 		return DK_E_ARG;
 	}
 	dk_state *s;
-	int rc = dk_alloc_state(src->nq, &s);
+	int rc = dk_alloc_state(src->nq, src->dtype, &s);
 	if (rc)
 		return rc;
 	rc = dk_launch_normalised_copy(src, s);
@@ -382,6 +427,28 @@ static int copy_to_host(double *dst, const void *src, size_t count)
 	return DK_OK;
 }
 
+// Raw amplitudes [first, first+count) as interleaved doubles (complex64
+// states are widened on the device first).
+static int read_amplitudes(const dk_state *s, u64 first, u64 count,
+			   double *values)
+{
+	if (s->dtype == DK_C128)
+		return copy_to_host(values, s->a128() + first, 2 * count);
+	const u64 chunk = DK_PINNED / 2;
+	double2 *tmp = nullptr;
+	DK_HIP(hipMallocAsync((void **)&tmp, std::min(count, chunk) * sizeof(double2),
+			      g_ctx.stream));
+	int rc = DK_OK;
+	for (u64 done = 0; !rc && done < count; done += chunk) {
+		const u64 n = std::min(chunk, count - done);
+		rc = dk_launch_widen(s, first + done, n, tmp);
+		if (!rc)
+			rc = copy_to_host(values + 2 * done, tmp, 2 * n);
+	}
+	(void)hipFreeAsync(tmp, g_ctx.stream);
+	return rc;
+}
+
 extern "C" int dk_state_norm(const dk_state *s, double *norm)
 {
 	DK_INIT();
@@ -407,20 +474,30 @@ extern "C" int dk_gate_new(unsigned num_qubits, const double *matrix,
 	g->k = num_qubits;
 	g->dim = 1u << num_qubits;
 	g->dmat = nullptr;
+	g->dmat32 = nullptr;
 	const size_t n = (size_t)g->dim * g->dim;
 	memcpy(g->hmat, matrix, n * sizeof(double2));
 	if (num_qubits > 1) {
-		hipError_t e = hipMalloc((void **)&g->dmat, n * sizeof(double2));
+		// FP64 and FP32 copies in one allocation
+		std::vector<float2> m32(n);
+		for (size_t i = 0; i < n; ++i)
+			m32[i] = make_float2((float)g->hmat[i].x, (float)g->hmat[i].y);
+		char *buf = nullptr;
+		const size_t b64 = n * sizeof(double2), b32 = n * sizeof(float2);
+		hipError_t e = hipMalloc((void **)&buf, b64 + b32);
 		if (e == hipSuccess)
-			e = hipMemcpy(g->dmat, g->hmat, n * sizeof(double2),
-				      hipMemcpyHostToDevice);
+			e = hipMemcpy(buf, g->hmat, b64, hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(buf + b64, m32.data(), b32, hipMemcpyHostToDevice);
 		if (e != hipSuccess) {
 			dk_set_error("gate upload failed: %s", hipGetErrorString(e));
-			if (g->dmat)
-				(void)hipFree(g->dmat);
+			if (buf)
+				(void)hipFree(buf);
 			delete g;
 			return DK_E_DEVICE;
 		}
+		g->dmat = (double2 *)buf;
+		g->dmat32 = (float2 *)(buf + b64);
 	}
 	*out = g;
 	return DK_OK;
@@ -477,7 +554,7 @@ extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
 	}
 	dk_state *o = in;
 	if (!in_place) {
-		int rc = dk_alloc_state(n, &o);
+		int rc = dk_alloc_state(n, in->dtype, &o);
 		if (rc)
 			return rc;
 	}
@@ -566,9 +643,9 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 	// P1 of the first measured qubit
 	rc = dk_launch_prob_partials(cur, qs[0], &nparts);
 	if (!rc && cur->nq > 1)
-		rc = dk_alloc_state(cur->nq - 1, &next);
+		rc = dk_alloc_state(cur->nq - 1, s->dtype, &next);
 	if (!rc)
-		rc = dk_launch_decide(g_ctx.partials, 1, nparts, rolls[0],
+		rc = dk_launch_decide(g_ctx.partials, nparts, rolls[0],
 				      g_ctx.dec + 0, next);
 	for (size_t k = 0; !rc && k < qs.size(); ++k) {
 		if (cur->nq == 1)
@@ -585,9 +662,9 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 		next = nullptr;
 		if (next_q >= 0) {
 			if (cur->nq > 1)
-				rc = dk_alloc_state(cur->nq - 1, &next);
+				rc = dk_alloc_state(cur->nq - 1, s->dtype, &next);
 			if (!rc)
-				rc = dk_launch_decide(g_ctx.partials, 1, nparts,
+				rc = dk_launch_decide(g_ctx.partials, nparts,
 						      rolls[k + 1],
 						      g_ctx.dec + 2 * (k + 1), next);
 		}
@@ -614,7 +691,7 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 		const double p = dec[2 * k] != 0.0 ? dec[2 * k + 1]
 						   : 1.0 - dec[2 * k + 1];
 		const bool leaves_qubits = (n - 1 - k) > 0;
-		if (leaves_qubits && !(sqrt(p) != 0.0)) {
+		if (leaves_qubits && sqrt(p) == 0.0) {
 			if (owned)
 				dk_release_state(owned);
 			dk_set_error("New normalization constant is 0. Please report "
@@ -640,8 +717,12 @@ extern "C" int dk_join(const dk_state *most, const dk_state *least,
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
+	if (most->dtype != least->dtype) {
+		dk_set_error("joined states must share an amplitude type");
+		return DK_E_ARG;
+	}
 	dk_state *o;
-	int rc = dk_alloc_state(most->nq + least->nq, &o);
+	int rc = dk_alloc_state(most->nq + least->nq, most->dtype, &o);
 	if (rc)
 		return rc;
 	rc = dk_launch_join(most, least, o);
@@ -667,7 +748,7 @@ extern "C" int dk_read_raw(const dk_state *s, uint64_t first, uint64_t count,
 	}
 	int rc = DK_OK;
 	if (count > 0 && values)
-		rc = copy_to_host(values, s->amp + first, 2 * count);
+		rc = read_amplitudes(s, first, count, values);
 	if (!rc && norm)
 		rc = copy_to_host(norm, s->scal + DK_SCAL_NORM, 1);
 	return rc;
@@ -696,7 +777,7 @@ extern "C" int dk_first_nonzero(const dk_state *s, uint64_t *index)
 	// most states have a nonzero amplitude near the start: look there first
 	const u64 head = std::min<u64>(s->len, 2048);
 	std::vector<double> buf(2 * head);
-	int rc = copy_to_host(buf.data(), s->amp, 2 * head);
+	int rc = read_amplitudes(s, 0, head, buf.data());
 	if (rc)
 		return rc;
 	for (u64 i = 0; i < head; ++i)
@@ -837,6 +918,10 @@ extern "C" int dk_tune(const char *key, int64_t value)
 	}
 	if (!strcmp(key, "time_apply")) {
 		g_ctx.time_apply = value != 0;
+		return DK_OK;
+	}
+	if (!strcmp(key, "dense_mfma")) {
+		g_ctx.dense_mfma = value != 0;
 		return DK_OK;
 	}
 	dk_set_error("unknown tunable %s", key);

@@ -40,8 +40,16 @@ extern "C" {
 #define DK_MAX_QUBITS 62    /* MAX_NUM_QUBITS = log2_64(INTMAX_MAX), platform.h:270 */
 #define DK_MAX_GATE_QUBITS 6
 
+/* Amplitude storage types.  DK_C128 is the reference's COMPLEX_TYPE
+ * (double _Complex, platform.h:199-216).  DK_C64 (float2, FP32 arithmetic)
+ * is an extra mode with no reference counterpart; its norms, probabilities
+ * and renormalisations are still computed in FP64. */
+#define DK_C128 0
+#define DK_C64 1
+
 typedef struct dk_state dk_state;
 typedef struct dk_gate dk_gate;
+typedef struct dk_circuit dk_circuit;
 
 /* Library set-up on a HIP device (first call wins; later calls are no-ops).
  * No reference counterpart: the reference has no device. */
@@ -75,6 +83,18 @@ int dk_state_from_host(unsigned num_qubits, const double *values,
 /* Normalised copy (raw / norm, norm 1).  Replaces state_clone,
  * qstate.c:90-105 (registry_clone, doki.c:343-398). */
 int dk_state_clone(const dk_state *src, dk_state **out);
+
+/* dk_state_new with a storage type (DK_C128 or DK_C64).  No reference
+ * counterpart for DK_C64 (the reference fixes PRECISION 2, platform.h:199). */
+int dk_state_new_typed(unsigned num_qubits, int zero_state, int dtype,
+		       dk_state **out);
+
+/* Storage type of a state (DK_C128 or DK_C64). */
+int dk_state_dtype(const dk_state *s);
+
+/* Normalised copy into another storage type (state_clone with a type
+ * change; qstate.c:90-105 for DK_C128 -> DK_C128). */
+int dk_state_convert(const dk_state *src, int dtype, dk_state **out);
 
 /* Releases the state's device memory.  Replaces state_clear,
  * qstate.c:107-121 (registry_del / capsule destructor, doki.c:258-270). */
@@ -165,6 +185,52 @@ int dk_reduced_density(const dk_state *s, uint64_t keep_mask, double *rho);
 int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask,
 				size_t count, const uint64_t *rows,
 				const uint64_t *cols, double *out);
+
+/* ---- gate lists and fusion ------------------------------------------------
+ * No single reference counterpart: a caller (QSimov's circuit layer) issues
+ * the same gates one registry_apply at a time (doki.c:722-941).  A circuit
+ * records them with registry_apply's argument meaning; fusion groups them
+ * into dense blocks of at most max_block_qubits qubits (controls absorbed),
+ * each applied as one gate, so a run costs one state pass per block. */
+
+int dk_circuit_new(unsigned num_qubits, dk_circuit **out);
+void dk_circuit_free(dk_circuit *c);
+
+/* Appends a gate: as dk_apply's gate/targets/ctrl_mask/actrl_mask. */
+int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
+		   const unsigned *targets, uint64_t ctrl_mask,
+		   uint64_t actrl_mask);
+
+/* Plans fused blocks of at most max_block_qubits (1..6) qubits; 0 runs the
+ * gates one by one.  *num_blocks (may be NULL) receives the block count. */
+int dk_circuit_fuse(dk_circuit *c, unsigned max_block_qubits,
+		    size_t *num_blocks);
+
+int dk_circuit_num_gates(const dk_circuit *c, size_t *num_gates);
+
+/* Block i of the plan: its qubits (ascending; local bit b = qubits[b]) and
+ * its dense 2^m x 2^m unitary (row-major, interleaved).  Either output may
+ * be NULL. */
+int dk_circuit_block(const dk_circuit *c, size_t i, unsigned *num_qubits,
+		     unsigned *qubits, double *matrix);
+
+/* Runs the circuit on st: the first step writes a new state (*out) unless
+ * in_place, later steps run in place.  Works on DK_C128 and DK_C64 states. */
+int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place, dk_state **out);
+
+/* ---- timing, tuning, device ------------------------------------------------
+ * No reference counterparts. */
+
+/* Records event `slot` (0..63) on the library stream. */
+int dk_event_record(int slot);
+/* Milliseconds between two recorded events (waits for the second). */
+int dk_event_elapsed(int first_slot, int second_slot, float *ms);
+/* Tunables: "apply_blocks" (grid cap of grid-stride kernels, 0 = auto),
+ * "time_apply" (events 62/63 bracket every gate kernel), "dense_mfma"
+ * (dense k >= 4 gates on MFMA, else on the vector ALUs). */
+int dk_tune(const char *key, int64_t value);
+/* Compute units and architecture name of the library's device. */
+int dk_device_info(int *num_cus, char *arch, size_t arch_len);
 
 #ifdef __cplusplus
 }
