@@ -13,7 +13,8 @@
 // (four 16-column tiles), so its A fragments — 16 rows x all D columns of M —
 // stay in registers for the whole launch while tiles stream through LDS.
 // Operand layouts (16x16x4, wave64): lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; the accumulator holds D[4*(l/16) + r][l%16], r = 0..3.
+// B[l/16][l%16]; accumulator element r of lane l holds
+// D[out_row(l/16, r)][l%16] (see mfma16 below).
 // Input scaling by 1/norm is applied to the outputs (the product is linear).
 #include <algorithm>
 
@@ -27,11 +28,18 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 template <typename R> struct mfma16;
+// out_row(g, r): row of the 16x16 result held in element r of a lane of row
+// group g = lane / 16 (measured on gfx950: FP64 interleaves the groups,
+// FP32 keeps four consecutive rows per group).
 template <> struct mfma16<double> {
 	typedef d4_t acc;
 	static __device__ __forceinline__ acc op(double a, double b, acc c)
 	{
 		return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+	}
+	static __device__ __forceinline__ int out_row(int g, int r)
+	{
+		return g + 4 * r;
 	}
 };
 template <> struct mfma16<float> {
@@ -39,6 +47,10 @@ template <> struct mfma16<float> {
 	static __device__ __forceinline__ acc op(float a, float b, acc c)
 	{
 		return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+	}
+	static __device__ __forceinline__ int out_row(int g, int r)
+	{
+		return 4 * g + r;
 	}
 };
 
@@ -82,7 +94,7 @@ __global__ __launch_bounds__(TPB) void k_apply_dense(
 	int out_slot[4];
 #pragma unroll
 	for (int r = 0; r < 4; ++r)
-		out_slot[r] = ta.colslot[rt * 16 + 4 * lk + r];
+		out_slot[r] = ta.colslot[rt * 16 + MF::out_row(lk, r)];
 
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	double acc = 0.0;
