@@ -106,6 +106,71 @@ def gpu_leg(args):
     }
 
 
+def fused_leg(args):
+    """Fused dense gates on the matrix cores at the same register size:
+    per-k kernel rates (FP64 complex128 and FP32 complex64), the brickwork
+    circuit plain vs fused (circuits/s), and the complex64 error against the
+    FP64 result."""
+    import numpy as np
+
+    from doki_amd import capi, workloads as W
+    n = args.qubits
+    out = {"dense_kernel": {}}
+    rng = W.rng_for(4242)
+    capi.tune("time_apply", 1)
+    for dtype, name, amp_bytes in ((capi.C128, "f64", 16), (capi.C64, "f32", 8)):
+        s = capi.new_typed(n, dtype)
+        for k in (4, 5, 6):
+            g = capi.Gate(W.haar_unitary(rng, k))
+            ms = []
+            for rep in range(3):
+                tg = list(range(rep * 7 % (n - k), rep * 7 % (n - k) + k))
+                capi.apply(s, g, tg, in_place=True)
+                ms.append(capi.event_elapsed_ms(62, 63))
+            t = min(ms) * 1e-3
+            out["dense_kernel"][f"{name}_k{k}"] = {
+                "kernel_ms": round(t * 1e3, 3),
+                "GBps": round(2 * amp_bytes * (1 << n) / t / 1e9, 1),
+                "TFLOPs": round(8 * (1 << k) * (1 << n) / t / 1e12, 2)}
+        del s
+        capi.synchronize()
+    capi.tune("time_apply", 0)
+    log(f"[bench] dense kernels {out['dense_kernel']}")
+
+    depth = args.fused_depth
+    steps = W.brickwork_circuit(77, n, depth)
+    res = {}
+    for label, k, dtype in (("plain_f64", 0, capi.C128), ("fused5_f64", 5, capi.C128),
+                            ("fused4_f64", 4, capi.C128), ("fused6_f64", 6, capi.C128),
+                            ("fused5_f32", 5, capi.C64)):
+        circ = capi.Circuit(n, steps)
+        nb = circ.fuse(k)
+        st = capi.new_typed(n, dtype)
+        circ.run(st, in_place=True)      # warm (gate uploads)
+        capi.synchronize()
+        st = capi.new_typed(n, dtype)
+        capi.synchronize()
+        t0 = time.perf_counter()
+        circ.run(st, in_place=True)
+        capi.synchronize()
+        el = time.perf_counter() - t0
+        res[label] = {"passes": nb or len(steps), "seconds": round(el, 4),
+                      "circuits_per_s": round(1.0 / el, 3)}
+        if label == "fused5_f64":
+            ref_state = st
+        elif label == "fused5_f32":
+            a = ref_state.raw(0, 1 << 22)[0] / ref_state.norm
+            b = st.raw(0, 1 << 22)[0] / st.norm
+            res[label]["max_abs_err_vs_f64_first_4M"] = float(np.max(np.abs(a - b)))
+        del st
+        capi.synchronize()
+        log(f"[bench] {label}: {res[label]}")
+    out["brickwork_circuit"] = {"qubits": n, "depth": depth, "gates": len(steps), **res}
+    out["fused_speedup_vs_plain"] = round(
+        res["plain_f64"]["seconds"] / res["fused5_f64"]["seconds"], 2)
+    return out
+
+
 def cpu_leg(args, gb_per_gate_unit=True):
     """The reference's own CPU path (oracle/_ref, compiled from its sources),
     OpenMP on the host cores, over a bounded sample of the same workload:
@@ -161,6 +226,8 @@ def main():
     ap.add_argument("--cpu-qubits", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--fused-depth", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -171,6 +238,7 @@ def main():
         sys.exit("--steps must be >= 1")
 
     g = gpu_leg(args)
+    fused = None if args.no_fused else fused_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
     if cpu:
         log(f"[bench] cpu baseline {cpu}")
@@ -210,6 +278,7 @@ def main():
             "gate_kernel_avg_ms": round(g["kavg_ms"], 4),
             "per_target_kernel": g["per_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
+            "fused": fused,
         },
     }
     print(json.dumps(out), flush=True)

@@ -130,6 +130,20 @@ __device__ __forceinline__ u64 tile_index(u64 base, int s, const TileArgs &ta)
 	return base | ta.dep_lo[s & 63] | ta.dep_hi[s >> 6];
 }
 
+// LDS position of tile slot s.  Gate kernels read a tile along strides of
+// 16, 256, ... slots (one group per lane); XOR-folding the higher slot bits
+// into the low ones spreads such reads over all LDS banks.  A bijection on
+// every aligned run of 16 (complex128) or 32 (complex64) slots.
+template <typename T> __device__ __forceinline__ int lds_pos(int s);
+template <> __device__ __forceinline__ int lds_pos<double2>(int s)
+{
+	return s ^ (((s >> 4) ^ (s >> 8)) & 15);
+}
+template <> __device__ __forceinline__ int lds_pos<float2>(int s)
+{
+	return s ^ (((s >> 5) ^ (s >> 10)) & 31);
+}
+
 // Slot of group g: its bits deposited around the (sorted) target slots.
 template <int K>
 __device__ __forceinline__ int group_slot(int g, const TileArgs &ta)
@@ -142,6 +156,7 @@ __device__ __forceinline__ int group_slot(int g, const TileArgs &ta)
 
 } // namespace dkd
 
-// Host helper: tile description for a gate on `targets` in an n-qubit state.
+// Host helper: tile description for a gate on `targets` in an n-qubit state;
+// the tile spans min(tile_bits, n) index bits (tile_bits <= 12).
 void dk_tile_args(unsigned n, unsigned k, const unsigned *targets,
-		  dkd::TileArgs &ta);
+		  dkd::TileArgs &ta, int tile_bits = dkd::TILE_BITS);

@@ -157,10 +157,17 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	double acc = 0.0;
 
+	// slot s = threadIdx.x + j * TPB: index bits = thread part | part of j*TPB
+	const u64 tdep = ta.dep_lo[threadIdx.x & 63] | ta.dep_hi[threadIdx.x >> 6];
 	for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		const u64 base = tile_base(tile, ta);
-		for (int s = threadIdx.x; s < tsz; s += TPB)
-			lds[s] = cscale(in[tile_index(base, s, ta)], inv);
+		const u64 base = tile_base(tile, ta) | tdep;
+#pragma unroll
+		for (int j = 0; j < OPT; ++j) {
+			const int s = threadIdx.x + j * TPB;
+			if (s < tsz)
+				lds[lds_pos<T>(s)] =
+					cscale(in[base | ta.dep_hi[(j * TPB) >> 6]], inv);
+		}
 		__syncthreads();
 
 		T res[IPT][RR];
@@ -177,13 +184,13 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 			rbs[it] = rb;
 			const int gs = group_slot<K>(g, ta);
 			gsl[it] = gs;
-			const u64 gi = tile_index(base, gs, ta);
+			const u64 gi = tile_index(base & ~tdep, gs, ta);
 			if ((gi & cm) == cm && (gi & acm) == 0) {
 #pragma unroll
 				for (int r = 0; r < RR; ++r)
 					res[it][r] = czero<T>();
 				for (int j = 0; j < D; ++j) {
-					const T x = lds[gs | ta.colslot[j]];
+					const T x = lds[lds_pos<T>(gs | ta.colslot[j])];
 					const T *mrow = M + (size_t)(rb * RR) * D + j;
 #pragma unroll
 					for (int r = 0; r < RR; ++r)
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 			} else {
 #pragma unroll
 				for (int r = 0; r < RR; ++r)
-					res[it][r] = lds[gs | ta.colslot[rb * RR + r]];
+					res[it][r] = lds[lds_pos<T>(gs | ta.colslot[rb * RR + r])];
 			}
 		}
 		__syncthreads();
@@ -202,13 +209,17 @@ __global__ __launch_bounds__(TPB) void k_apply_tile(
 				continue;
 #pragma unroll
 			for (int r = 0; r < RR; ++r)
-				lds[gsl[it] | ta.colslot[rbs[it] * RR + r]] = res[it][r];
+				lds[lds_pos<T>(gsl[it] | ta.colslot[rbs[it] * RR + r])] = res[it][r];
 		}
 		__syncthreads();
-		for (int s = threadIdx.x; s < tsz; s += TPB) {
-			const T o = lds[s];
-			out[tile_index(base, s, ta)] = o;
-			acc += cabs2d(o);
+#pragma unroll
+		for (int j = 0; j < OPT; ++j) {
+			const int s = threadIdx.x + j * TPB;
+			if (s < tsz) {
+				const T o = lds[lds_pos<T>(s)];
+				out[base | ta.dep_hi[(j * TPB) >> 6]] = o;
+				acc += cabs2d(o);
+			}
 		}
 		__syncthreads();
 	}
@@ -722,10 +733,12 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 
 // ===========================================================================
 
-void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta)
+void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta,
+		  int tile_bits)
 {
 	memset(&ta, 0, sizeof(ta));
-	const int npos = (int)std::min<unsigned>(TILE_BITS, n);
+	const int npos = (int)std::min<unsigned>(
+		(unsigned)std::min(tile_bits, (int)TILE_BITS), n);
 	bool is_t[64] = {false};
 	for (unsigned b = 0; b < k; ++b)
 		is_t[targets[b]] = true;
@@ -780,7 +793,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	if (g_ctx.time_apply)
 		DK_HIP(hipEventRecord(g_ctx.events[62], st));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
-	    in->nq >= (unsigned)TILE_BITS) {
+	    in->nq >= g->k + 6) {
 		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
 	} else {
 		rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
