@@ -84,6 +84,17 @@ def lib():
         "dk_circuit_block": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_uint),
                                        C.POINTER(C.c_uint), dp]),
         "dk_circuit_run": (C.c_int, [P, P, C.c_int, PP]),
+        "dk_batch_new": (C.c_int, [C.c_int, PP]),
+        "dk_batch_free": (None, [P]),
+        "dk_batch_set": (C.c_int, [P, C.c_char_p, C.c_int64]),
+        "dk_batch_add": (C.c_int, [P, C.c_uint, C.c_size_t, dp, C.POINTER(C.c_uint),
+                                   C.POINTER(u64), C.POINTER(u64), dp, u64,
+                                   C.POINTER(C.c_size_t)]),
+        "dk_batch_run": (C.c_int, [P]),
+        "dk_batch_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "dk_batch_outcomes": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_int8), dp]),
+        "dk_batch_density": (C.c_int, [P, C.c_size_t, dp]),
+        "dk_batch_state": (C.c_int, [P, C.c_size_t, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -304,6 +315,74 @@ class Circuit:
         p = C.c_void_p()
         check(lib().dk_circuit_run(self.ptr, s.ptr, int(in_place), C.byref(p)))
         return s if in_place else State(p.value)
+
+
+class Batch:
+    """Many small registers run in one persistent launch (dk_batch_*)."""
+
+    def __init__(self, dtype=C128, **settings):
+        p = C.c_void_p()
+        check(lib().dk_batch_new(dtype, C.byref(p)))
+        self.ptr = p
+        self.nqs = []
+        self.keeps = []
+        for k, v in settings.items():
+            check(lib().dk_batch_set(self.ptr, k.encode(), int(v)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _lib is not None:
+            _lib.dk_batch_free(self.ptr)
+            self.ptr = None
+
+    def add(self, nq, steps, rolls=None, density_keep=()):
+        """steps: (2x2 matrix, [target], controls, anticontrols) tuples."""
+        ng = len(steps)
+        mats = np.zeros((max(ng, 1), 4), dtype=np.complex128)
+        tg = np.zeros(max(ng, 1), dtype=np.uint32)
+        cm = np.zeros(max(ng, 1), dtype=np.uint64)
+        am = np.zeros(max(ng, 1), dtype=np.uint64)
+        for i, (m, t, c, a) in enumerate(steps):
+            if len(t) != 1:
+                raise ValueError("batched registers take 1-qubit gates")
+            mats[i] = np.asarray(m, dtype=np.complex128).ravel()
+            tg[i] = t[0]
+            cm[i] = mask(c)
+            am[i] = mask(a)
+        mf = np.ascontiguousarray(mats.view(np.float64))
+        r = None
+        if rolls is not None:
+            r = np.ascontiguousarray(np.asarray(rolls, dtype=np.float64))
+            assert r.size == nq
+        idx = C.c_size_t()
+        u = C.POINTER(C.c_uint64)
+        check(lib().dk_batch_add(
+            self.ptr, nq, ng, _dp(mf), tg.ctypes.data_as(C.POINTER(C.c_uint)),
+            cm.ctypes.data_as(u), am.ctypes.data_as(u),
+            _dp(r) if r is not None else None, mask(density_keep), C.byref(idx)))
+        self.nqs.append(nq)
+        self.keeps.append(len(density_keep))
+        return idx.value
+
+    def run(self):
+        check(lib().dk_batch_run(self.ptr))
+
+    def outcomes(self, i):
+        n = self.nqs[i]
+        res = (C.c_int8 * n)()
+        p1 = np.zeros(n)
+        check(lib().dk_batch_outcomes(self.ptr, i, res, _dp(p1)))
+        return [None if res[k] < 0 else bool(res[k]) for k in range(n)], p1
+
+    def density(self, i):
+        d = 1 << self.keeps[i]
+        out = np.empty(2 * d * d)
+        check(lib().dk_batch_density(self.ptr, i, _dp(out)))
+        return out.view(np.complex128).reshape(d, d)
+
+    def state(self, i):
+        out = np.empty(2 << self.nqs[i])
+        check(lib().dk_batch_state(self.ptr, i, _dp(out)))
+        return out.view(np.complex128)
 
 
 def synchronize():

@@ -1,0 +1,722 @@
+// Batched small registers: one persistent launch runs many independent
+// registers of uneven size, each with its own gate list, an optional reduced
+// density matrix and a full measurement with injected rolls.
+//
+// Work distribution: workgroups take registers from an atomic counter,
+// largest first.  A register of up to 2^LB amplitudes (LB = 12 for
+// complex128, 13 for complex64) lives entirely in LDS.  Larger registers live
+// in a device pool and run as cache-blocked passes: a pass is a run of
+// consecutive gates whose targets fit one 2^LB-amplitude tile; the workgroup
+// streams the register through LDS tile by tile, applying the whole run to
+// each tile (controls outside the tile are decided per tile).
+//
+// Gate semantics are registry_apply's (qops.c:176-261); since every gate is
+// unitary, renormalising once after the last gate gives the reference's
+// per-gate renormalised values to rounding.  Measurement visits qubits from
+// the most significant down (doki.c:1097-1121) with outcome P1 > roll
+// (qops.c:132); P1 of each step is the squared norm of the upper half of the
+// still-possible index range over that of the whole range, in FP64.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dk_device.h"
+
+using namespace dkd;
+
+namespace {
+
+constexpr int BT = 512;            // threads per workgroup
+constexpr int BW = BT / 64;        // waves
+constexpr int LDS_BYTES = 65536;   // tile storage per workgroup
+
+struct BReg {
+	u64 off;        // amplitude offset of its pool slot (pooled registers)
+	u64 keep;       // density: kept qubits (0 = no density)
+	unsigned nq;
+	unsigned pass_begin, pass_end;
+	unsigned roll_off;  // first roll; also first outcome slot
+	unsigned dens_off;  // first element of its density output
+	unsigned flags;     // 1 measure, 2 density, 4 export final state
+};
+
+struct BPass {
+	u64 pmask;                 // index bits covered by the tile
+	unsigned gate_begin, gate_end;
+	unsigned npos;
+	unsigned char pos[16];     // tile positions, ascending
+	unsigned char slot_of[64]; // qubit -> tile slot bit (positions only)
+};
+
+struct BGate {
+	double m[8];  // 2x2 row-major, interleaved
+	u64 cm, am;
+	unsigned t;
+	unsigned pad;
+};
+
+enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4 };
+
+__device__ __forceinline__ double bsum(double v, double *red)
+{
+	v = wave_sum(v);
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	double t = 0.0;
+#pragma unroll
+	for (int w = 0; w < BW; ++w)
+		t += red[w];
+	__syncthreads();
+	return t; // valid in every thread
+}
+
+__device__ __forceinline__ u64 deposit(u64 x, const unsigned char *pos, int n)
+{
+	u64 r = 0;
+	for (int b = 0; b < n; ++b)
+		r |= ((x >> b) & 1ull) << pos[b];
+	return r;
+}
+
+// Applies one 1-qubit gate to the tile (2^tb slots) whose slot bit b is
+// index bit pos[b]; `base` holds the tile's fixed index bits.
+template <typename T>
+__device__ void tile_gate(T *tile, int tb, const BGate &g, const BPass &ps,
+			  u64 base)
+{
+	typedef typename real_of<T>::type R;
+	// controls and anticontrols outside the tile decide the whole tile
+	u64 cm_in = 0, am_in = 0;
+	const u64 cm_out = g.cm & ~ps.pmask, am_out = g.am & ~ps.pmask;
+	for (u64 m = g.cm & ps.pmask; m; m &= m - 1)
+		cm_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
+	for (u64 m = g.am & ps.pmask; m; m &= m - 1)
+		am_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
+	if ((base & cm_out) != cm_out || (base & am_out) != 0)
+		return;
+	const int sb = ps.slot_of[g.t];
+	const T m00 = mk<T>((R)g.m[0], (R)g.m[1]), m01 = mk<T>((R)g.m[2], (R)g.m[3]);
+	const T m10 = mk<T>((R)g.m[4], (R)g.m[5]), m11 = mk<T>((R)g.m[6], (R)g.m[7]);
+	const int half = 1 << (tb - 1);
+	for (int p = threadIdx.x; p < half; p += BT) {
+		const int s0 = (int)ins0((u64)p, (unsigned)sb);
+		if (((u64)s0 & cm_in) != cm_in || ((u64)s0 & am_in) != 0)
+			continue;
+		const int s1 = s0 | (1 << sb);
+		const T x = tile[lds_pos<T>(s0)], y = tile[lds_pos<T>(s1)];
+		tile[lds_pos<T>(s0)] = cmac(cmac(czero<T>(), m00, x), m01, y);
+		tile[lds_pos<T>(s1)] = cmac(cmac(czero<T>(), m10, x), m11, y);
+	}
+}
+
+// Value of amplitude i of the register (LDS-resident or pooled).
+template <typename T>
+__device__ __forceinline__ T amp_at(const T *tile, const T *st, bool resident,
+				    u64 i)
+{
+	return resident ? tile[lds_pos<T>((int)i)] : st[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_batch(
+	const BReg *__restrict__ regs, const unsigned *__restrict__ order,
+	unsigned nregs, const BPass *__restrict__ passes,
+	const BGate *__restrict__ gates, const double *__restrict__ rolls,
+	T *__restrict__ pool, signed char *__restrict__ outcomes,
+	double *__restrict__ p1s, double2 *__restrict__ dens,
+	unsigned *__restrict__ counter)
+{
+	constexpr int LT = LDS_BYTES / (int)sizeof(T);
+	constexpr int LB = sizeof(T) == 16 ? 12 : 13;
+	__shared__ T tile[LT];
+	__shared__ double red[BW];
+	__shared__ unsigned item;
+	__shared__ BPass ps;
+	__shared__ BGate gs;
+
+	for (;;) {
+		if (threadIdx.x == 0)
+			item = atomicAdd(counter, 1u);
+		__syncthreads();
+		const unsigned k = item;
+		__syncthreads();
+		if (k >= nregs)
+			break;
+		const BReg R = regs[order[k]];
+		const unsigned n = R.nq;
+		const u64 len = 1ull << n;
+		const bool resident = len <= (u64)LT;
+		T *st = pool + R.off;
+
+		if (resident) {
+			for (int s = threadIdx.x; s < (int)len; s += BT) {
+				T v = czero<T>();
+				if (s == 0)
+					v.x = 1;
+				tile[lds_pos<T>(s)] = v;
+			}
+			__syncthreads();
+			for (unsigned pi = R.pass_begin; pi < R.pass_end; ++pi) {
+				if (threadIdx.x == 0)
+					ps = passes[pi];
+				__syncthreads();
+				for (unsigned gi = ps.gate_begin; gi < ps.gate_end; ++gi) {
+					if (threadIdx.x == 0)
+						gs = gates[gi];
+					__syncthreads();
+					tile_gate(tile, (int)n, gs, ps, 0);
+					__syncthreads();
+				}
+			}
+		} else {
+			for (u64 i = threadIdx.x; i < len; i += BT) {
+				T v = czero<T>();
+				if (i == 0)
+					v.x = 1;
+				st[i] = v;
+			}
+			__syncthreads();
+			for (unsigned pi = R.pass_begin; pi < R.pass_end; ++pi) {
+				if (threadIdx.x == 0)
+					ps = passes[pi];
+				__syncthreads();
+				const u64 ntiles = len >> LB;
+				for (u64 tl = 0; tl < ntiles; ++tl) {
+					u64 base = tl;
+					for (int b = 0; b < (int)ps.npos; ++b)
+						base = ins0(base, ps.pos[b]);
+					for (int s = threadIdx.x; s < LT; s += BT)
+						tile[lds_pos<T>(s)] =
+							st[base | deposit((u64)s, ps.pos, ps.npos)];
+					__syncthreads();
+					for (unsigned gi = ps.gate_begin; gi < ps.gate_end; ++gi) {
+						if (threadIdx.x == 0)
+							gs = gates[gi];
+						__syncthreads();
+						tile_gate(tile, LB, gs, ps, base);
+						__syncthreads();
+					}
+					for (int s = threadIdx.x; s < LT; s += BT)
+						st[base | deposit((u64)s, ps.pos, ps.npos)] =
+							tile[lds_pos<T>(s)];
+					__syncthreads();
+				}
+			}
+		}
+
+		// squared norm of the final state
+		double acc = 0.0;
+		for (u64 i = threadIdx.x; i < len; i += BT)
+			acc += cabs2d(amp_at(tile, st, resident, i));
+		const double norm2 = bsum(acc, red);
+		const double inv = 1.0 / sqrt(norm2);
+
+		if ((R.flags & BF_EXPORT) && resident) {
+			for (u64 i = threadIdx.x; i < len; i += BT)
+				st[i] = tile[lds_pos<T>((int)i)];
+			__syncthreads();
+		}
+
+		if (R.flags & BF_DENSITY) {
+			// rho[i][j] = sum_t psi(i,t) conj(psi(j,t)) / norm^2 over the
+			// kept qubits (ascending = reduced index bit order)
+			unsigned char kp[8];
+			int a = 0;
+			for (unsigned q = 0; q < n; ++q)
+				if ((R.keep >> q) & 1)
+					kp[a++] = (unsigned char)q;
+			const int m = (int)n - a;
+			const int dim = 1 << a;
+			const u64 T_ = 1ull << m;
+			for (int e = threadIdx.x; e < dim * dim; e += BT) {
+				const u64 ki = deposit((u64)(e / dim), kp, a);
+				const u64 kj = deposit((u64)(e % dim), kp, a);
+				double2 s = make_double2(0.0, 0.0);
+				for (u64 t = 0; t < T_; ++t) {
+					u64 td = t;
+					for (int b = 0; b < a; ++b)
+						td = ins0(td, kp[b]);
+					const T x = amp_at(tile, st, resident, ki | td);
+					const T y = amp_at(tile, st, resident, kj | td);
+					const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+					s.x = fma(xr, yr, fma(xi, yi, s.x));
+					s.y = fma(xi, yr, fma(-xr, yi, s.y));
+				}
+				dens[R.dens_off + e] = make_double2(s.x / norm2, s.y / norm2);
+			}
+			__syncthreads();
+		}
+
+		if (R.flags & BF_MEASURE) {
+			u64 prefix = 0;
+			for (unsigned step = 0; step < n; ++step) {
+				const unsigned q = n - 1 - step;
+				const u64 start = prefix << (q + 1), half = 1ull << q;
+				double lo = 0.0, hi = 0.0;
+				for (u64 i = threadIdx.x; i < half; i += BT) {
+					lo += cabs2d(amp_at(tile, st, resident, start + i));
+					hi += cabs2d(amp_at(tile, st, resident, start + half + i));
+				}
+				const double slo = bsum(lo, red);
+				const double shi = bsum(hi, red);
+				const double p1 = shi / (slo + shi);
+				const bool v = p1 > rolls[R.roll_off + step];
+				if (threadIdx.x == 0) {
+					outcomes[R.roll_off + step] = v ? 1 : 0;
+					p1s[R.roll_off + step] = p1;
+				}
+				prefix = (prefix << 1) | (v ? 1u : 0u);
+			}
+		}
+		(void)inv;
+		__syncthreads();
+	}
+}
+
+__global__ void k_reset_counter(unsigned *c)
+{
+	if (threadIdx.x == 0)
+		*c = 0u;
+}
+
+} // namespace
+
+// ===========================================================================
+
+struct dk_batch {
+	int dtype;
+	std::vector<BReg> regs;
+	std::vector<BPass> passes;
+	std::vector<BGate> gates;
+	std::vector<double> rolls;
+	unsigned dens_total = 0;
+	int export_states = 0;
+	int use_graph = 0;
+	int pass_qubits = 0; // 0 = tile size
+	// device side
+	bool uploaded = false;
+	char *dbuf = nullptr;
+	u64 dbuf_bytes = 0;
+	void *pool = nullptr;
+	u64 pool_bytes = 0;
+	BReg *d_regs = nullptr;
+	unsigned *d_order = nullptr;
+	BPass *d_passes = nullptr;
+	BGate *d_gates = nullptr;
+	double *d_rolls = nullptr;
+	signed char *d_out = nullptr;
+	double *d_p1 = nullptr;
+	double2 *d_dens = nullptr;
+	unsigned *d_counter = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	signed char *h_out = nullptr;  // pinned, so graph replays can copy
+	double *h_p1 = nullptr;
+	double2 *h_dens = nullptr;
+	bool ran = false;
+};
+
+static int batch_lb(int dtype)
+{
+	return dtype == DK_C64 ? 13 : 12;
+}
+
+static void batch_release_device(dk_batch *b)
+{
+	if (b->exec)
+		(void)hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+	if (b->dbuf)
+		dk_device_free(b->dbuf, b->dbuf_bytes);
+	if (b->pool)
+		dk_device_free(b->pool, b->pool_bytes);
+	if (b->h_p1)
+		(void)hipHostFree(b->h_p1);
+	b->h_out = nullptr;
+	b->h_p1 = nullptr;
+	b->h_dens = nullptr;
+	b->dbuf = nullptr;
+	b->pool = nullptr;
+	b->uploaded = false;
+}
+
+extern "C" int dk_batch_new(int dtype, dk_batch **out)
+{
+	if (!out || (dtype != DK_C128 && dtype != DK_C64)) {
+		dk_set_error("unknown amplitude type");
+		return DK_E_ARG;
+	}
+	dk_batch *b = new dk_batch;
+	b->dtype = dtype;
+	*out = b;
+	return DK_OK;
+}
+
+extern "C" void dk_batch_free(dk_batch *b)
+{
+	if (!b)
+		return;
+	batch_release_device(b);
+	delete b;
+}
+
+extern "C" int dk_batch_set(dk_batch *b, const char *key, int64_t value)
+{
+	if (!b || !key) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	if (!strcmp(key, "export_states"))
+		b->export_states = value != 0;
+	else if (!strcmp(key, "use_graph"))
+		b->use_graph = value != 0;
+	else if (!strcmp(key, "pass_qubits"))
+		b->pass_qubits = (int)value;
+	else {
+		dk_set_error("unknown batch setting %s", key);
+		return DK_E_ARG;
+	}
+	batch_release_device(b);
+	return DK_OK;
+}
+
+// Gate record of a 1-qubit gate (registry_apply's argument meaning).
+extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
+			    const double *matrices, const unsigned *targets,
+			    const uint64_t *ctrl_masks, const uint64_t *actrl_masks,
+			    const double *rolls, uint64_t density_keep,
+			    size_t *index)
+{
+	if (!b || nq < 1 || nq > 26 || (num_gates && (!matrices || !targets))) {
+		dk_set_error("batched registers hold 1..26 qubits");
+		return DK_E_ARG;
+	}
+	const u64 valid = (1ull << nq) - 1;
+	if ((density_keep & ~valid) || __builtin_popcountll(density_keep) > 6) {
+		dk_set_error("density keeps at most 6 of the register's qubits");
+		return DK_E_ARG;
+	}
+	if (rolls)
+		for (unsigned i = 0; i < nq; ++i)
+			if (!(rolls[i] >= 0.0 && rolls[i] < 1.0)) {
+				dk_set_error("roll not in interval [0, 1)!");
+				return DK_E_ROLL;
+			}
+	std::vector<BGate> gl(num_gates);
+	for (size_t g = 0; g < num_gates; ++g) {
+		BGate &x = gl[g];
+		memcpy(x.m, matrices + 8 * g, sizeof(x.m));
+		x.t = targets[g];
+		x.cm = ctrl_masks ? ctrl_masks[g] : 0;
+		x.am = actrl_masks ? actrl_masks[g] : 0;
+		x.pad = 0;
+		if (x.t >= nq || ((x.cm | x.am) & ~valid) || (x.cm & x.am) ||
+		    (((x.cm | x.am) >> x.t) & 1)) {
+			dk_set_error("invalid gate %zu for a %u-qubit register", g, nq);
+			return DK_E_ARG;
+		}
+	}
+	BReg R;
+	memset(&R, 0, sizeof(R));
+	R.nq = nq;
+	R.keep = density_keep;
+	R.roll_off = (unsigned)b->rolls.size();
+	R.flags = (rolls ? BF_MEASURE : 0) | (density_keep ? BF_DENSITY : 0);
+	R.dens_off = b->dens_total;
+	if (density_keep) {
+		const unsigned d = 1u << __builtin_popcountll(density_keep);
+		b->dens_total += d * d;
+	}
+	for (unsigned i = 0; i < nq; ++i)
+		b->rolls.push_back(rolls ? rolls[i] : 0.0);
+	// passes: runs of gates whose targets fit one tile
+	const int lb = batch_lb(b->dtype);
+	const int tb = (int)std::min<unsigned>(nq, (unsigned)lb);
+	int budget = b->pass_qubits > 0 ? std::min(b->pass_qubits, tb) : tb;
+	R.pass_begin = (unsigned)b->passes.size();
+	size_t g = 0;
+	do {
+		BPass P;
+		memset(&P, 0, sizeof(P));
+		P.gate_begin = (unsigned)b->gates.size();
+		u64 tmask = 0;
+		while (g < num_gates) {
+			const u64 nm = tmask | (1ull << gl[g].t);
+			if (__builtin_popcountll(nm) > budget)
+				break;
+			tmask = nm;
+			b->gates.push_back(gl[g]);
+			++g;
+		}
+		P.gate_end = (unsigned)b->gates.size();
+		// positions: the targets plus the lowest other qubits
+		u64 pm = tmask;
+		for (unsigned q = 0; q < nq && __builtin_popcountll(pm) < tb; ++q)
+			pm |= 1ull << q;
+		int c = 0;
+		for (unsigned q = 0; q < nq; ++q)
+			if ((pm >> q) & 1) {
+				P.slot_of[q] = (unsigned char)c;
+				P.pos[c++] = (unsigned char)q;
+			}
+		P.npos = (unsigned)c;
+		P.pmask = pm;
+		b->passes.push_back(P);
+	} while (g < num_gates);
+	R.pass_end = (unsigned)b->passes.size();
+	if (index)
+		*index = b->regs.size();
+	b->regs.push_back(R);
+	batch_release_device(b);
+	b->ran = false;
+	return DK_OK;
+}
+
+static int batch_upload(dk_batch *b)
+{
+	const int lb = batch_lb(b->dtype);
+	const size_t asz = dk_amp_size(b->dtype);
+	// pool slots: registers larger than a tile, and every register when the
+	// final states are exported
+	u64 pool_amps = 0;
+	for (auto &R : b->regs) {
+		R.flags = (R.flags & ~BF_EXPORT) | (b->export_states ? BF_EXPORT : 0);
+		const u64 len = 1ull << R.nq;
+		if (R.nq > (unsigned)lb || b->export_states) {
+			R.off = pool_amps;
+			pool_amps += len;
+		} else {
+			R.off = 0;
+		}
+	}
+	std::vector<unsigned> order(b->regs.size());
+	for (size_t i = 0; i < order.size(); ++i)
+		order[i] = (unsigned)i;
+	std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) {
+		return b->regs[x].nq > b->regs[y].nq;
+	});
+	auto al = [](u64 x) { return (x + 255) & ~255ull; };
+	const u64 nr = b->regs.size(), nroll = b->rolls.size();
+	u64 o_regs = 0, o_order = al(o_regs + nr * sizeof(BReg));
+	u64 o_passes = al(o_order + nr * sizeof(unsigned));
+	u64 o_gates = al(o_passes + b->passes.size() * sizeof(BPass));
+	u64 o_rolls = al(o_gates + std::max<size_t>(1, b->gates.size()) * sizeof(BGate));
+	u64 o_out = al(o_rolls + std::max<u64>(1, nroll) * sizeof(double));
+	u64 o_p1 = al(o_out + std::max<u64>(1, nroll));
+	u64 o_dens = al(o_p1 + std::max<u64>(1, nroll) * sizeof(double));
+	u64 o_cnt = al(o_dens + std::max<u64>(1, b->dens_total) * sizeof(double2));
+	const u64 total = o_cnt + 256;
+	int rc = dk_device_alloc(total, (void **)&b->dbuf);
+	if (rc)
+		return rc;
+	b->dbuf_bytes = total;
+	b->pool_bytes = std::max<u64>(256, pool_amps * asz);
+	rc = dk_device_alloc(b->pool_bytes, &b->pool);
+	if (rc) {
+		dk_device_free(b->dbuf, b->dbuf_bytes);
+		b->dbuf = nullptr;
+		return rc;
+	}
+	b->d_regs = (BReg *)(b->dbuf + o_regs);
+	b->d_order = (unsigned *)(b->dbuf + o_order);
+	b->d_passes = (BPass *)(b->dbuf + o_passes);
+	b->d_gates = (BGate *)(b->dbuf + o_gates);
+	b->d_rolls = (double *)(b->dbuf + o_rolls);
+	b->d_out = (signed char *)(b->dbuf + o_out);
+	b->d_p1 = (double *)(b->dbuf + o_p1);
+	b->d_dens = (double2 *)(b->dbuf + o_dens);
+	b->d_counter = (unsigned *)(b->dbuf + o_cnt);
+	hipStream_t st = g_ctx.stream;
+	DK_HIP(hipMemcpyAsync(b->d_regs, b->regs.data(), nr * sizeof(BReg),
+			      hipMemcpyHostToDevice, st));
+	DK_HIP(hipMemcpyAsync(b->d_order, order.data(), nr * sizeof(unsigned),
+			      hipMemcpyHostToDevice, st));
+	DK_HIP(hipMemcpyAsync(b->d_passes, b->passes.data(),
+			      b->passes.size() * sizeof(BPass),
+			      hipMemcpyHostToDevice, st));
+	if (!b->gates.empty())
+		DK_HIP(hipMemcpyAsync(b->d_gates, b->gates.data(),
+				      b->gates.size() * sizeof(BGate),
+				      hipMemcpyHostToDevice, st));
+	if (nroll)
+		DK_HIP(hipMemcpyAsync(b->d_rolls, b->rolls.data(), nroll * sizeof(double),
+				      hipMemcpyHostToDevice, st));
+	DK_HIP(hipStreamSynchronize(st));
+	// one pinned block: p1 (doubles), density (double2), outcomes (bytes)
+	const u64 hb = al(std::max<u64>(1, nroll) * sizeof(double)) +
+		       al(std::max<u64>(1, b->dens_total) * sizeof(double2)) +
+		       al(std::max<u64>(1, nroll));
+	char *hp = nullptr;
+	DK_HIP(hipHostMalloc((void **)&hp, hb, hipHostMallocDefault));
+	memset(hp, 0, hb);
+	b->h_p1 = (double *)hp;
+	b->h_dens = (double2 *)(hp + al(std::max<u64>(1, nroll) * sizeof(double)));
+	b->h_out = (signed char *)(hp + al(std::max<u64>(1, nroll) * sizeof(double)) +
+				   al(std::max<u64>(1, b->dens_total) * sizeof(double2)));
+	b->uploaded = true;
+	return DK_OK;
+}
+
+static int batch_enqueue(dk_batch *b)
+{
+	hipStream_t st = g_ctx.stream;
+	const unsigned nregs = (unsigned)b->regs.size();
+	hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
+	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
+	if (b->dtype == DK_C64)
+		hipLaunchKernelGGL(k_batch<float2>, dim3(grid), dim3(BT), 0, st,
+				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
+				   b->d_rolls, (float2 *)b->pool, b->d_out, b->d_p1,
+				   b->d_dens, b->d_counter);
+	else
+		hipLaunchKernelGGL(k_batch<double2>, dim3(grid), dim3(BT), 0, st,
+				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
+				   b->d_rolls, (double2 *)b->pool, b->d_out, b->d_p1,
+				   b->d_dens, b->d_counter);
+	DK_LAUNCHED();
+	const u64 nroll = b->rolls.size();
+	if (nroll) {
+		DK_HIP(hipMemcpyAsync(b->h_out, b->d_out, nroll,
+				      hipMemcpyDeviceToHost, st));
+		DK_HIP(hipMemcpyAsync(b->h_p1, b->d_p1, nroll * sizeof(double),
+				      hipMemcpyDeviceToHost, st));
+	}
+	if (b->dens_total)
+		DK_HIP(hipMemcpyAsync(b->h_dens, b->d_dens,
+				      b->dens_total * sizeof(double2),
+				      hipMemcpyDeviceToHost, st));
+	return DK_OK;
+}
+
+// Runs every register of the batch in one persistent launch.  With
+// "use_graph" set, the launch sequence is captured once as a HIP graph and
+// replayed on later runs.
+extern "C" int dk_batch_run(dk_batch *b)
+{
+	if (!b) {
+		dk_set_error("NULL batch");
+		return DK_E_ARG;
+	}
+	int rc = dk_ensure_init();
+	if (rc)
+		return rc;
+	if (b->regs.empty()) {
+		b->ran = true;
+		return DK_OK;
+	}
+	if (!b->uploaded) {
+		rc = batch_upload(b);
+		if (rc)
+			return rc;
+	}
+	hipStream_t st = g_ctx.stream;
+	if (b->use_graph) {
+		if (!b->exec) {
+			DK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+			rc = batch_enqueue(b);
+			hipGraph_t gr = nullptr;
+			hipError_t e = hipStreamEndCapture(st, &gr);
+			if (rc)
+				return rc;
+			if (e != hipSuccess) {
+				dk_set_error("graph capture failed: %s", hipGetErrorString(e));
+				return DK_E_DEVICE;
+			}
+			b->graph = gr;
+			DK_HIP(hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0));
+		}
+		DK_HIP(hipGraphLaunch(b->exec, st));
+	} else {
+		rc = batch_enqueue(b);
+		if (rc)
+			return rc;
+	}
+	DK_HIP(hipStreamSynchronize(st));
+	b->ran = true;
+	return DK_OK;
+}
+
+extern "C" int dk_batch_size(const dk_batch *b, size_t *n)
+{
+	if (!b || !n) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	*n = b->regs.size();
+	return DK_OK;
+}
+
+// Outcomes of register i in the order of registry_measure's list (slot s is
+// qubit nq-1-s) and the P1 each was decided on.
+extern "C" int dk_batch_outcomes(const dk_batch *b, size_t i, int8_t *results,
+				 double *p1)
+{
+	if (!b || i >= b->regs.size() || !b->ran) {
+		dk_set_error("batch not run or index out of range");
+		return DK_E_ARG;
+	}
+	const BReg &R = b->regs[i];
+	for (unsigned s = 0; s < R.nq; ++s) {
+		if (results)
+			results[s] = (R.flags & BF_MEASURE) ? b->h_out[R.roll_off + s] : -1;
+		if (p1)
+			p1[s] = b->h_p1[R.roll_off + s];
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_batch_density(const dk_batch *b, size_t i, double *rho)
+{
+	if (!b || i >= b->regs.size() || !b->ran || !rho) {
+		dk_set_error("batch not run or index out of range");
+		return DK_E_ARG;
+	}
+	const BReg &R = b->regs[i];
+	if (!(R.flags & BF_DENSITY)) {
+		dk_set_error("register %zu has no density request", i);
+		return DK_E_ARG;
+	}
+	const unsigned d = 1u << __builtin_popcountll(R.keep);
+	memcpy(rho, b->h_dens + R.dens_off, (size_t)d * d * sizeof(double2));
+	return DK_OK;
+}
+
+// Final (pre-measurement) state of register i, normalised, as complex128.
+// Needs "export_states" set before the run.
+extern "C" int dk_batch_state(const dk_batch *b, size_t i, double *values)
+{
+	if (!b || i >= b->regs.size() || !b->ran || !values || !b->export_states) {
+		dk_set_error("batch not run with export_states, or bad index");
+		return DK_E_ARG;
+	}
+	const BReg &R = b->regs[i];
+	const u64 len = 1ull << R.nq;
+	const size_t asz = dk_amp_size(b->dtype);
+	std::vector<char> raw(len * asz);
+	DK_HIP(hipMemcpy(raw.data(), (char *)b->pool + R.off * asz, len * asz,
+			 hipMemcpyDeviceToHost));
+	double n2 = 0.0;
+	for (u64 k = 0; k < len; ++k) {
+		double re, im;
+		if (b->dtype == DK_C64) {
+			const float *f = (const float *)raw.data();
+			re = f[2 * k];
+			im = f[2 * k + 1];
+		} else {
+			const double *f = (const double *)raw.data();
+			re = f[2 * k];
+			im = f[2 * k + 1];
+		}
+		values[2 * k] = re;
+		values[2 * k + 1] = im;
+		n2 += re * re + im * im;
+	}
+	const double inv = 1.0 / sqrt(n2);
+	for (u64 k = 0; k < 2 * len; ++k)
+		values[k] *= inv;
+	return DK_OK;
+}

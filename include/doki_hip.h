@@ -218,6 +218,54 @@ int dk_circuit_block(const dk_circuit *c, size_t i, unsigned *num_qubits,
  * in_place, later steps run in place.  Works on DK_C128 and DK_C64 states. */
 int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place, dk_state **out);
 
+/* ---- batches of small registers ---------------------------------------------
+ * No single reference counterpart: the reference runs each register through
+ * registry_new / registry_apply / registry_density / registry_measure
+ * (doki.c:302-1192) one call at a time.  A batch holds many independent
+ * registers (1..26 qubits, each starting at |0...0>), each with a list of
+ * (anti)controlled 1-qubit gates, an optional reduced density matrix of at
+ * most 6 kept qubits (taken after the gates) and an optional measurement of
+ * every qubit with injected rolls; dk_batch_run executes all of them in one
+ * persistent kernel launch. */
+typedef struct dk_batch dk_batch;
+
+int dk_batch_new(int dtype, dk_batch **out);
+void dk_batch_free(dk_batch *b);
+
+/* Settings: "export_states" (keep final states readable by dk_batch_state),
+ * "use_graph" (capture the launch as a HIP graph, replay on later runs),
+ * "pass_qubits" (target qubits per cache-blocked pass, 0 = tile width). */
+int dk_batch_set(dk_batch *b, const char *key, int64_t value);
+
+/* Adds a register.  matrices: num_gates 2x2 gates, 8 doubles each
+ * (row-major, interleaved); targets/ctrl_masks/actrl_masks per gate, with
+ * registry_apply's meaning (masks may be NULL).  rolls: num_qubits rolls in
+ * [0, 1) consumed from the most significant qubit down, or NULL for no
+ * measurement.  density_keep: kept qubits of the reduced density (0 = none).
+ * *index (may be NULL) receives the register's position. */
+int dk_batch_add(dk_batch *b, unsigned num_qubits, size_t num_gates,
+		 const double *matrices, const unsigned *targets,
+		 const uint64_t *ctrl_masks, const uint64_t *actrl_masks,
+		 const double *rolls, uint64_t density_keep, size_t *index);
+
+/* Runs every register of the batch in one persistent launch. */
+int dk_batch_run(dk_batch *b);
+
+int dk_batch_size(const dk_batch *b, size_t *num_registers);
+
+/* Outcomes of register i in registry_measure's list order (slot s is qubit
+ * num_qubits-1-s; -1 when not measured) and the P1 each was decided on. */
+int dk_batch_outcomes(const dk_batch *b, size_t i, int8_t *results,
+		      double *p1);
+
+/* Reduced density of register i: 2^a x 2^a (a = kept qubits), row-major,
+ * interleaved; kept qubits in ascending order are the index bits. */
+int dk_batch_density(const dk_batch *b, size_t i, double *rho);
+
+/* Final normalised state of register i as complex128 (needs
+ * "export_states"). */
+int dk_batch_state(const dk_batch *b, size_t i, double *values);
+
 /* ---- timing, tuning, device ------------------------------------------------
  * No reference counterparts. */
 

@@ -1,0 +1,91 @@
+"""The batched engine (dk_batch_*): many registers of uneven size in one
+persistent launch, checked register by register against the oracle running
+the same gates, density requests and rolls.  Complex128: states and
+densities within 1e-12, outcomes identical.  Complex64: states within 5e-6,
+outcomes identical wherever no roll lies within 1e-4 of its P1."""
+import numpy as np
+import pytest
+
+from doki_amd import workloads as W
+from helpers import ATOL, oracle_run, reduced_density_einsum
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch_specs(seed, count, nmin=4, nmax=16):
+    rng = W.rng_for(seed)
+    specs = []
+    for i in range(count):
+        # log-uniform qubit count (p ~ 1/n), then the mixed random circuit
+        ns = np.arange(nmin, nmax + 1)
+        p = 1.0 / ns
+        n = int(rng.choice(ns, p=p / p.sum()))
+        depth = int(rng.integers(10, 61))
+        steps, rolls = W.random_circuit(seed * 1000 + i, n, depth)
+        specs.append((n, steps, rolls))
+    return specs
+
+
+def check_batch(capi, oracle, dtype, specs, tol, **settings):
+    b = capi.Batch(dtype, export_states=1, **settings)
+    keeps = []
+    for n, steps, rolls in specs:
+        keep = tuple(range(min(6, n))) if n == 14 else ()
+        keeps.append(keep)
+        b.add(n, steps, rolls, keep)
+    b.run()
+    for i, (n, steps, rolls) in enumerate(specs):
+        st = oracle_run(oracle, oracle.new(n), steps)
+        want = st.values()
+        got = b.state(i)
+        assert np.max(np.abs(got - want)) <= tol, (i, n)
+        outs, p1 = b.outcomes(i)
+        rc, left, olst = oracle.measure(st, (1 << n) - 1, rolls)
+        if dtype == capi.C128 or min(abs(p1[k] - rolls[k]) for k in range(n)) > 1e-4:
+            assert outs == olst, (i, n)
+        if keeps[i]:
+            rho = b.density(i)
+            assert np.max(np.abs(rho - reduced_density_einsum(want, keeps[i]))) <= \
+                max(tol, ATOL)
+    return b
+
+
+def test_batch_c128_mixed_sizes(capi, oracle):
+    specs = make_batch_specs(11, 70)
+    specs += [(14, *W.random_circuit(5001, 14, 40)), (13, *W.random_circuit(5002, 13, 30)),
+              (16, *W.random_circuit(5003, 16, 25))]
+    check_batch(capi, oracle, capi.C128, specs, ATOL)
+
+
+def test_batch_c128_small_passes(capi, oracle):
+    # few target qubits per pass: many cache-blocked passes per register
+    specs = [(15, *W.random_circuit(6000 + i, 15, 30)) for i in range(3)]
+    specs += [(9, *W.random_circuit(6100, 9, 30))]
+    check_batch(capi, oracle, capi.C128, specs, ATOL, pass_qubits=3)
+
+
+def test_batch_c64(capi, oracle):
+    specs = make_batch_specs(12, 40, 4, 15)
+    specs += [(14, *W.random_circuit(7001, 14, 40)), (15, *W.random_circuit(7002, 15, 30))]
+    check_batch(capi, oracle, capi.C64, specs, 5e-6)
+
+
+def test_batch_graph_replays_match(capi, oracle):
+    specs = make_batch_specs(13, 30, 4, 14)
+    ref = capi.Batch(capi.C128)
+    g = capi.Batch(capi.C128, use_graph=1)
+    for n, steps, rolls in specs:
+        ref.add(n, steps, rolls)
+        g.add(n, steps, rolls)
+    ref.run()
+    want = [ref.outcomes(i) for i in range(len(specs))]
+    for rep in range(3):
+        g.run()
+        for i in range(len(specs)):
+            outs, p1 = g.outcomes(i)
+            assert outs == want[i][0]
+            assert np.array_equal(p1, want[i][1])
+        # eager work on the same stream between replays
+        s = capi.new(12)
+        capi.apply(s, capi.Gate(W.gate_h()), [3], in_place=True)
+        capi.synchronize()
