@@ -171,6 +171,86 @@ def fused_leg(args):
     return out
 
 
+def batch_specs(seed, count, nmin=4, nmax=20):
+    """Registers with qubit counts log-uniform (p ~ 1/n) on [nmin, nmax],
+    each a mixed random circuit of depth U[10, 100] with rolls for a full
+    measurement; members of 14 qubits also ask for a 6-qubit density."""
+    import numpy as np
+
+    from doki_amd import workloads as W
+    rng = W.rng_for(seed)
+    ns = np.arange(nmin, nmax + 1)
+    p = 1.0 / ns
+    p /= p.sum()
+    specs = []
+    for i in range(count):
+        n = int(rng.choice(ns, p=p))
+        depth = int(rng.integers(10, 101))
+        steps, rolls = W.random_circuit(seed * 100003 + i, n, depth)
+        keep = tuple(range(6)) if n == 14 else ()
+        specs.append((n, steps, rolls, keep))
+    return specs
+
+
+def batch_leg(args):
+    """Many small registers: one persistent launch (graph replay) against
+    plain per-gate execution of the same registers one at a time."""
+    import numpy as np
+
+    from doki_amd import capi
+    t0 = time.perf_counter()
+    specs = batch_specs(31, args.batch_registers)
+    log(f"[bench] batch: {len(specs)} registers generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = {"registers": len(specs),
+           "gates": int(sum(len(s[1]) for s in specs)),
+           "qubits_hist": {int(n): int(c) for n, c in
+                           zip(*np.unique([s[0] for s in specs], return_counts=True))}}
+    for dtype, name in ((capi.C128, "c128"), (capi.C64, "c64")):
+        b = capi.Batch(dtype, use_graph=1)
+        for n, steps, rolls, keep in specs:
+            b.add(n, steps, rolls, keep)
+        b.run()                          # upload + capture
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            b.run()
+            ts.append(time.perf_counter() - t1)
+        t = min(ts)
+        out[name] = {"seconds": round(t, 4),
+                     "circuits_per_s": round(len(specs) / t, 1)}
+        log(f"[bench] batch {name}: {out[name]}")
+        del b
+    # plain: each register through the per-gate C ABI path, one at a time
+    rng = np.random.default_rng(5)
+    sample = rng.choice(len(specs), size=min(len(specs), args.batch_plain_sample),
+                        replace=False)
+    gate_cache = {}
+    t1 = time.perf_counter()
+    for i in sample:
+        n, steps, rolls, keep = specs[i]
+        s = capi.new(n)
+        for m, t, c, a in steps:
+            key = m.tobytes()
+            g = gate_cache.get(key)
+            if g is None:
+                g = gate_cache[key] = capi.Gate(m)
+            capi.apply(s, g, t, c, a, in_place=True)
+        if keep:
+            capi.reduced_density(s, keep)
+        capi.measure(s, (1 << n) - 1, rolls)
+        del s
+    capi.synchronize()
+    tp = time.perf_counter() - t1
+    per = tp / len(sample)
+    out["plain_c128"] = {"sample": int(len(sample)), "seconds_per_register": round(per, 6),
+                         "circuits_per_s": round(1.0 / per, 1)}
+    out["batched_speedup_vs_plain_c128"] = round(
+        out["c128"]["circuits_per_s"] / out["plain_c128"]["circuits_per_s"], 1)
+    log(f"[bench] batch plain: {out['plain_c128']}")
+    return out
+
+
 def cpu_leg(args, gb_per_gate_unit=True):
     """The reference's own CPU path (oracle/_ref, compiled from its sources),
     OpenMP on the host cores, over a bounded sample of the same workload:
@@ -228,6 +308,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--fused-depth", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-registers", type=int, default=4096)
+    ap.add_argument("--batch-plain-sample", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -239,6 +322,7 @@ def main():
 
     g = gpu_leg(args)
     fused = None if args.no_fused else fused_leg(args)
+    batched = None if args.no_batch else batch_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
     if cpu:
         log(f"[bench] cpu baseline {cpu}")
@@ -279,6 +363,7 @@ def main():
             "per_target_kernel": g["per_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "fused": fused,
+            "batched": batched,
         },
     }
     print(json.dumps(out), flush=True)
