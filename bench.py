@@ -251,6 +251,79 @@ def batch_leg(args):
     return out
 
 
+def serve_leg(args):
+    """Served mix (Poisson arrivals, 8-30 qubits, prob/measure/density/join):
+    the continuous-batching server against plain per-request execution of the
+    same arrival trace."""
+    import numpy as np
+
+    from doki_amd import capi, serve
+    from doki_amd import workloads as W
+    specs = W.serving_requests(41, args.serve_requests, args.serve_qps)
+    for s in specs:
+        # the client sends its gates packed
+        s["packed"] = [capi.pack_steps(st) for _, st in s["regs"]]
+    # warm-up outside the trace: one batched and one lane request
+    warm = W.serving_requests(42, 40, 1000.0, nmax=20, depth=(20, 40))
+    with serve.Server(workers=args.serve_workers) as srv:
+        for h in [srv.submit(s) for s in warm]:
+            h._ev.wait()
+    out = {"requests": len(specs), "qps_offered": args.serve_qps,
+           "ops": {op: sum(1 for s in specs if s["op"] == op) for op in W.SERVE_OPS},
+           "qubits_max": int(max(s["regs"][0][0] for s in specs))}
+
+    def served(time_scale):
+        with serve.Server(workers=args.serve_workers) as srv:
+            hs, t0 = serve.serve_trace(srv, specs, time_scale)
+            batches, batched = srv.batches, srv.batched
+        ok = [h for h in hs if h.error is None]
+        rej = [h for h in hs if isinstance(h.error, serve.Rejected)]
+        failed = [h for h in hs if h.error is not None and h not in rej]
+        if failed:
+            raise failed[0].error
+        lat = [h.t_done - (t0 + h.spec["arrival"] * time_scale) for h in ok]
+        span = max(h.t_done for h in hs) - t0
+        return hs, {"completed": len(ok), "rejected_over_budget": len(rej),
+                    "seconds": round(span, 3),
+                    "throughput_rps": round(len(ok) / span, 2),
+                    "batch_launches": batches, "batched_requests": batched,
+                    "workers": args.serve_workers, **serve.latency_stats(lat)}
+
+    def plain_run(time_scale):
+        res, span = serve.run_plain(specs, time_scale=time_scale)
+        pl = [lt for r, lt in res if not isinstance(r, Exception)]
+        return res, {"completed": len(pl), "seconds": round(span, 3),
+                     "throughput_rps": round(len(pl) / span, 2),
+                     **serve.latency_stats(pl)}
+
+    # at the offered rate: latency; all requests at once: capacity
+    hs, out["served"] = served(1.0)
+    log(f"[bench] serve: {out['served']}")
+    plain_res, out["plain"] = plain_run(1.0)
+    log(f"[bench] serve plain: {out['plain']}")
+    _, out["served_burst"] = served(0.0)
+    log(f"[bench] serve burst: {out['served_burst']}")
+    _, out["plain_burst"] = plain_run(0.0)
+    log(f"[bench] serve plain burst: {out['plain_burst']}")
+    out["speedup_p99"] = round(out["plain"]["p99_ms"] / out["served"]["p99_ms"], 2)
+    out["speedup_capacity"] = round(out["served_burst"]["throughput_rps"] /
+                                    out["plain_burst"]["throughput_rps"], 2)
+    # agreement of the two paths on every completed request
+    worst = 0.0
+    flips = 0
+    for h, (res, _) in zip(hs, plain_res):
+        if h.error is not None or isinstance(res, Exception):
+            continue
+        for k in ("probs", "rho"):
+            if k in res:
+                worst = max(worst, float(np.max(np.abs(h.result[k] - res[k]))))
+        if "outcomes" in res and res["outcomes"] != h.result["outcomes"]:
+            flips += 1
+    out["served_vs_plain_max_abs"] = worst
+    out["served_vs_plain_outcome_mismatches"] = flips
+    return out
+
+
 def cpu_leg(args, gb_per_gate_unit=True):
     """The reference's own CPU path (oracle/_ref, compiled from its sources),
     OpenMP on the host cores, over a bounded sample of the same workload:
@@ -311,6 +384,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-registers", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-qps", type=float, default=60.0)
+    ap.add_argument("--serve-workers", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -323,6 +400,7 @@ def main():
     g = gpu_leg(args)
     fused = None if args.no_fused else fused_leg(args)
     batched = None if args.no_batch else batch_leg(args)
+    served = None if args.no_serve else serve_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
     if cpu:
         log(f"[bench] cpu baseline {cpu}")
@@ -364,6 +442,7 @@ def main():
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "fused": fused,
             "batched": batched,
+            "served": served,
         },
     }
     print(json.dumps(out), flush=True)

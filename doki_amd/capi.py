@@ -95,6 +95,13 @@ def lib():
         "dk_batch_outcomes": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_int8), dp]),
         "dk_batch_density": (C.c_int, [P, C.c_size_t, dp]),
         "dk_batch_state": (C.c_int, [P, C.c_size_t, dp]),
+        "dk_batch_probs": (C.c_int, [P, C.c_size_t, dp]),
+        "dk_batch_clear": (C.c_int, [P]),
+        "dk_memory_stats": (C.c_int, [C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+        "dk_memory_trim": (C.c_int, []),
+        "dk_lane_new": (C.c_int, [PP]),
+        "dk_lane_bind": (C.c_int, [P]),
+        "dk_lane_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -297,6 +304,18 @@ class Circuit:
                                    _dp(m.view(np.float64).ravel()), t,
                                    mask(controls), mask(anticontrols)))
 
+    def add_packed(self, packed):
+        """1-qubit gates packed as by pack_steps."""
+        mf, tg, cm, am, ng = packed
+        L = lib()
+        dp = C.POINTER(C.c_double)
+        base = mf.ctypes.data
+        t = (C.c_uint * 1)()
+        for g in range(ng):
+            t[0] = int(tg[g])
+            check(L.dk_circuit_add(self.ptr, 1, C.cast(base + 64 * g, dp), t,
+                                   int(cm[g]), int(am[g])))
+
     def fuse(self, k):
         n = C.c_size_t()
         check(lib().dk_circuit_fuse(self.ptr, k, C.byref(n)))
@@ -315,6 +334,24 @@ class Circuit:
         p = C.c_void_p()
         check(lib().dk_circuit_run(self.ptr, s.ptr, int(in_place), C.byref(p)))
         return s if in_place else State(p.value)
+
+
+def pack_steps(steps):
+    """(2x2 matrix, [target], controls, anticontrols) tuples -> the packed
+    arrays dk_batch_add takes."""
+    ng = len(steps)
+    mats = np.zeros((max(ng, 1), 4), dtype=np.complex128)
+    tg = np.zeros(max(ng, 1), dtype=np.uint32)
+    cm = np.zeros(max(ng, 1), dtype=np.uint64)
+    am = np.zeros(max(ng, 1), dtype=np.uint64)
+    for i, (m, t, c, a) in enumerate(steps):
+        if len(t) != 1:
+            raise ValueError("batched registers take 1-qubit gates")
+        mats[i] = np.asarray(m, dtype=np.complex128).ravel()
+        tg[i] = t[0]
+        cm[i] = mask(c)
+        am[i] = mask(a)
+    return np.ascontiguousarray(mats.view(np.float64)), tg, cm, am, ng
 
 
 class Batch:
@@ -336,19 +373,13 @@ class Batch:
 
     def add(self, nq, steps, rolls=None, density_keep=()):
         """steps: (2x2 matrix, [target], controls, anticontrols) tuples."""
-        ng = len(steps)
-        mats = np.zeros((max(ng, 1), 4), dtype=np.complex128)
-        tg = np.zeros(max(ng, 1), dtype=np.uint32)
-        cm = np.zeros(max(ng, 1), dtype=np.uint64)
-        am = np.zeros(max(ng, 1), dtype=np.uint64)
-        for i, (m, t, c, a) in enumerate(steps):
-            if len(t) != 1:
-                raise ValueError("batched registers take 1-qubit gates")
-            mats[i] = np.asarray(m, dtype=np.complex128).ravel()
-            tg[i] = t[0]
-            cm[i] = mask(c)
-            am[i] = mask(a)
-        mf = np.ascontiguousarray(mats.view(np.float64))
+        return self.add_packed(nq, pack_steps(steps), rolls, mask(density_keep))
+
+    def add_packed(self, nq, packed, rolls=None, keep_mask=0):
+        """packed: (matrices as float64 [g, 8], targets uint32 [g],
+        control masks uint64 [g], anticontrol masks uint64 [g], g); the
+        arrays hold at least one row."""
+        mf, tg, cm, am, ng = packed
         r = None
         if rolls is not None:
             r = np.ascontiguousarray(np.asarray(rolls, dtype=np.float64))
@@ -358,13 +389,23 @@ class Batch:
         check(lib().dk_batch_add(
             self.ptr, nq, ng, _dp(mf), tg.ctypes.data_as(C.POINTER(C.c_uint)),
             cm.ctypes.data_as(u), am.ctypes.data_as(u),
-            _dp(r) if r is not None else None, mask(density_keep), C.byref(idx)))
+            _dp(r) if r is not None else None, int(keep_mask), C.byref(idx)))
         self.nqs.append(nq)
-        self.keeps.append(len(density_keep))
+        self.keeps.append(bin(int(keep_mask)).count("1"))
         return idx.value
+
+    def clear(self):
+        check(lib().dk_batch_clear(self.ptr))
+        self.nqs = []
+        self.keeps = []
 
     def run(self):
         check(lib().dk_batch_run(self.ptr))
+
+    def probs(self, i):
+        out = np.empty(self.nqs[i])
+        check(lib().dk_batch_probs(self.ptr, i, _dp(out)))
+        return out
 
     def outcomes(self, i):
         n = self.nqs[i]
@@ -387,6 +428,40 @@ class Batch:
 
 def synchronize():
     check(lib().dk_synchronize())
+
+
+def memory_stats():
+    a, f, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib().dk_memory_stats(C.byref(a), C.byref(f), C.byref(c)))
+    return {"arena_chunks": a.value, "arena_free_chunks": f.value,
+            "cached_bytes": c.value}
+
+
+def memory_trim():
+    check(lib().dk_memory_trim())
+
+
+class Lane:
+    """An execution lane (HIP stream plus scratch).  bind() makes it the
+    calling thread's lane; ctypes calls release the GIL, so threads bound to
+    different lanes keep the device busy concurrently."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().dk_lane_new(C.byref(p)))
+        self.ptr = p
+
+    def bind(self):
+        check(lib().dk_lane_bind(self.ptr))
+
+    @staticmethod
+    def unbind():
+        check(lib().dk_lane_bind(None))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and _lib is not None:
+            _lib.dk_lane_free(self.ptr)
+            self.ptr = None
 
 
 def event_record(slot):

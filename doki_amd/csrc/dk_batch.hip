@@ -2,6 +2,9 @@
 // registers of uneven size, each with its own gate list, an optional reduced
 // density matrix and a full measurement with injected rolls.
 //
+// Optionally every register also reports the marginal P(qubit q = 1) of its
+// final state for every q (registry_prob, qops.c:72-95), in FP64.
+//
 // Work distribution: workgroups take registers from an atomic counter,
 // largest first.  A register of up to 2^LB amplitudes (LB = 12 for
 // complex128, 13 for complex64) lives entirely in LDS.  Larger registers live
@@ -37,7 +40,7 @@ struct BReg {
 	unsigned pass_begin, pass_end;
 	unsigned roll_off;  // first roll; also first outcome slot
 	unsigned dens_off;  // first element of its density output
-	unsigned flags;     // 1 measure, 2 density, 4 export final state
+	unsigned flags;     // 1 measure, 2 density, 4 export, 8 marginal probs
 };
 
 struct BPass {
@@ -55,7 +58,7 @@ struct BGate {
 	unsigned pad;
 };
 
-enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4 };
+enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4, BF_PROBS = 8 };
 
 __device__ __forceinline__ double bsum(double v, double *red)
 {
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(BT) void k_batch(
 	const BGate *__restrict__ gates, const double *__restrict__ rolls,
 	T *__restrict__ pool, signed char *__restrict__ outcomes,
 	double *__restrict__ p1s, double2 *__restrict__ dens,
-	unsigned *__restrict__ counter)
+	double *__restrict__ probs, unsigned *__restrict__ counter)
 {
 	constexpr int LT = LDS_BYTES / (int)sizeof(T);
 	constexpr int LB = sizeof(T) == 16 ? 12 : 13;
@@ -212,6 +215,19 @@ __global__ __launch_bounds__(BT) void k_batch(
 		const double norm2 = bsum(acc, red);
 		const double inv = 1.0 / sqrt(norm2);
 
+		if (R.flags & BF_PROBS) {
+			for (unsigned q = 0; q < n; ++q) {
+				const u64 half = len >> 1;
+				double acc1 = 0.0;
+				for (u64 i = threadIdx.x; i < half; i += BT)
+					acc1 += cabs2d(amp_at(tile, st, resident,
+							      ins0(i, q) | (1ull << q)));
+				const double s1 = bsum(acc1, red);
+				if (threadIdx.x == 0)
+					probs[R.roll_off + q] = s1 / norm2;
+			}
+		}
+
 		if ((R.flags & BF_EXPORT) && resident) {
 			for (u64 i = threadIdx.x; i < len; i += BT)
 				st[i] = tile[lds_pos<T>((int)i)];
@@ -294,12 +310,15 @@ struct dk_batch {
 	int export_states = 0;
 	int use_graph = 0;
 	int pass_qubits = 0; // 0 = tile size
-	// device side
+	int probs = 0;       // marginal probabilities of every register
+	// device side; buffers are kept (and grown) across uploads
 	bool uploaded = false;
 	char *dbuf = nullptr;
 	u64 dbuf_bytes = 0;
 	void *pool = nullptr;
 	u64 pool_bytes = 0;
+	char *hbuf = nullptr;
+	u64 hbuf_bytes = 0;
 	BReg *d_regs = nullptr;
 	unsigned *d_order = nullptr;
 	BPass *d_passes = nullptr;
@@ -307,12 +326,14 @@ struct dk_batch {
 	double *d_rolls = nullptr;
 	signed char *d_out = nullptr;
 	double *d_p1 = nullptr;
+	double *d_probs = nullptr;
 	double2 *d_dens = nullptr;
 	unsigned *d_counter = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	signed char *h_out = nullptr;  // pinned, so graph replays can copy
 	double *h_p1 = nullptr;
+	double *h_probs = nullptr;
 	double2 *h_dens = nullptr;
 	bool ran = false;
 };
@@ -322,7 +343,8 @@ static int batch_lb(int dtype)
 	return dtype == DK_C64 ? 13 : 12;
 }
 
-static void batch_release_device(dk_batch *b)
+// The uploaded layout (and a captured graph) no longer match the batch.
+static void batch_invalidate(dk_batch *b)
 {
 	if (b->exec)
 		(void)hipGraphExecDestroy(b->exec);
@@ -330,18 +352,24 @@ static void batch_release_device(dk_batch *b)
 		(void)hipGraphDestroy(b->graph);
 	b->exec = nullptr;
 	b->graph = nullptr;
+	b->uploaded = false;
+}
+
+static void batch_release_device(dk_batch *b)
+{
+	batch_invalidate(b);
 	if (b->dbuf)
 		dk_device_free(b->dbuf, b->dbuf_bytes);
 	if (b->pool)
 		dk_device_free(b->pool, b->pool_bytes);
-	if (b->h_p1)
-		(void)hipHostFree(b->h_p1);
-	b->h_out = nullptr;
-	b->h_p1 = nullptr;
-	b->h_dens = nullptr;
+	if (b->hbuf)
+		(void)hipHostFree(b->hbuf);
+	b->hbuf = nullptr;
+	b->hbuf_bytes = 0;
 	b->dbuf = nullptr;
+	b->dbuf_bytes = 0;
 	b->pool = nullptr;
-	b->uploaded = false;
+	b->pool_bytes = 0;
 }
 
 extern "C" int dk_batch_new(int dtype, dk_batch **out)
@@ -376,11 +404,30 @@ extern "C" int dk_batch_set(dk_batch *b, const char *key, int64_t value)
 		b->use_graph = value != 0;
 	else if (!strcmp(key, "pass_qubits"))
 		b->pass_qubits = (int)value;
+	else if (!strcmp(key, "probs"))
+		b->probs = value != 0;
 	else {
 		dk_set_error("unknown batch setting %s", key);
 		return DK_E_ARG;
 	}
-	batch_release_device(b);
+	batch_invalidate(b);
+	return DK_OK;
+}
+
+// Drops every register; device and pinned buffers stay for the next fill.
+extern "C" int dk_batch_clear(dk_batch *b)
+{
+	if (!b) {
+		dk_set_error("NULL batch");
+		return DK_E_ARG;
+	}
+	b->regs.clear();
+	b->passes.clear();
+	b->gates.clear();
+	b->rolls.clear();
+	b->dens_total = 0;
+	b->ran = false;
+	batch_invalidate(b);
 	return DK_OK;
 }
 
@@ -471,7 +518,7 @@ extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
 	if (index)
 		*index = b->regs.size();
 	b->regs.push_back(R);
-	batch_release_device(b);
+	batch_invalidate(b);
 	b->ran = false;
 	return DK_OK;
 }
@@ -484,7 +531,8 @@ static int batch_upload(dk_batch *b)
 	// final states are exported
 	u64 pool_amps = 0;
 	for (auto &R : b->regs) {
-		R.flags = (R.flags & ~BF_EXPORT) | (b->export_states ? BF_EXPORT : 0);
+		R.flags = (R.flags & ~(BF_EXPORT | BF_PROBS)) |
+			  (b->export_states ? BF_EXPORT : 0) | (b->probs ? BF_PROBS : 0);
 		const u64 len = 1ull << R.nq;
 		if (R.nq > (unsigned)lb || b->export_states) {
 			R.off = pool_amps;
@@ -507,19 +555,30 @@ static int batch_upload(dk_batch *b)
 	u64 o_rolls = al(o_gates + std::max<size_t>(1, b->gates.size()) * sizeof(BGate));
 	u64 o_out = al(o_rolls + std::max<u64>(1, nroll) * sizeof(double));
 	u64 o_p1 = al(o_out + std::max<u64>(1, nroll));
-	u64 o_dens = al(o_p1 + std::max<u64>(1, nroll) * sizeof(double));
+	u64 o_probs = al(o_p1 + std::max<u64>(1, nroll) * sizeof(double));
+	u64 o_dens = al(o_probs + std::max<u64>(1, nroll) * sizeof(double));
 	u64 o_cnt = al(o_dens + std::max<u64>(1, b->dens_total) * sizeof(double2));
 	const u64 total = o_cnt + 256;
-	int rc = dk_device_alloc(total, (void **)&b->dbuf);
-	if (rc)
-		return rc;
-	b->dbuf_bytes = total;
-	b->pool_bytes = std::max<u64>(256, pool_amps * asz);
-	rc = dk_device_alloc(b->pool_bytes, &b->pool);
-	if (rc) {
-		dk_device_free(b->dbuf, b->dbuf_bytes);
+	if (b->dbuf_bytes < total) {
+		if (b->dbuf)
+			dk_device_free(b->dbuf, b->dbuf_bytes);
 		b->dbuf = nullptr;
-		return rc;
+		b->dbuf_bytes = 0;
+		int rc = dk_device_alloc(total, (void **)&b->dbuf);
+		if (rc)
+			return rc;
+		b->dbuf_bytes = total;
+	}
+	const u64 pool_need = std::max<u64>(256, pool_amps * asz);
+	if (b->pool_bytes < pool_need) {
+		if (b->pool)
+			dk_device_free(b->pool, b->pool_bytes);
+		b->pool = nullptr;
+		b->pool_bytes = 0;
+		int rc = dk_device_alloc(pool_need, &b->pool);
+		if (rc)
+			return rc;
+		b->pool_bytes = pool_need;
 	}
 	b->d_regs = (BReg *)(b->dbuf + o_regs);
 	b->d_order = (unsigned *)(b->dbuf + o_order);
@@ -528,9 +587,10 @@ static int batch_upload(dk_batch *b)
 	b->d_rolls = (double *)(b->dbuf + o_rolls);
 	b->d_out = (signed char *)(b->dbuf + o_out);
 	b->d_p1 = (double *)(b->dbuf + o_p1);
+	b->d_probs = (double *)(b->dbuf + o_probs);
 	b->d_dens = (double2 *)(b->dbuf + o_dens);
 	b->d_counter = (unsigned *)(b->dbuf + o_cnt);
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	DK_HIP(hipMemcpyAsync(b->d_regs, b->regs.data(), nr * sizeof(BReg),
 			      hipMemcpyHostToDevice, st));
 	DK_HIP(hipMemcpyAsync(b->d_order, order.data(), nr * sizeof(unsigned),
@@ -546,24 +606,32 @@ static int batch_upload(dk_batch *b)
 		DK_HIP(hipMemcpyAsync(b->d_rolls, b->rolls.data(), nroll * sizeof(double),
 				      hipMemcpyHostToDevice, st));
 	DK_HIP(hipStreamSynchronize(st));
-	// one pinned block: p1 (doubles), density (double2), outcomes (bytes)
-	const u64 hb = al(std::max<u64>(1, nroll) * sizeof(double)) +
-		       al(std::max<u64>(1, b->dens_total) * sizeof(double2)) +
-		       al(std::max<u64>(1, nroll));
-	char *hp = nullptr;
-	DK_HIP(hipHostMalloc((void **)&hp, hb, hipHostMallocDefault));
+	// one pinned block: p1 and probs (doubles), density (double2),
+	// outcomes (bytes)
+	const u64 h_roll = al(std::max<u64>(1, nroll) * sizeof(double));
+	const u64 h_dens = al(std::max<u64>(1, b->dens_total) * sizeof(double2));
+	const u64 hb = 2 * h_roll + h_dens + al(std::max<u64>(1, nroll));
+	if (b->hbuf_bytes < hb) {
+		if (b->hbuf)
+			(void)hipHostFree(b->hbuf);
+		b->hbuf = nullptr;
+		b->hbuf_bytes = 0;
+		DK_HIP(hipHostMalloc((void **)&b->hbuf, hb, hipHostMallocDefault));
+		b->hbuf_bytes = hb;
+	}
+	char *hp = b->hbuf;
 	memset(hp, 0, hb);
 	b->h_p1 = (double *)hp;
-	b->h_dens = (double2 *)(hp + al(std::max<u64>(1, nroll) * sizeof(double)));
-	b->h_out = (signed char *)(hp + al(std::max<u64>(1, nroll) * sizeof(double)) +
-				   al(std::max<u64>(1, b->dens_total) * sizeof(double2)));
+	b->h_probs = (double *)(hp + h_roll);
+	b->h_dens = (double2 *)(hp + 2 * h_roll);
+	b->h_out = (signed char *)(hp + 2 * h_roll + h_dens);
 	b->uploaded = true;
 	return DK_OK;
 }
 
 static int batch_enqueue(dk_batch *b)
 {
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	const unsigned nregs = (unsigned)b->regs.size();
 	hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
 	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
@@ -571,12 +639,12 @@ static int batch_enqueue(dk_batch *b)
 		hipLaunchKernelGGL(k_batch<float2>, dim3(grid), dim3(BT), 0, st,
 				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
 				   b->d_rolls, (float2 *)b->pool, b->d_out, b->d_p1,
-				   b->d_dens, b->d_counter);
+				   b->d_dens, b->d_probs, b->d_counter);
 	else
 		hipLaunchKernelGGL(k_batch<double2>, dim3(grid), dim3(BT), 0, st,
 				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
 				   b->d_rolls, (double2 *)b->pool, b->d_out, b->d_p1,
-				   b->d_dens, b->d_counter);
+				   b->d_dens, b->d_probs, b->d_counter);
 	DK_LAUNCHED();
 	const u64 nroll = b->rolls.size();
 	if (nroll) {
@@ -585,6 +653,9 @@ static int batch_enqueue(dk_batch *b)
 		DK_HIP(hipMemcpyAsync(b->h_p1, b->d_p1, nroll * sizeof(double),
 				      hipMemcpyDeviceToHost, st));
 	}
+	if (nroll && b->probs)
+		DK_HIP(hipMemcpyAsync(b->h_probs, b->d_probs, nroll * sizeof(double),
+				      hipMemcpyDeviceToHost, st));
 	if (b->dens_total)
 		DK_HIP(hipMemcpyAsync(b->h_dens, b->d_dens,
 				      b->dens_total * sizeof(double2),
@@ -613,7 +684,7 @@ extern "C" int dk_batch_run(dk_batch *b)
 		if (rc)
 			return rc;
 	}
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	if (b->use_graph) {
 		if (!b->exec) {
 			DK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -666,6 +737,19 @@ extern "C" int dk_batch_outcomes(const dk_batch *b, size_t i, int8_t *results,
 		if (p1)
 			p1[s] = b->h_p1[R.roll_off + s];
 	}
+	return DK_OK;
+}
+
+// Marginal P(qubit q = 1) of register i's final state, q = 0..nq-1.  Needs
+// "probs" set before the run.
+extern "C" int dk_batch_probs(const dk_batch *b, size_t i, double *p)
+{
+	if (!b || i >= b->regs.size() || !b->ran || !p || !b->probs) {
+		dk_set_error("batch not run with probs, or bad index");
+		return DK_E_ARG;
+	}
+	const BReg &R = b->regs[i];
+	memcpy(p, b->h_probs + R.roll_off, R.nq * sizeof(double));
 	return DK_OK;
 }
 

@@ -307,6 +307,7 @@ extern "C" int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place,
 	int rc = dk_ensure_init();
 	if (rc)
 		return rc;
+	DK_SETTLE(st);
 	dk_state *cur = st;
 	bool first = !in_place;
 	auto step = [&](const dk_gate *g, const unsigned *t, u64 cm, u64 am) {
@@ -322,6 +323,7 @@ extern "C" int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place,
 				dk_release_state(o);
 			return r;
 		}
+		o->lane = dk_cur();
 		cur = o;
 		first = false;
 		return (int)DK_OK;

@@ -162,7 +162,7 @@ int launch_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 						      : (const T *)g->dmat32;
 	const T *src = (const T *)in->amp;
 	T *dst = (T *)out->amp;
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	const dim3 grid((unsigned)nblocks);
 	switch (g->k) {
 	case 4:

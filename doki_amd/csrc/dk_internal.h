@@ -25,6 +25,13 @@ void dk_set_error(const char *fmt, ...);
 		}                                                           \
 	} while (0)
 
+#define DK_TRY(call)                                                        \
+	do {                                                                \
+		int rc_ = (call);                                           \
+		if (rc_)                                                    \
+			return rc_;                                         \
+	} while (0)
+
 #define DK_LAUNCHED()                                                       \
 	do {                                                                \
 		hipError_t e_ = hipGetLastError();                          \
@@ -41,11 +48,16 @@ void dk_set_error(const char *fmt, ...);
 //   [0] norm constant (reads return raw / norm)
 enum { DK_SCAL_NORM = 0, DK_SCAL_BYTES = 256 };
 
+struct dk_lane;
+
 struct dk_state {
 	unsigned nq;
 	int dtype;      // DK_C128 or DK_C64
+	dk_lane *lane;  // lane whose stream last wrote the state
 	u64 len;        // 2^nq amplitudes
-	char *base;     // device allocation: amplitudes then scalars
+	char *base;     // device allocation: amplitudes (then scalars, unless
+	                // sbase holds them)
+	char *sbase;    // separate scalars block of arena-sized states, or null
 	double *scal;   // device scalars
 	void *amp;      // device amplitudes (raw), double2 or float2
 	u64 bytes;      // bytes of the allocation
@@ -63,39 +75,75 @@ struct dk_gate {
 	unsigned dim;       // 2^k
 	double2 *dmat;      // device FP64 copy, row-major (k > 1)
 	float2 *dmat32;     // device FP32 copy, row-major (k > 1)
+	mutable dk_lane *lane; // lane that last queued work reading dmat
 	double2 hmat[64 * 64];
+	float2 hmat32[64 * 64]; // source of the FP32 upload
 };
 
 // ---- library context ---------------------------------------------------------
 
 enum { DK_PARTIALS = 1 << 20 }; // initial doubles of reduction scratch
 
-struct dk_ctx {
-	int device = -1;
-	int num_cus = 0;
+// An execution lane: a HIP stream with its own reduction scratch.  Each host
+// thread works on its bound lane (the default lane unless dk_lane_bind chose
+// another), so threads on different lanes run concurrently on the device.
+struct dk_lane {
 	hipStream_t stream = nullptr;
 	double *partials = nullptr;   // reduction scratch, grows on demand
 	u64 partials_cap = 0;         // doubles held by `partials`
 	double *dec = nullptr;        // measurement decisions scratch (device)
 	double *host_pinned = nullptr;// pinned staging (DK_PINNED doubles)
-	u64 budget = 0, used = 0;
 	hipEvent_t events[64] = {};
+};
+
+struct dk_ctx {
+	int device = -1;
+	int num_cus = 0;
+	bool ready = false;
+	dk_lane lane0;                // the default lane
+	u64 budget = 0, used = 0;
 	std::mutex mu;
 	// tunables
 	int apply_blocks = 0;   // 0 = auto
 	int time_apply = 0;     // record events[62]/[63] around the gate kernel
 	int dense_mfma = 1;     // dense k>=4 gates on MFMA (else VALU tiles)
+	u64 arena_max_chunks = 0;     // cap on 2 GiB state-arena chunks
+	u64 arena_default_chunks = 0; // the cap at init: free memory less 6 GiB
 };
 
 enum { DK_PINNED = 1 << 16 };
 
 extern dk_ctx g_ctx;
 
+// The calling thread's lane, and its stream.
+dk_lane *dk_cur();
+#define DK_STREAM (dk_cur()->stream)
+int dk_lane_setup(dk_lane *l);
+
+// Waits for the lane that last wrote `s` when that is not the calling
+// thread's lane.  Work on one state from several lanes is ordered this way;
+// the caller orders reads it issues concurrently from different lanes.
+int dk_settle(const dk_state *s);
+#define DK_SETTLE(s)                                                        \
+	do {                                                                \
+		int rc_ = dk_settle(s);                                     \
+		if (rc_)                                                    \
+			return rc_;                                         \
+	} while (0)
+
 int dk_ensure_init();
 int dk_alloc_state(unsigned nq, int dtype, dk_state **out);
 void dk_release_state(dk_state *s);
+// Device memory (dk_memory.cpp).  dk_device_free releases in stream order
+// on the calling lane and takes the size given to dk_device_alloc.  Blocks of
+// DK_ARENA_MIN bytes and more come from the chunk arena, rounded up to
+// dk_block_bytes(bytes) (a power of two up to 1 GiB, whole 2 GiB chunks
+// beyond).
+enum : u64 { DK_ARENA_MIN = 64ull << 20 };
 int dk_device_alloc(u64 bytes, void **out);
 void dk_device_free(void *p, u64 bytes);
+u64 dk_block_bytes(u64 bytes);
+bool dk_closed();
 
 // ---- kernel launchers (dk_kernels.hip, dk_dense.hip) --------------------------
 

@@ -675,7 +675,7 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		       const unsigned *targets, u64 cm, u64 acm, double *parts,
 		       u64 *nblocks_out)
 {
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	const T *src = (const T *)in->amp;
 	T *dst = (T *)out->amp;
 	u64 nblocks = 0;
@@ -786,12 +786,18 @@ void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta,
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm)
 {
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	double *parts = nullptr;
 	u64 nblocks = 0;
 	int rc;
+	if (g->lane != dk_cur()) {
+		// the gate's upload (or last use) was queued on another lane
+		if (g->dmat)
+			DK_HIP(hipStreamSynchronize(g->lane->stream));
+		g->lane = dk_cur();
+	}
 	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(g_ctx.events[62], st));
+		DK_HIP(hipEventRecord(dk_cur()->events[62], st));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
 	    in->nq >= g->k + 6) {
 		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
@@ -808,7 +814,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	if (rc)
 		return rc;
 	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(g_ctx.events[63], st));
+		DK_HIP(hipEventRecord(dk_cur()->events[63], st));
 	return dk_finish_partials(parts, nblocks, out->scal + DK_SCAL_NORM, 1);
 }
 
@@ -817,7 +823,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 // The partials buffer holds room for the chunk sums after `count`.
 int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt)
 {
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	if (count <= 4 * RCHUNK) {
 		hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, st, parts,
 				   (int)count, dst, take_sqrt);
@@ -839,15 +845,16 @@ int dk_partials_reserve(u64 count, double **out)
 	u64 need = count + 16;
 	for (u64 c = count; c > 4 * RCHUNK; c = (c + RCHUNK - 1) / RCHUNK)
 		need += (c + RCHUNK - 1) / RCHUNK;
-	if (need > g_ctx.partials_cap) {
+	if (need > dk_cur()->partials_cap) {
 		double *p = nullptr;
-		DK_HIP(hipMallocAsync((void **)&p, need * sizeof(double), g_ctx.stream));
-		if (g_ctx.partials)
-			DK_HIP(hipFreeAsync(g_ctx.partials, g_ctx.stream));
-		g_ctx.partials = p;
-		g_ctx.partials_cap = need;
+		DK_TRY(dk_device_alloc(need * sizeof(double), (void **)&p));
+		if (dk_cur()->partials)
+			dk_device_free(dk_cur()->partials,
+				       dk_cur()->partials_cap * sizeof(double));
+		dk_cur()->partials = p;
+		dk_cur()->partials_cap = need;
 	}
-	*out = g_ctx.partials;
+	*out = dk_cur()->partials;
 	return DK_OK;
 }
 
@@ -865,8 +872,8 @@ int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out)
 	const u64 half = s->len >> 1;
 	const unsigned nb = grid_for(half, (u64)TPB * 4);
 #define L(T)                                                                \
-	hipLaunchKernelGGL(k_prob<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream,  \
-			    (const T *)s->amp, s->scal, q, half, g_ctx.partials)
+	hipLaunchKernelGGL(k_prob<T>, dim3(nb), dim3(TPB), 0, DK_STREAM,  \
+			    (const T *)s->amp, s->scal, q, half, dk_cur()->partials)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
@@ -880,8 +887,8 @@ int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
 	int rc = dk_launch_prob_partials(s, q, &nb);
 	if (rc)
 		return rc;
-	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, g_ctx.stream,
-			   g_ctx.partials, nb, dst, 0);
+	hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(TPB), 0, DK_STREAM,
+			   dk_cur()->partials, nb, dst, 0);
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -901,13 +908,13 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 	const unsigned nb = grid_for(nch, WAVES);
 #define L(T)                                                                  \
 	hipLaunchKernelGGL((k_prob_all<T, LU>), dim3(nb), dim3(TPB), 0,        \
-			    g_ctx.stream, (const T *)s->amp, s->scal, s->len,       \
-			    g_ctx.partials)
+			    DK_STREAM, (const T *)s->amp, s->scal, s->len,       \
+			    dk_cur()->partials)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
-	hipLaunchKernelGGL(k_finish_cols, dim3(s->nq), dim3(TPB), 0, g_ctx.stream,
-			   g_ctx.partials, (int)nb, PQ, dst);
+	hipLaunchKernelGGL(k_finish_cols, dim3(s->nq), dim3(TPB), 0, DK_STREAM,
+			   dk_cur()->partials, (int)nb, PQ, dst);
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -915,7 +922,7 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
 {
 	const unsigned nb = grid_for(src->len, TPB * 4);
-	hipStream_t st = g_ctx.stream;
+	hipStream_t st = DK_STREAM;
 	if (src->dtype == DK_C128 && dst->dtype == DK_C128)
 		hipLaunchKernelGGL((k_normalised_copy<double2, double2>), dim3(nb),
 				   dim3(TPB), 0, st, src->a128(), src->scal,
@@ -939,7 +946,7 @@ int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
 int dk_launch_widen(const dk_state *s, u64 first, u64 count, double2 *dst)
 {
 	const unsigned nb = grid_for(count, TPB * 4);
-	hipLaunchKernelGGL(k_widen<float2>, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+	hipLaunchKernelGGL(k_widen<float2>, dim3(nb), dim3(TPB), 0, DK_STREAM,
 			   s->a64() + first, count, dst);
 	DK_LAUNCHED();
 	return DK_OK;
@@ -948,7 +955,7 @@ int dk_launch_widen(const dk_state *s, u64 first, u64 count, double2 *dst)
 int dk_launch_decide(const double *parts, int nparts, double roll,
 		     double *dec_slot, dk_state *next)
 {
-	hipLaunchKernelGGL(k_decide, dim3(1), dim3(TPB), 0, g_ctx.stream, parts,
+	hipLaunchKernelGGL(k_decide, dim3(1), dim3(TPB), 0, DK_STREAM, parts,
 			   nparts, roll, dec_slot, next ? next->scal : nullptr);
 	DK_LAUNCHED();
 	return DK_OK;
@@ -960,9 +967,9 @@ int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 	const u64 olen = s->len >> 1;
 	const unsigned nb = grid_for(olen, (u64)TPB * 4);
 #define L(T)                                                                  \
-	hipLaunchKernelGGL(k_collapse<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream, \
+	hipLaunchKernelGGL(k_collapse<T>, dim3(nb), dim3(TPB), 0, DK_STREAM, \
 			    (const T *)s->amp, s->scal, q, dec_slot, (T *)o->amp,  \
-			    o->scal, next_q, olen, g_ctx.partials)
+			    o->scal, next_q, olen, dk_cur()->partials)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
@@ -974,7 +981,7 @@ int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
 {
 	const unsigned nb = grid_for(o->len, TPB * 4);
 #define L(T)                                                                  \
-	hipLaunchKernelGGL(k_join<T>, dim3(nb), dim3(TPB), 0, g_ctx.stream,     \
+	hipLaunchKernelGGL(k_join<T>, dim3(nb), dim3(TPB), 0, DK_STREAM,     \
 			    (const T *)a->amp, a->scal, (const T *)b->amp, b->scal, \
 			    b->nq, o->len, (T *)o->amp, o->scal)
 	DK_BY_TYPE(o, L);
@@ -985,7 +992,7 @@ int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
 
 int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev)
 {
-	hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, g_ctx.stream,
+	hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, DK_STREAM,
 			   (unsigned long long *)dst_dev, s->len);
 	DK_LAUNCHED();
 	if (start >= s->len)
@@ -993,7 +1000,7 @@ int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev)
 	const unsigned nb = grid_for(s->len - start, TPB * 4);
 #define L(T)                                                                  \
 	hipLaunchKernelGGL(k_first_nonzero<T>, dim3(nb), dim3(TPB), 0,         \
-			    g_ctx.stream, (const T *)s->amp, start, s->len,         \
+			    DK_STREAM, (const T *)s->amp, start, s->len,         \
 			    (unsigned long long *)dst_dev)
 	DK_BY_TYPE(s, L);
 #undef L
@@ -1043,20 +1050,20 @@ int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
 	nch = std::min<u64>(p, 65535);
 	const u64 tchunk = T / nch;
 	double2 *part = nullptr;
-	DK_HIP(hipMallocAsync((void **)&part, nch * dim * dim * sizeof(double2),
-			      g_ctx.stream));
+	const u64 part_bytes = nch * dim * dim * sizeof(double2);
+	DK_TRY(dk_device_alloc(part_bytes, (void **)&part));
 #define L(T_)                                                                  \
 	hipLaunchKernelGGL(k_density_tile<T_>, dim3((unsigned)ntiles, (unsigned)nch), \
-			    dim3(TPB), 0, g_ctx.stream, (const T_ *)s->amp, s->scal, \
+			    dim3(TPB), 0, DK_STREAM, (const T_ *)s->amp, s->scal, \
 			    da, tchunk, tpr, part)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
 	const unsigned nb = grid_for(dim * dim, TPB);
-	hipLaunchKernelGGL(k_density_sum, dim3(nb), dim3(TPB), 0, g_ctx.stream,
+	hipLaunchKernelGGL(k_density_sum, dim3(nb), dim3(TPB), 0, DK_STREAM,
 			   part, (int)nch, dim * dim, rho_dev);
 	DK_LAUNCHED();
-	DK_HIP(hipFreeAsync(part, g_ctx.stream));
+	dk_device_free(part, part_bytes);
 	return DK_OK;
 }
 
@@ -1070,7 +1077,7 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 		return DK_OK;
 #define L(T)                                                                  \
 	hipLaunchKernelGGL(k_density_elements<T>, dim3((unsigned)count),       \
-			    dim3(TPB), 0, g_ctx.stream, (const T *)s->amp, s->scal, \
+			    dim3(TPB), 0, DK_STREAM, (const T *)s->amp, s->scal, \
 			    da, rows_dev, cols_dev, out_dev)
 	DK_BY_TYPE(s, L);
 #undef L
@@ -1083,7 +1090,7 @@ int dk_launch_fill_zero_state(dk_state *s)
 	const unsigned nb = grid_for(s->len, TPB * 4);
 #define L(T)                                                                  \
 	hipLaunchKernelGGL(k_fill_zero_state<T>, dim3(nb), dim3(TPB), 0,       \
-			    g_ctx.stream, (T *)s->amp, s->scal, s->len)
+			    DK_STREAM, (T *)s->amp, s->scal, s->len)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
@@ -1092,7 +1099,7 @@ int dk_launch_fill_zero_state(dk_state *s)
 
 int dk_launch_set_norm(dk_state *s, double v)
 {
-	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, g_ctx.stream,
+	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, DK_STREAM,
 			   s->scal + DK_SCAL_NORM, v);
 	DK_LAUNCHED();
 	return DK_OK;

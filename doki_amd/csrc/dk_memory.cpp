@@ -1,0 +1,412 @@
+// Device memory of libdoki_hip.so: state vectors, batch buffers and the like.
+//
+// Physical memory for everything of 64 MiB and more comes from an arena of
+// 2 GiB chunks (hipMemCreate) that stay with the process:
+//  * blocks of 2 GiB and more (state vectors of 2^27 and more complex128
+//    amplitudes) are virtual ranges mapped onto as many whole chunks as they
+//    need, sizes rounded up to whole chunks;
+//  * blocks of 64 MiB up to 1 GiB are power-of-two pieces of "split" chunks,
+//    each split chunk mapped at an address of its own and divided by a buddy
+//    system (sizes rounded up to a power of two);
+//  * blocks below 64 MiB come from the device's stream-ordered pool
+//    (hipMallocAsync, unlimited release threshold).
+// Released blocks of the first two kinds stay in size-keyed caches and are
+// handed out again; a chunk shortage is met by creating chunks while the
+// device has memory, then by unmapping cached blocks and merging cached
+// pieces back into whole chunks.  Only when the pool itself runs dry do free
+// chunks go back to the driver, one at a time.
+//
+// The arena exists because memory returned to the driver is wiped before the
+// driver hands it out again, at a few tens of GB/s: freeing a 275 GB state
+// with hipFree and allocating 34 GB afterwards stalls for seconds, while
+// unmapping and remapping the same chunks takes about 0.3 ms per GiB.
+//
+// Every released block carries an event recorded on the releasing lane's
+// stream after the work queued there.  A lane reusing it waits for that event
+// on the device; unmapping and merging wait for it on the host.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <set>
+#include <unordered_map>
+#include <vector>
+
+#include "dk_buddy.h"
+#include "dk_internal.h"
+
+namespace {
+
+enum : u64 {
+	MID_BLOCK = dkb::PIECE,    // smallest arena piece
+	CHUNK = dkb::CHUNK,        // physical chunk
+};
+static_assert(dkb::PIECE == DK_ARENA_MIN, "arena pieces start at DK_ARENA_MIN");
+using dkb::order_of;
+
+typedef hipMemGenericAllocationHandle_t Handle;
+
+struct Cached {
+	void *p;
+	hipEvent_t done;  // after the releasing lane's queued work (or null)
+	dk_lane *lane;
+};
+
+struct Mapped {          // a block over whole chunks
+	u64 bytes;
+	std::vector<Handle> chunks;
+};
+
+std::mutex g_mem_mu;
+std::vector<Handle> g_free_chunks;             // created, unmapped
+u64 g_chunks_total = 0;
+std::multimap<u64, Cached> g_whole_free;        // released whole-chunk blocks
+std::unordered_map<void *, Mapped> g_mapped;    // every whole-chunk block
+std::multimap<u64, Cached> g_piece_free;        // released buddy pieces
+struct Split {
+	Handle h;
+	void *resv;  // its address reservation (two chunks, for alignment)
+};
+std::unordered_map<u64, Split> g_split;         // split chunk base -> chunk
+dkb::Buddy g_buddy;                             // free pieces of split chunks
+
+hipMemAllocationProp chunk_prop()
+{
+	hipMemAllocationProp prop = {};
+	prop.type = hipMemAllocationTypePinned;
+	prop.location.type = hipMemLocationTypeDevice;
+	prop.location.id = g_ctx.device;
+	return prop;
+}
+
+Cached released_here(void *p)
+{
+	Cached c{p, nullptr, dk_cur()};
+	if (hipEventCreateWithFlags(&c.done, hipEventDisableTiming) == hipSuccess)
+		(void)hipEventRecord(c.done, DK_STREAM);
+	else
+		c.done = nullptr;
+	return c;
+}
+
+// Hands a cached block to the calling lane: its stream waits for the
+// previous owner's work when that ran on another lane.
+void *take(const Cached &c)
+{
+	if (c.done) {
+		if (c.lane != dk_cur())
+			(void)hipStreamWaitEvent(DK_STREAM, c.done, 0);
+		(void)hipEventDestroy(c.done);
+	}
+	return c.p;
+}
+
+void settle_host(const Cached &c)
+{
+	if (c.done) {
+		(void)hipEventSynchronize(c.done);
+		(void)hipEventDestroy(c.done);
+	}
+}
+
+int oom(u64 bytes, hipError_t e)
+{
+	(void)hipGetLastError();
+	dk_set_error("Failed to allocate state chunk: %llu bytes: %s",
+		     (unsigned long long)bytes, hipGetErrorString(e));
+	return DK_E_ALLOC;
+}
+
+// ---- buddy pieces of split chunks -------------------------------------------
+
+// Cached pieces go back into the buddy system (merging where they can).
+bool flush_pieces()
+{
+	if (g_piece_free.empty())
+		return false;
+	for (auto &kv : g_piece_free) {
+		settle_host(kv.second);
+		g_buddy.release((u64)kv.second.p, order_of(kv.first));
+	}
+	g_piece_free.clear();
+	return true;
+}
+
+// Split chunks whose pieces are all free become whole free chunks again.
+bool unsplit_free_chunks()
+{
+	return g_buddy.drain_whole([](u64 base) {
+		auto sp = g_split.find(base);
+		(void)hipMemUnmap((void *)base, CHUNK);
+		(void)hipMemAddressFree(sp->second.resv, 2 * CHUNK);
+		g_free_chunks.push_back(sp->second.h);
+		g_split.erase(sp);
+	});
+}
+
+// ---- whole chunks ---------------------------------------------------------------
+
+void unmap_whole(void *va)
+{
+	auto it = g_mapped.find(va);
+	if (it == g_mapped.end())
+		return;
+	(void)hipMemUnmap(va, it->second.bytes);
+	(void)hipMemAddressFree(va, it->second.bytes);
+	for (auto h : it->second.chunks)
+		g_free_chunks.push_back(h);
+	g_mapped.erase(it);
+}
+
+// Unmaps one cached whole-chunk block (the largest) and keeps its chunks.
+bool harvest_whole()
+{
+	if (g_whole_free.empty())
+		return false;
+	auto it = std::prev(g_whole_free.end());
+	Cached c = it->second;
+	g_whole_free.erase(it);
+	settle_host(c);
+	unmap_whole(c.p);
+	return true;
+}
+
+// Makes `need` chunks free: new ones while the device (and the cap) allow,
+// then chunks of cached blocks and pieces.
+int gather_chunks(size_t need, u64 bytes)
+{
+	const hipMemAllocationProp prop = chunk_prop();
+	bool trimmed = false;
+	while (g_free_chunks.size() < need) {
+		const bool capped = g_chunks_total >= g_ctx.arena_max_chunks;
+		hipError_t e = hipErrorOutOfMemory;
+		if (!capped) {
+			Handle h;
+			e = hipMemCreate(&h, CHUNK, &prop, 0);
+			if (e == hipSuccess) {
+				g_free_chunks.push_back(h);
+				++g_chunks_total;
+				continue;
+			}
+			(void)hipGetLastError();
+		}
+		if (harvest_whole())
+			continue;
+		if (flush_pieces() && unsplit_free_chunks())
+			continue;
+		if (unsplit_free_chunks())
+			continue;
+		if (!capped && !trimmed) {
+			// memory parked in the stream-ordered pool
+			hipMemPool_t pool;
+			if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess) {
+				(void)hipStreamSynchronize(DK_STREAM);
+				(void)hipMemPoolTrimTo(pool, 0);
+			}
+			trimmed = true;
+			continue;
+		}
+		return oom(bytes, e);
+	}
+	return DK_OK;
+}
+
+int alloc_whole(u64 bytes, void **out)
+{
+	auto it = g_whole_free.find(bytes);
+	if (it != g_whole_free.end()) {
+		*out = take(it->second);
+		g_whole_free.erase(it);
+		return DK_OK;
+	}
+	const size_t need = bytes / CHUNK;
+	int rc = gather_chunks(need, bytes);
+	if (rc)
+		return rc;
+	void *va = nullptr;
+	hipError_t e = hipMemAddressReserve(&va, bytes, CHUNK, nullptr, 0);
+	if (e != hipSuccess)
+		return oom(bytes, e);
+	Mapped m;
+	m.bytes = bytes;
+	for (size_t i = 0; i < need && e == hipSuccess; ++i) {
+		Handle h = g_free_chunks.back();
+		e = hipMemMap((char *)va + i * CHUNK, CHUNK, 0, h, 0);
+		if (e == hipSuccess) {
+			g_free_chunks.pop_back();
+			m.chunks.push_back(h);
+		}
+	}
+	if (e == hipSuccess) {
+		const hipMemAllocationProp prop = chunk_prop();
+		hipMemAccessDesc ad = {};
+		ad.location = prop.location;
+		ad.flags = hipMemAccessFlagsProtReadWrite;
+		e = hipMemSetAccess(va, bytes, &ad, 1);
+	}
+	g_mapped.emplace(va, std::move(m));
+	if (e != hipSuccess) {
+		unmap_whole(va);
+		return oom(bytes, e);
+	}
+	*out = va;
+	return DK_OK;
+}
+
+int alloc_piece(u64 bytes, void **out)
+{
+	auto it = g_piece_free.find(bytes);
+	if (it != g_piece_free.end()) {
+		*out = take(it->second);
+		g_piece_free.erase(it);
+		return DK_OK;
+	}
+	const int o = order_of(bytes);
+	u64 addr;
+	if (!g_buddy.take(o, &addr) && !(flush_pieces() && g_buddy.take(o, &addr))) {
+		// split a chunk, mapped at an address of its own
+		int rc = gather_chunks(1, bytes);
+		if (rc)
+			return rc;
+		// the buddy arithmetic needs a chunk-aligned base: reserve two
+		// chunks of addresses and map at the aligned one inside
+		void *resv = nullptr;
+		hipError_t e = hipMemAddressReserve(&resv, 2 * CHUNK, 0, nullptr, 0);
+		if (e != hipSuccess)
+			return oom(bytes, e);
+		void *va = (void *)(((u64)resv + CHUNK - 1) & ~(CHUNK - 1));
+		Handle h = g_free_chunks.back();
+		e = hipMemMap(va, CHUNK, 0, h, 0);
+		if (e == hipSuccess) {
+			const hipMemAllocationProp prop = chunk_prop();
+			hipMemAccessDesc ad = {};
+			ad.location = prop.location;
+			ad.flags = hipMemAccessFlagsProtReadWrite;
+			e = hipMemSetAccess(va, CHUNK, &ad, 1);
+			if (e != hipSuccess)
+				(void)hipMemUnmap(va, CHUNK);
+		}
+		if (e != hipSuccess) {
+			(void)hipMemAddressFree(resv, 2 * CHUNK);
+			return oom(bytes, e);
+		}
+		g_free_chunks.pop_back();
+		g_split.emplace((u64)va, Split{h, resv});
+		g_buddy.release((u64)va, dkb::ORDERS - 1);
+		g_buddy.take(o, &addr);
+	}
+	*out = (void *)addr;
+	return DK_OK;
+}
+
+// Pool shortage: cached blocks and pieces go back to chunks, and free chunks
+// go back to the driver one at a time until `retry` succeeds.
+template <typename F>
+hipError_t relieve_pool(F retry)
+{
+	hipError_t e = retry();
+	while (e != hipSuccess) {
+		(void)hipGetLastError();
+		if (g_free_chunks.empty()) {
+			if (!harvest_whole() && !(flush_pieces(), unsplit_free_chunks()))
+				break;
+			continue;
+		}
+		(void)hipMemRelease(g_free_chunks.back());
+		g_free_chunks.pop_back();
+		--g_chunks_total;
+		e = retry();
+	}
+	return e;
+}
+
+} // namespace
+
+u64 dk_block_bytes(u64 bytes)
+{
+	if (bytes < MID_BLOCK)
+		return bytes;
+	if (bytes <= CHUNK / 2)
+		return MID_BLOCK << order_of(bytes);
+	return (bytes + CHUNK - 1) / CHUNK * CHUNK;
+}
+
+int dk_device_alloc(u64 bytes, void **out)
+{
+	*out = nullptr;
+	if (bytes < MID_BLOCK) {
+		hipError_t e = hipMallocAsync(out, bytes, DK_STREAM);
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			std::lock_guard<std::mutex> lk(g_mem_mu);
+			e = relieve_pool([&] { return hipMallocAsync(out, bytes, DK_STREAM); });
+		}
+		if (e != hipSuccess) {
+			*out = nullptr;
+			return oom(bytes, e);
+		}
+		return DK_OK;
+	}
+	const u64 b = dk_block_bytes(bytes);
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	return b >= CHUNK ? alloc_whole(b, out) : alloc_piece(b, out);
+}
+
+void dk_device_free(void *p, u64 bytes)
+{
+	if (!p || !g_ctx.ready || dk_closed())
+		return;
+	if (bytes < MID_BLOCK) {
+		(void)hipFreeAsync(p, DK_STREAM);
+		return;
+	}
+	const u64 b = dk_block_bytes(bytes);
+	Cached c = released_here(p);
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	if (b >= CHUNK)
+		g_whole_free.emplace(b, c);
+	else
+		g_piece_free.emplace(b, c);
+}
+
+// Cached blocks and pieces, and every chunk no live block needs, go back to
+// the driver; the pool is trimmed.
+extern "C" int dk_memory_trim(void)
+{
+	int rc = dk_ensure_init();
+	if (rc)
+		return rc;
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	while (harvest_whole()) {
+	}
+	flush_pieces();
+	unsplit_free_chunks();
+	for (auto h : g_free_chunks)
+		(void)hipMemRelease(h);
+	g_chunks_total -= g_free_chunks.size();
+	g_free_chunks.clear();
+	hipMemPool_t pool;
+	if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess) {
+		(void)hipDeviceSynchronize();
+		(void)hipMemPoolTrimTo(pool, 0);
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_memory_stats(uint64_t *arena_chunks, uint64_t *arena_free_chunks,
+			       uint64_t *cached_bytes)
+{
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	u64 cached = 0;
+	for (auto &kv : g_whole_free)
+		cached += kv.first;
+	for (auto &kv : g_piece_free)
+		cached += kv.first;
+	if (arena_chunks)
+		*arena_chunks = g_chunks_total;
+	if (arena_free_chunks)
+		*arena_free_chunks = g_free_chunks.size();
+	if (cached_bytes)
+		*cached_bytes = cached;
+	return DK_OK;
+}

@@ -26,7 +26,7 @@ extern "C" const char *dk_last_error(void)
 	return t_err;
 }
 
-// Device slots inside g_ctx.dec
+// Device slots inside dk_cur()->dec
 enum {
 	DEC_SLOTS = 64,            // measurement decisions: 2 doubles each
 	DEC_SCALAR = 2 * DEC_SLOTS,// one device scalar result
@@ -35,10 +35,35 @@ enum {
 	DEC_TOTAL = DEC_PROBS + 64
 };
 
+static thread_local dk_lane *t_lane = nullptr;
+
+dk_lane *dk_cur()
+{
+	return t_lane ? t_lane : &g_ctx.lane0;
+}
+
+// Stream, scratch, pinned staging and timing events of a lane.
+int dk_lane_setup(dk_lane *l)
+{
+	hipStream_t st;
+	DK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+	DK_HIP(hipMallocAsync((void **)&l->partials, DK_PARTIALS * sizeof(double),
+			      st));  // DK_PARTIALS doubles: a pool-sized block
+	l->partials_cap = DK_PARTIALS;
+	DK_HIP(hipMalloc((void **)&l->dec, 2 * 64 * sizeof(double) + 2048));
+	DK_HIP(hipHostMalloc((void **)&l->host_pinned, DK_PINNED * sizeof(double),
+			     hipHostMallocDefault));
+	for (auto &ev : l->events)
+		DK_HIP(hipEventCreate(&ev));
+	DK_HIP(hipStreamSynchronize(st));
+	l->stream = st;
+	return DK_OK;
+}
+
 extern "C" int dk_init(int device)
 {
 	std::lock_guard<std::mutex> lk(g_ctx.mu);
-	if (g_ctx.stream)
+	if (g_ctx.ready)
 		return DK_OK;
 	int count = 0;
 	hipError_t e = hipGetDeviceCount(&count);
@@ -61,16 +86,9 @@ extern "C" int dk_init(int device)
 	}
 	g_ctx.device = device;
 	g_ctx.num_cus = prop.multiProcessorCount;
-	hipStream_t st;
-	DK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-	DK_HIP(hipMallocAsync((void **)&g_ctx.partials, DK_PARTIALS * sizeof(double),
-			      st));
-	g_ctx.partials_cap = DK_PARTIALS;
-	DK_HIP(hipMalloc((void **)&g_ctx.dec, DEC_TOTAL * sizeof(double)));
-	DK_HIP(hipHostMalloc((void **)&g_ctx.host_pinned,
-			     DK_PINNED * sizeof(double), hipHostMallocDefault));
-	for (auto &ev : g_ctx.events)
-		DK_HIP(hipEventCreate(&ev));
+	int rc = dk_lane_setup(&g_ctx.lane0);
+	if (rc)
+		return rc;
 	hipMemPool_t pool;
 	DK_HIP(hipDeviceGetDefaultMemPool(&pool, device));
 	uint64_t thr = UINT64_MAX;
@@ -79,13 +97,17 @@ extern "C" int dk_init(int device)
 	DK_HIP(hipMemGetInfo(&fr, &tot));
 	g_ctx.budget = (u64)((double)fr * 0.92);
 	g_ctx.used = 0;
-	g_ctx.stream = st;
+	// the state arena leaves 6 GiB of the free memory to the pool
+	const u64 headroom = 6ull << 30, chunk = 2ull << 30;
+	g_ctx.arena_default_chunks = fr > headroom + chunk ? (fr - headroom) / chunk : 1;
+	g_ctx.arena_max_chunks = g_ctx.arena_default_chunks;
+	g_ctx.ready = true;
 	return DK_OK;
 }
 
 int dk_ensure_init()
 {
-	if (g_ctx.stream)
+	if (g_ctx.ready)
 		return DK_OK;
 	const char *dev = getenv("DOKI_HIP_DEVICE");
 	return dk_init(dev ? atoi(dev) : 0);
@@ -101,18 +123,81 @@ int dk_ensure_init()
 extern "C" int dk_synchronize(void)
 {
 	DK_INIT();
-	DK_HIP(hipStreamSynchronize(g_ctx.stream));
+	DK_HIP(hipStreamSynchronize(DK_STREAM));
 	return DK_OK;
 }
 
 // Marks the library closed: later frees only drop host bookkeeping.  The
 // extension calls it when the interpreter finalises, so that any state freed
 // after that point never reaches a HIP runtime that may be tearing down.
+bool dk_closed()
+{
+	return g_closed;
+}
+
 extern "C" void dk_shutdown(void)
 {
-	if (g_ctx.stream)
-		(void)hipStreamSynchronize(g_ctx.stream);
+	if (g_ctx.ready)
+		(void)hipDeviceSynchronize();
 	g_closed = true;
+}
+
+// ---- lanes ------------------------------------------------------------------
+
+// Freed lanes are kept for reuse rather than destroyed: states remember the
+// lane that last wrote them, and that pointer must stay valid.
+static std::vector<dk_lane *> g_spare_lanes;
+
+extern "C" int dk_lane_new(dk_lane **out)
+{
+	DK_INIT();
+	if (!out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	{
+		std::lock_guard<std::mutex> lk(g_ctx.mu);
+		if (!g_spare_lanes.empty()) {
+			*out = g_spare_lanes.back();
+			g_spare_lanes.pop_back();
+			return DK_OK;
+		}
+	}
+	dk_lane *l = new dk_lane;
+	int rc = dk_lane_setup(l);
+	if (rc) {
+		delete l;
+		return rc;
+	}
+	*out = l;
+	return DK_OK;
+}
+
+extern "C" int dk_lane_bind(dk_lane *l)
+{
+	DK_INIT();
+	t_lane = l;
+	return DK_OK;
+}
+
+extern "C" void dk_lane_free(dk_lane *l)
+{
+	if (!l || l == &g_ctx.lane0)
+		return;
+	if (t_lane == l)
+		t_lane = nullptr;
+	if (!g_closed)
+		(void)hipStreamSynchronize(l->stream);
+	std::lock_guard<std::mutex> lk(g_ctx.mu);
+	g_spare_lanes.push_back(l);
+}
+
+// A state last written on another lane: wait for that lane's work first.
+int dk_settle(const dk_state *s)
+{
+	if (s && s->lane && s->lane != dk_cur())
+		DK_HIP(hipStreamSynchronize(s->lane->stream));
+	return DK_OK;
 }
 
 extern "C" int dk_memory_budget(uint64_t *budget, uint64_t *used)
@@ -132,94 +217,6 @@ extern "C" int dk_set_memory_budget(uint64_t budget)
 	std::lock_guard<std::mutex> lk(g_ctx.mu);
 	g_ctx.budget = budget;
 	return DK_OK;
-}
-
-// ---- device memory ------------------------------------------------------------
-//
-// Small blocks come from the stream-ordered pool (cheap, recycled in stream
-// order).  Large blocks (state vectors of 2^22+ amplitudes) are plain
-// hipMalloc allocations kept in a size-keyed cache and handed out again in
-// stream order: they map with large pages, which high-stride gate passes over
-// multi-GiB states need to stream at full HBM rate, and reusing them avoids a
-// synchronising hipFree per gate.  Every block is used only on the library
-// stream, so a block released by one call can serve the next call at once.
-
-enum : u64 { DK_BIG_BLOCK = 64ull << 20 };
-
-static std::multimap<u64, void *> g_big_free;
-static u64 g_big_cached = 0;
-
-static void release_cached_blocks()
-{
-	if (g_big_free.empty())
-		return;
-	(void)hipStreamSynchronize(g_ctx.stream);
-	for (auto &kv : g_big_free)
-		(void)hipFree(kv.second);
-	g_big_free.clear();
-	g_big_cached = 0;
-}
-
-int dk_device_alloc(u64 bytes, void **out)
-{
-	*out = nullptr;
-	if (bytes < DK_BIG_BLOCK) {
-		hipError_t e = hipMallocAsync(out, bytes, g_ctx.stream);
-		if (e != hipSuccess) {
-			(void)hipGetLastError();
-			std::lock_guard<std::mutex> lk(g_ctx.mu);
-			release_cached_blocks();
-			hipMemPool_t pool;
-			if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
-				(void)hipMemPoolTrimTo(pool, 0);
-			e = hipMallocAsync(out, bytes, g_ctx.stream);
-		}
-		if (e != hipSuccess) {
-			(void)hipGetLastError();
-			dk_set_error("Failed to allocate state chunk: %s",
-				     hipGetErrorString(e));
-			return DK_E_ALLOC;
-		}
-		return DK_OK;
-	}
-	std::lock_guard<std::mutex> lk(g_ctx.mu);
-	auto it = g_big_free.find(bytes);
-	if (it != g_big_free.end()) {
-		*out = it->second;
-		g_big_free.erase(it);
-		g_big_cached -= bytes;
-		return DK_OK;
-	}
-	hipError_t e = hipMalloc(out, bytes);
-	if (e != hipSuccess) {
-		(void)hipGetLastError();
-		release_cached_blocks();
-		hipMemPool_t pool;
-		(void)hipStreamSynchronize(g_ctx.stream);
-		if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess)
-			(void)hipMemPoolTrimTo(pool, 0);
-		e = hipMalloc(out, bytes);
-	}
-	if (e != hipSuccess) {
-		(void)hipGetLastError();
-		*out = nullptr;
-		dk_set_error("Failed to allocate state chunk: %s", hipGetErrorString(e));
-		return DK_E_ALLOC;
-	}
-	return DK_OK;
-}
-
-void dk_device_free(void *p, u64 bytes)
-{
-	if (!p || g_closed || !g_ctx.stream)
-		return;
-	if (bytes < DK_BIG_BLOCK) {
-		(void)hipFreeAsync(p, g_ctx.stream);
-		return;
-	}
-	std::lock_guard<std::mutex> lk(g_ctx.mu);
-	g_big_free.emplace(bytes, p);
-	g_big_cached += bytes;
 }
 
 // ---- states -----------------------------------------------------------------
@@ -242,7 +239,8 @@ int dk_alloc_state(unsigned nq, int dtype, dk_state **out)
 		return DK_E_BUDGET;
 	}
 	const u64 len = 1ull << nq;
-	const u64 bytes = DK_SCAL_BYTES + len * dk_amp_size(dtype);
+	const u64 abytes = len * dk_amp_size(dtype);
+	const u64 bytes = DK_SCAL_BYTES + abytes;
 	{
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		if (g_ctx.used + bytes > g_ctx.budget) {
@@ -255,8 +253,17 @@ int dk_alloc_state(unsigned nq, int dtype, dk_state **out)
 		}
 		g_ctx.used += bytes;
 	}
-	char *base = nullptr;
-	int rc = dk_device_alloc(bytes, (void **)&base);
+	// amplitudes first, so they start on the allocation's own alignment;
+	// arena blocks (power-of-two sizes) hold only the amplitudes, and the
+	// scalars take a small block of their own
+	const bool split = abytes >= DK_ARENA_MIN;
+	char *base = nullptr, *sbase = nullptr;
+	int rc = dk_device_alloc(split ? abytes : bytes, (void **)&base);
+	if (!rc && split) {
+		rc = dk_device_alloc(DK_SCAL_BYTES, (void **)&sbase);
+		if (rc)
+			dk_device_free(base, abytes);
+	}
 	if (rc) {
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		g_ctx.used -= bytes;
@@ -265,11 +272,12 @@ int dk_alloc_state(unsigned nq, int dtype, dk_state **out)
 	dk_state *s = new dk_state;
 	s->nq = nq;
 	s->dtype = dtype;
+	s->lane = dk_cur();
 	s->len = len;
 	s->base = base;
-	// amplitudes first, so they start on the allocation's own alignment
+	s->sbase = sbase;
 	s->amp = base;
-	s->scal = (double *)(base + len * dk_amp_size(dtype));
+	s->scal = (double *)(split ? sbase : base + abytes);
 	s->bytes = bytes;
 	*out = s;
 	return DK_OK;
@@ -279,8 +287,14 @@ void dk_release_state(dk_state *s)
 {
 	if (!s)
 		return;
-	if (!g_closed && g_ctx.stream) {
-		dk_device_free(s->base, s->bytes);
+	if (!g_closed && g_ctx.ready) {
+		(void)dk_settle(s);
+		if (s->sbase) {
+			dk_device_free(s->base, s->bytes - DK_SCAL_BYTES);
+			dk_device_free(s->sbase, DK_SCAL_BYTES);
+		} else {
+			dk_device_free(s->base, s->bytes);
+		}
 		std::lock_guard<std::mutex> lk(g_ctx.mu);
 		g_ctx.used -= s->bytes;
 	}
@@ -319,6 +333,7 @@ extern "C" int dk_state_convert(const dk_state *src, int dtype, dk_state **out)
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(src);
 	dk_state *s;
 	int rc = dk_alloc_state(src->nq, dtype, &s);
 	if (rc)
@@ -359,9 +374,9 @@ extern "C" int dk_state_from_host(unsigned num_qubits, const double *values,
 	if (rc)
 		return rc;
 	hipError_t e = hipMemcpyAsync(s->amp, values, s->len * sizeof(double2),
-				      hipMemcpyHostToDevice, g_ctx.stream);
+				      hipMemcpyHostToDevice, DK_STREAM);
 	if (e == hipSuccess)
-		e = hipStreamSynchronize(g_ctx.stream);
+		e = hipStreamSynchronize(DK_STREAM);
 	if (e != hipSuccess) {
 		dk_set_error("copy to device failed: %s", hipGetErrorString(e));
 		dk_release_state(s);
@@ -382,6 +397,7 @@ extern "C" int dk_state_clone(const dk_state *src, dk_state **out)
 		dk_set_error("NULL state");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(src);
 	dk_state *s;
 	int rc = dk_alloc_state(src->nq, src->dtype, &s);
 	if (rc)
@@ -416,10 +432,10 @@ static int copy_to_host(double *dst, const void *src, size_t count)
 	const char *p = (const char *)src;
 	while (count > 0) {
 		const size_t n = std::min<size_t>(count, DK_PINNED);
-		DK_HIP(hipMemcpyAsync(g_ctx.host_pinned, p, n * sizeof(double),
-				      hipMemcpyDeviceToHost, g_ctx.stream));
-		DK_HIP(hipStreamSynchronize(g_ctx.stream));
-		memcpy(dst, g_ctx.host_pinned, n * sizeof(double));
+		DK_HIP(hipMemcpyAsync(dk_cur()->host_pinned, p, n * sizeof(double),
+				      hipMemcpyDeviceToHost, DK_STREAM));
+		DK_HIP(hipStreamSynchronize(DK_STREAM));
+		memcpy(dst, dk_cur()->host_pinned, n * sizeof(double));
 		dst += n;
 		p += n * sizeof(double);
 		count -= n;
@@ -436,8 +452,8 @@ static int read_amplitudes(const dk_state *s, u64 first, u64 count,
 		return copy_to_host(values, s->a128() + first, 2 * count);
 	const u64 chunk = DK_PINNED / 2;
 	double2 *tmp = nullptr;
-	DK_HIP(hipMallocAsync((void **)&tmp, std::min(count, chunk) * sizeof(double2),
-			      g_ctx.stream));
+	const u64 tmp_bytes = std::min(count, chunk) * sizeof(double2);
+	DK_TRY(dk_device_alloc(tmp_bytes, (void **)&tmp));
 	int rc = DK_OK;
 	for (u64 done = 0; !rc && done < count; done += chunk) {
 		const u64 n = std::min(chunk, count - done);
@@ -445,7 +461,7 @@ static int read_amplitudes(const dk_state *s, u64 first, u64 count,
 		if (!rc)
 			rc = copy_to_host(values + 2 * done, tmp, 2 * n);
 	}
-	(void)hipFreeAsync(tmp, g_ctx.stream);
+	dk_device_free(tmp, tmp_bytes);
 	return rc;
 }
 
@@ -456,6 +472,7 @@ extern "C" int dk_state_norm(const dk_state *s, double *norm)
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(s);
 	return copy_to_host(norm, s->scal + DK_SCAL_NORM, 1);
 }
 
@@ -475,24 +492,30 @@ extern "C" int dk_gate_new(unsigned num_qubits, const double *matrix,
 	g->dim = 1u << num_qubits;
 	g->dmat = nullptr;
 	g->dmat32 = nullptr;
+	g->lane = dk_cur();
 	const size_t n = (size_t)g->dim * g->dim;
 	memcpy(g->hmat, matrix, n * sizeof(double2));
 	if (num_qubits > 1) {
-		// FP64 and FP32 copies in one allocation
-		std::vector<float2> m32(n);
+		// FP64 and FP32 copies in one stream-ordered allocation of the
+		// calling lane; the host sources live in the gate itself
 		for (size_t i = 0; i < n; ++i)
-			m32[i] = make_float2((float)g->hmat[i].x, (float)g->hmat[i].y);
+			g->hmat32[i] = make_float2((float)g->hmat[i].x, (float)g->hmat[i].y);
 		char *buf = nullptr;
 		const size_t b64 = n * sizeof(double2), b32 = n * sizeof(float2);
-		hipError_t e = hipMalloc((void **)&buf, b64 + b32);
+		hipStream_t st = DK_STREAM;
+		int rc = dk_device_alloc(b64 + b32, (void **)&buf);
+		if (rc) {
+			delete g;
+			return rc;
+		}
+		hipError_t e = hipMemcpyAsync(buf, g->hmat, b64, hipMemcpyHostToDevice, st);
 		if (e == hipSuccess)
-			e = hipMemcpy(buf, g->hmat, b64, hipMemcpyHostToDevice);
-		if (e == hipSuccess)
-			e = hipMemcpy(buf + b64, m32.data(), b32, hipMemcpyHostToDevice);
+			e = hipMemcpyAsync(buf + b64, g->hmat32, b32, hipMemcpyHostToDevice,
+					   st);
 		if (e != hipSuccess) {
 			dk_set_error("gate upload failed: %s", hipGetErrorString(e));
-			if (buf)
-				(void)hipFree(buf);
+			dk_device_free(buf, b64 + b32);
+			(void)hipStreamSynchronize(st);
 			delete g;
 			return DK_E_DEVICE;
 		}
@@ -508,9 +531,11 @@ extern "C" void dk_gate_free(dk_gate *g)
 	if (!g)
 		return;
 	if (g->dmat && !g_closed) {
-		// the gate may still be read by queued kernels
-		(void)hipStreamSynchronize(g_ctx.stream);
-		(void)hipFree(g->dmat);
+		// queued uploads and kernels reading the gate finish first; the
+		// release is then ordered on the calling lane
+		(void)hipStreamSynchronize(g->lane->stream);
+		const size_t n = (size_t)g->dim * g->dim;
+		dk_device_free(g->dmat, n * (sizeof(double2) + sizeof(float2)));
 	}
 	delete g;
 }
@@ -552,6 +577,7 @@ extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
 		dk_set_error("A target cannot also be a control or an anticontrol");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(in);
 	dk_state *o = in;
 	if (!in_place) {
 		int rc = dk_alloc_state(n, in->dtype, &o);
@@ -564,6 +590,7 @@ extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
 			dk_release_state(o);
 		return rc;
 	}
+	o->lane = dk_cur();
 	*out = o;
 	return DK_OK;
 }
@@ -579,7 +606,8 @@ extern "C" int dk_prob(const dk_state *s, unsigned q, double *p)
 		dk_set_error("Qubit id out of range");
 		return DK_E_ARG;
 	}
-	double *dst = g_ctx.dec + DEC_SCALAR;
+	DK_SETTLE(s);
+	double *dst = dk_cur()->dec + DEC_SCALAR;
 	int rc = dk_launch_prob(s, q, dst);
 	if (rc)
 		return rc;
@@ -593,7 +621,8 @@ extern "C" int dk_prob_all(const dk_state *s, double *p)
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
-	double *dst = g_ctx.dec + DEC_PROBS;
+	DK_SETTLE(s);
+	double *dst = dk_cur()->dec + DEC_PROBS;
 	int rc = dk_launch_prob_all(s, dst);
 	if (rc)
 		return rc;
@@ -612,6 +641,7 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 		return DK_E_ARG;
 	}
 	*out = nullptr;
+	DK_SETTLE(s);
 	const unsigned n = s->nq;
 	// qubits to measure, most significant first (bits >= n are ignored)
 	std::vector<unsigned> qs;
@@ -645,13 +675,13 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 	if (!rc && cur->nq > 1)
 		rc = dk_alloc_state(cur->nq - 1, s->dtype, &next);
 	if (!rc)
-		rc = dk_launch_decide(g_ctx.partials, nparts, rolls[0],
-				      g_ctx.dec + 0, next);
+		rc = dk_launch_decide(dk_cur()->partials, nparts, rolls[0],
+				      dk_cur()->dec + 0, next);
 	for (size_t k = 0; !rc && k < qs.size(); ++k) {
 		if (cur->nq == 1)
 			break; // last qubit: nothing left to collapse into
 		const int next_q = k + 1 < qs.size() ? (int)qs[k + 1] : -1;
-		rc = dk_launch_collapse(cur, qs[k], g_ctx.dec + 2 * k, next,
+		rc = dk_launch_collapse(cur, qs[k], dk_cur()->dec + 2 * k, next,
 					next_q, &nparts);
 		if (rc)
 			break;
@@ -664,9 +694,9 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 			if (cur->nq > 1)
 				rc = dk_alloc_state(cur->nq - 1, s->dtype, &next);
 			if (!rc)
-				rc = dk_launch_decide(g_ctx.partials, nparts,
+				rc = dk_launch_decide(dk_cur()->partials, nparts,
 						      rolls[k + 1],
-						      g_ctx.dec + 2 * (k + 1), next);
+						      dk_cur()->dec + 2 * (k + 1), next);
 		}
 	}
 	if (rc) {
@@ -677,7 +707,7 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 		return rc;
 	}
 	std::vector<double> dec(2 * qs.size());
-	rc = copy_to_host(dec.data(), g_ctx.dec, dec.size());
+	rc = copy_to_host(dec.data(), dk_cur()->dec, dec.size());
 	if (rc) {
 		if (owned)
 			dk_release_state(owned);
@@ -721,6 +751,8 @@ extern "C" int dk_join(const dk_state *most, const dk_state *least,
 		dk_set_error("joined states must share an amplitude type");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(most);
+	DK_SETTLE(least);
 	dk_state *o;
 	int rc = dk_alloc_state(most->nq + least->nq, most->dtype, &o);
 	if (rc)
@@ -746,6 +778,7 @@ extern "C" int dk_read_raw(const dk_state *s, uint64_t first, uint64_t count,
 		dk_set_error("Out of bounds");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(s);
 	int rc = DK_OK;
 	if (count > 0 && values)
 		rc = read_amplitudes(s, first, count, values);
@@ -774,6 +807,7 @@ extern "C" int dk_first_nonzero(const dk_state *s, uint64_t *index)
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(s);
 	// most states have a nonzero amplitude near the start: look there first
 	const u64 head = std::min<u64>(s->len, 2048);
 	std::vector<double> buf(2 * head);
@@ -785,7 +819,7 @@ extern "C" int dk_first_nonzero(const dk_state *s, uint64_t *index)
 			*index = i;
 			return DK_OK;
 		}
-	u64 *dst = (u64 *)(g_ctx.dec + DEC_U64);
+	u64 *dst = (u64 *)(dk_cur()->dec + DEC_U64);
 	rc = dk_launch_first_nonzero(s, head, dst);
 	if (rc)
 		return rc;
@@ -815,6 +849,7 @@ extern "C" int dk_reduced_density(const dk_state *s, uint64_t keep_mask,
 		dk_set_error("kept qubit out of range");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(s);
 	const int a = popcount64(keep_mask);
 	if (a > 12) {
 		dk_set_error("reduced density of %d qubits is too large to "
@@ -823,12 +858,11 @@ extern "C" int dk_reduced_density(const dk_state *s, uint64_t keep_mask,
 	}
 	const u64 dim = 1ull << a;
 	double2 *dev = nullptr;
-	DK_HIP(hipMallocAsync((void **)&dev, dim * dim * sizeof(double2),
-			      g_ctx.stream));
+	DK_TRY(dk_device_alloc(dim * dim * sizeof(double2), (void **)&dev));
 	int rc = dk_launch_density(s, keep_mask, dev);
 	if (!rc)
 		rc = copy_to_host(rho, dev, 2 * dim * dim);
-	(void)hipFreeAsync(dev, g_ctx.stream);
+	dk_device_free(dev, dim * dim * sizeof(double2));
 	return rc;
 }
 
@@ -846,6 +880,7 @@ extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask
 		dk_set_error("kept qubit out of range");
 		return DK_E_ARG;
 	}
+	DK_SETTLE(s);
 	const u64 dim = 1ull << popcount64(keep_mask);
 	for (size_t e = 0; e < count; ++e)
 		if (rows[e] >= dim || cols[e] >= dim) {
@@ -856,17 +891,17 @@ extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask
 		return DK_OK;
 	char *buf = nullptr;
 	const size_t bytes = count * (2 * sizeof(u64) + sizeof(double2));
-	DK_HIP(hipMallocAsync((void **)&buf, bytes, g_ctx.stream));
+	DK_TRY(dk_device_alloc(bytes, (void **)&buf));
 	u64 *drows = (u64 *)buf, *dcols = drows + count;
 	double2 *dout = (double2 *)(dcols + count);
 	int rc = DK_OK;
 	hipError_t e = hipMemcpyAsync(drows, rows, count * sizeof(u64),
-				      hipMemcpyHostToDevice, g_ctx.stream);
+				      hipMemcpyHostToDevice, DK_STREAM);
 	if (e == hipSuccess)
 		e = hipMemcpyAsync(dcols, cols, count * sizeof(u64),
-				   hipMemcpyHostToDevice, g_ctx.stream);
+				   hipMemcpyHostToDevice, DK_STREAM);
 	if (e == hipSuccess)
-		e = hipStreamSynchronize(g_ctx.stream);
+		e = hipStreamSynchronize(DK_STREAM);
 	if (e != hipSuccess) {
 		dk_set_error("copy failed: %s", hipGetErrorString(e));
 		rc = DK_E_DEVICE;
@@ -876,7 +911,7 @@ extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask
 						dout);
 	if (!rc)
 		rc = copy_to_host(out, dout, 2 * count);
-	(void)hipFreeAsync(buf, g_ctx.stream);
+	dk_device_free(buf, bytes);
 	return rc;
 }
 
@@ -889,7 +924,7 @@ extern "C" int dk_event_record(int slot)
 		dk_set_error("event slot out of range");
 		return DK_E_ARG;
 	}
-	DK_HIP(hipEventRecord(g_ctx.events[slot], g_ctx.stream));
+	DK_HIP(hipEventRecord(dk_cur()->events[slot], DK_STREAM));
 	return DK_OK;
 }
 
@@ -900,8 +935,8 @@ extern "C" int dk_event_elapsed(int a, int b, float *ms)
 		dk_set_error("event slot out of range");
 		return DK_E_ARG;
 	}
-	DK_HIP(hipEventSynchronize(g_ctx.events[b]));
-	DK_HIP(hipEventElapsedTime(ms, g_ctx.events[a], g_ctx.events[b]));
+	DK_HIP(hipEventSynchronize(dk_cur()->events[b]));
+	DK_HIP(hipEventElapsedTime(ms, dk_cur()->events[a], dk_cur()->events[b]));
 	return DK_OK;
 }
 
@@ -922,6 +957,11 @@ extern "C" int dk_tune(const char *key, int64_t value)
 	}
 	if (!strcmp(key, "dense_mfma")) {
 		g_ctx.dense_mfma = value != 0;
+		return DK_OK;
+	}
+	if (!strcmp(key, "arena_max_chunks")) {
+		g_ctx.arena_max_chunks = value > 0 ? (u64)value
+						   : g_ctx.arena_default_chunks;
 		return DK_OK;
 	}
 	dk_set_error("unknown tunable %s", key);

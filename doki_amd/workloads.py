@@ -137,3 +137,48 @@ def random_normalised_state(seed, n):
     v = rng.standard_normal(2 << n).view(np.complex128)
     v /= np.linalg.norm(v)
     return v
+
+
+SERVE_OPS = ("prob", "measure", "density", "join")
+
+
+def serving_qubits(rng, nmax=30):
+    """Register size of the served mix: 8-16 qubits with probability 0.70,
+    17-26 with 0.25, 27-30 with 0.05 (uniform inside each band); nmax caps
+    the top of every band (smaller test runs)."""
+    u = rng.random()
+    lo, hi = (8, 16) if u < 0.70 else (17, 26) if u < 0.95 else (27, 30)
+    hi = min(hi, nmax)
+    lo = min(lo, hi)
+    return int(rng.integers(lo, hi + 1))
+
+
+def serving_requests(seed, count, qps, nmax=30, depth=(20, 200)):
+    """Poisson arrivals at rate qps.  Each request holds one register (two for
+    a join) of serving_qubits() qubits running a random_gate_step circuit of
+    depth U[depth] gates from |0...0>, then one operation, uniform over
+    SERVE_OPS: the marginal probabilities of every qubit, a full measurement
+    with injected rolls, a reduced density over 1-4 random qubits, or the
+    join of two registers (first = high qubits) followed by the marginal
+    probabilities of the joined register.
+
+    Returns dicts: rid, arrival (s), op, regs [(nq, steps)], rolls, keep."""
+    rng = rng_for(seed)
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / qps))
+        op = SERVE_OPS[int(rng.integers(len(SERVE_OPS)))]
+        regs = []
+        for _ in range(2 if op == "join" else 1):
+            n = serving_qubits(rng, nmax)
+            d = int(rng.integers(depth[0], depth[1] + 1))
+            regs.append((n, [random_gate_step(rng, n) for _ in range(d)]))
+        n0 = regs[0][0]
+        rolls = rng.random(n0).tolist() if op == "measure" else None
+        keep = ()
+        if op == "density":
+            a = int(rng.integers(1, 5))
+            keep = tuple(sorted(int(q) for q in rng.choice(n0, size=a, replace=False)))
+        out.append(dict(rid=rid, arrival=t, op=op, regs=regs, rolls=rolls, keep=keep))
+    return out

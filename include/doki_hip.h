@@ -50,6 +50,7 @@ extern "C" {
 typedef struct dk_state dk_state;
 typedef struct dk_gate dk_gate;
 typedef struct dk_circuit dk_circuit;
+typedef struct dk_lane dk_lane;
 
 /* Library set-up on a HIP device (first call wins; later calls are no-ops).
  * No reference counterpart: the reference has no device. */
@@ -63,10 +64,35 @@ const char *dk_last_error(void);
  * synchronous). */
 int dk_synchronize(void);
 
-/* Device-memory budget in bytes for live states (default: 90% of free HBM at
+/* Execution lanes.  A lane is a HIP stream with its own reduction and
+ * measurement scratch; each host thread issues its work on the lane it bound
+ * last (the library's default lane until it binds one; dk_lane_bind(NULL)
+ * returns to it), so threads on different lanes run concurrently on the
+ * device.  An operation on a state last written on another lane first waits
+ * for that lane.  Reads of one state issued concurrently from several lanes
+ * are ordered by the caller.  dk_synchronize waits for the calling thread's
+ * lane.  dk_lane_free waits for the lane's work and keeps it for a later
+ * dk_lane_new; a thread still bound to it returns to the default lane.
+ * No reference counterpart: they serve concurrent request serving. */
+int dk_lane_new(dk_lane **out);
+int dk_lane_bind(dk_lane *lane);
+void dk_lane_free(dk_lane *lane);
+
+/* Device-memory budget in bytes for live states (default: 92% of free HBM at
  * init); allocations beyond it fail with DK_E_BUDGET.  `used` may be NULL. */
 int dk_memory_budget(uint64_t *budget, uint64_t *used);
 int dk_set_memory_budget(uint64_t budget);
+
+/* Device memory held by the library beyond live states: 2 GiB chunks of the
+ * state arena (all, and unmapped ones) and bytes of released blocks kept for
+ * reuse.  Any pointer may be NULL.  No reference counterpart. */
+int dk_memory_stats(uint64_t *arena_chunks, uint64_t *arena_free_chunks,
+		    uint64_t *cached_bytes);
+
+/* Returns cached blocks and every arena chunk no live state needs to the
+ * driver (which wipes memory before handing it out again, at tens of GB/s).
+ * No reference counterpart. */
+int dk_memory_trim(void);
 
 /* ---- states: qstate.c / qstate.h ---------------------------------------- */
 
@@ -234,8 +260,13 @@ void dk_batch_free(dk_batch *b);
 
 /* Settings: "export_states" (keep final states readable by dk_batch_state),
  * "use_graph" (capture the launch as a HIP graph, replay on later runs),
- * "pass_qubits" (target qubits per cache-blocked pass, 0 = tile width). */
+ * "pass_qubits" (target qubits per cache-blocked pass, 0 = tile width),
+ * "probs" (every register reports its marginal qubit probabilities). */
 int dk_batch_set(dk_batch *b, const char *key, int64_t value);
+
+/* Drops every register and keeps the device and pinned buffers for the next
+ * fill (a serving loop refills one batch per launch). */
+int dk_batch_clear(dk_batch *b);
 
 /* Adds a register.  matrices: num_gates 2x2 gates, 8 doubles each
  * (row-major, interleaved); targets/ctrl_masks/actrl_masks per gate, with
@@ -258,6 +289,10 @@ int dk_batch_size(const dk_batch *b, size_t *num_registers);
 int dk_batch_outcomes(const dk_batch *b, size_t i, int8_t *results,
 		      double *p1);
 
+/* Marginal P(qubit q = 1) of register i's final state for q = 0..nq-1, as
+ * registry_prob (doki.c:1194-1228) would return them (needs "probs"). */
+int dk_batch_probs(const dk_batch *b, size_t i, double *p);
+
 /* Reduced density of register i: 2^a x 2^a (a = kept qubits), row-major,
  * interleaved; kept qubits in ascending order are the index bits. */
 int dk_batch_density(const dk_batch *b, size_t i, double *rho);
@@ -275,7 +310,9 @@ int dk_event_record(int slot);
 int dk_event_elapsed(int first_slot, int second_slot, float *ms);
 /* Tunables: "apply_blocks" (grid cap of grid-stride kernels, 0 = auto),
  * "time_apply" (events 62/63 bracket every gate kernel), "dense_mfma"
- * (dense k >= 4 gates on MFMA, else on the vector ALUs). */
+ * (dense k >= 4 gates on MFMA, else on the vector ALUs), "arena_max_chunks"
+ * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the
+ * free memory at init less 6 GiB). */
 int dk_tune(const char *key, int64_t value);
 /* Compute units and architecture name of the library's device. */
 int dk_device_info(int *num_cus, char *arch, size_t arch_len);

@@ -1,0 +1,112 @@
+"""Device memory: arena-sized states (2 GiB and more of amplitudes, mapped
+onto 2 GiB physical chunks with their scalars in a block of their own), chunk
+reuse across sizes, and the budget."""
+import numpy as np
+import pytest
+
+from doki_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def check_ghz_like(capi, st, n):
+    # in place: H on the top qubit, then X on qubit 0 controlled by it
+    h = capi.Gate(W.gate_h())
+    x = capi.Gate(W.gate_x())
+    b = capi.apply(st, h, [n - 1], in_place=True)
+    capi.apply(b, x, [0], controls=[n - 1], in_place=True)
+    p = capi.prob_all(b)
+    want = np.zeros(n)
+    want[0] = want[n - 1] = 0.5
+    assert np.max(np.abs(p - want)) <= 1e-12
+    head, nrm = b.raw(0, 4)
+    assert abs(head[0] / nrm - np.sqrt(0.5)) <= 1e-12
+    tail, _ = b.raw((1 << (n - 1)) | 1, 1)
+    assert abs(tail[0] / nrm - np.sqrt(0.5)) <= 1e-12
+    return b
+
+
+def test_arena_states_and_reuse(capi):
+    for n in (27, 28, 27):
+        st = capi.new(n)
+        check_ghz_like(capi, st, n)
+        del st
+    capi.synchronize()
+    stats = capi.memory_stats()
+    assert stats["arena_chunks"] >= 2
+
+
+def pinned_chunks(capi):
+    """Chunks that live blocks hold, after everything else went back."""
+    capi.synchronize()
+    capi.memory_trim()
+    st = capi.memory_stats()
+    assert st["arena_free_chunks"] == 0 and st["cached_bytes"] == 0
+    return st["arena_chunks"]
+
+
+def test_arena_chunks_move_between_sizes(capi):
+    """With room for 8 more chunks, a 30-qubit state (8 chunks) takes the
+    chunks of released 29- and 28-qubit blocks; one more state beyond the
+    cap fails cleanly and leaves the library usable."""
+    # reduction scratch of this thread's lane sized for 30 qubits first
+    check_ghz_like(capi, capi.new(30), 30)
+    cap = pinned_chunks(capi) + 8
+    capi.tune("arena_max_chunks", cap)
+    try:
+        x = check_ghz_like(capi, capi.new(29), 29)
+        y = check_ghz_like(capi, capi.new(28), 28)
+        del x, y
+        z = check_ghz_like(capi, capi.new(30), 30)
+        with pytest.raises(capi.DokiHipError) as ei:
+            capi.new(27)
+        assert ei.value.code == 2  # DK_E_ALLOC
+        del z
+        check_ghz_like(capi, capi.new(27), 27)
+        assert capi.memory_stats()["arena_chunks"] == cap
+    finally:
+        capi.tune("arena_max_chunks", 0)
+
+
+def test_budget_refuses_and_recovers(capi):
+    import ctypes as C
+    L = capi.lib()
+    b0, u0 = C.c_uint64(), C.c_uint64()
+    capi.check(L.dk_memory_budget(C.byref(b0), C.byref(u0)))
+    capi.check(L.dk_set_memory_budget(u0.value + (1 << 24)))
+    try:
+        with pytest.raises(capi.DokiHipError) as ei:
+            capi.new(22)  # 64 MiB over a 16 MiB allowance
+        assert ei.value.code == 9  # DK_E_BUDGET
+        s = capi.new(18)
+        assert capi.prob_all(s).shape == (18,)
+    finally:
+        capi.check(L.dk_set_memory_budget(b0.value))
+
+
+def test_pieces_and_chunk_recycling(capi):
+    """States of 64 MiB to 1 GiB of amplitudes are buddy pieces of split
+    chunks; once released, their chunks serve whole-chunk states again."""
+    live = [check_ghz_like(capi, capi.new(n), n) for n in (22, 23, 24, 25, 26, 22, 26, 24)]
+    for st in live:
+        p = capi.prob_all(st)
+        assert abs(p[0] - 0.5) <= 1e-12
+    del live, st
+    cap = pinned_chunks(capi) + 3
+    capi.tune("arena_max_chunks", cap)
+    held = []
+    try:
+        # 3 chunks' worth of pieces, released into the piece cache
+        pieces = [capi.new(n) for n in (26, 26, 25, 25, 24, 24, 23, 23, 22, 22, 26)]
+        del pieces
+        while len(held) <= 3:
+            try:
+                held.append(capi.new(27))
+            except capi.DokiHipError as e:
+                assert e.code == 2
+                break
+        assert len(held) == 3
+        check_ghz_like(capi, held[-1], 27)
+    finally:
+        del held
+        capi.tune("arena_max_chunks", 0)
