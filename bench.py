@@ -93,6 +93,19 @@ def gpu_leg(args):
     kavg = sum(kms) / len(kms)
     log(f"[bench] gate kernel avg {kavg:.3f} ms -> {bpg / kavg / 1e6:.0f} GB/s")
 
+    # a plain copy of the state (registry_clone: 16 B read + 16 B written per
+    # amplitude, one kernel): the practical streaming ceiling on this device,
+    # and the kernel the HBM counters are calibrated on
+    copy_ms = []
+    for _ in range(3):
+        capi.event_record(0)
+        c = d.registry_clone(reg, -1, False)
+        capi.event_record(1)
+        copy_ms.append(capi.event_elapsed_ms(0, 1))
+        del c
+    log(f"[bench] state copy {min(copy_ms):.3f} ms -> "
+        f"{bpg / min(copy_ms) / 1e6:.0f} GB/s")
+
     # registry_prob on every qubit (end of the circuit)
     tp0 = time.perf_counter()
     probs = [d.registry_prob(reg, q, -1, False) for q in range(n)]
@@ -102,7 +115,7 @@ def gpu_leg(args):
     return {
         "elapsed": elapsed, "ngates": ngates, "bpg": bpg, "kavg_ms": kavg,
         "kernel_ms_layer": kms, "per_target": per_target,
-        "prob_all_ms": (tp1 - tp0) * 1e3,
+        "prob_all_ms": (tp1 - tp0) * 1e3, "copy_ms": min(copy_ms),
     }
 
 
@@ -438,6 +451,7 @@ def main():
         "cpu_baseline": cpu,
         "extras": {
             "gate_kernel_avg_ms": round(g["kavg_ms"], 4),
+            "state_copy_GBps": round(g["bpg"] / (g["copy_ms"] * 1e-3) / 1e9, 1),
             "per_target_kernel": g["per_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "fused": fused,

@@ -8,11 +8,13 @@
 // each a sequence of 16x16x4 MFMAs accumulating in the amplitude's precision
 // (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32).
 //
-// A block owns one tile of 64 groups (64 * D amplitudes, the gate's targets
-// plus the six lowest other index bits); wave w computes rows 16w..16w+15 of
-// all 64 columns (four 16-column tiles), its A fragments in registers.  The
-// tile is small enough that several blocks share a CU, so one block's loads
-// overlap another's matrix work.
+// A block owns one tile of 64 * CW groups (64 * CW * D amplitudes: the
+// gate's targets plus the 6 + log2(CW) lowest other index bits).  Its waves
+// form RT row tiles times CW column sets: wave (rt, cw) computes rows
+// 16rt..16rt+15 of the 64 columns of set cw (four 16-column tiles), its A
+// fragments in registers.  The tile is small enough that several blocks
+// share a CU, so one block's loads overlap another's matrix work; CW > 1
+// gives small gates (k = 4, one row tile) more bytes per block.
 // Operand layouts (16x16x4, wave64): lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; accumulator element r of lane l holds
 // D[out_row(l/16, r)][l%16] (see mfma16 below).
@@ -55,25 +57,30 @@ template <> struct mfma16<float> {
 	}
 };
 
-template <typename T, int K>
-__global__ __launch_bounds__(4 << K) void k_apply_dense(
+template <typename T, int K, int CW>
+__global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 	const T *in, T *out, const double *__restrict__ nrm,
-	const T *__restrict__ M, TileArgs ta, double *__restrict__ partials)
+	const T *__restrict__ M, TileArgs ta, double *__restrict__ partials,
+	u64 ntiles)
 {
 	typedef typename real_of<T>::type R;
 	typedef mfma16<R> MF;
 	typedef typename MF::acc acc_t;
-	constexpr int D = 1 << K;    // 16, 32, 64
-	constexpr int KS = D / 4;    // k-steps of the product
-	constexpr int RT = D / 16;   // row tiles = waves per block
-	constexpr int NT = 64 * RT;  // threads per block
-	constexpr int TS = 64 * D;   // tile: 64 groups of D amplitudes
+	constexpr int D = 1 << K;       // 16, 32, 64
+	constexpr int KS = D / 4;       // k-steps of the product
+	constexpr int RT = D / 16;      // row tiles
+	constexpr int NW = RT * CW;     // waves per block
+	constexpr int NT = 64 * NW;     // threads per block
+	constexpr int TS = 64 * CW * D; // tile: 64 * CW groups of D amplitudes
 	__shared__ T lds[TS];
-	__shared__ double red[RT];
+	__shared__ double red[NW];
 
-	const int lane = threadIdx.x & 63, rt = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int rt = wv % RT, cw = wv / RT;
 	const int li = lane & 15, lk = lane >> 4;
-	const u64 base = tile_base(blockIdx.x, ta);
+	// one pass per tile (more than one per block only beyond 2^31 threads)
+	for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+	const u64 base = tile_base(tile, ta);
 
 	// Slot s = threadIdx.x + j * NT: its index bits are this thread's part
 	// (computed once) OR the part of j * NT (the same for the whole block).
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(4 << K) void k_apply_dense(
 	int gs[4];
 #pragma unroll
 	for (int ct = 0; ct < 4; ++ct)
-		gs[ct] = group_slot<K>(ct * 16 + li, ta);
+		gs[ct] = group_slot<K>(cw * 64 + ct * 16 + li, ta);
 	int ks_slot[KS];
 #pragma unroll
 	for (int ks = 0; ks < KS; ++ks)
@@ -143,15 +150,27 @@ __global__ __launch_bounds__(4 << K) void k_apply_dense(
 	}
 	acc = wave_sum(acc);
 	if (lane == 0)
-		red[rt] = acc;
+		red[wv] = acc;
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		double t = 0.0;
 #pragma unroll
-		for (int w = 0; w < RT; ++w)
+		for (int w = 0; w < NW; ++w)
 			t += red[w];
-		partials[blockIdx.x] = t;
+		partials[tile] = t;
 	}
+	__syncthreads();
+	}
+}
+
+// Column sets per block for k = 4 (tunable "dense_k4_cw": 1, 2 or 4; 0
+// picks 2 for complex128 and 1 for complex64, the faster on gfx950 at 30
+// qubits: 6.2 against 10.7 ms, 3.3 against 3.4 ms).
+int k4_cw(int dtype)
+{
+	if (g_ctx.dense_k4_cw)
+		return g_ctx.dense_k4_cw;
+	return dtype == DK_C64 ? 1 : 2;
 }
 
 template <typename T>
@@ -163,19 +182,26 @@ int launch_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 	const T *src = (const T *)in->amp;
 	T *dst = (T *)out->amp;
 	hipStream_t st = DK_STREAM;
-	const dim3 grid((unsigned)nblocks);
+	const dim3 grid(dk_grid_cap(nblocks, (4u << g->k) * (g->k == 4 ? k4_cw(in->dtype) : 1)));
 	switch (g->k) {
 	case 4:
-		hipLaunchKernelGGL((k_apply_dense<T, 4>), grid, dim3(64), 0, st, src,
-				   dst, in->scal, M, ta, parts);
+		if (k4_cw(in->dtype) == 4)
+			hipLaunchKernelGGL((k_apply_dense<T, 4, 4>), grid, dim3(256), 0, st,
+					   src, dst, in->scal, M, ta, parts, nblocks);
+		else if (k4_cw(in->dtype) == 2)
+			hipLaunchKernelGGL((k_apply_dense<T, 4, 2>), grid, dim3(128), 0, st,
+					   src, dst, in->scal, M, ta, parts, nblocks);
+		else
+			hipLaunchKernelGGL((k_apply_dense<T, 4, 1>), grid, dim3(64), 0, st,
+					   src, dst, in->scal, M, ta, parts, nblocks);
 		break;
 	case 5:
-		hipLaunchKernelGGL((k_apply_dense<T, 5>), grid, dim3(128), 0, st, src,
-				   dst, in->scal, M, ta, parts);
+		hipLaunchKernelGGL((k_apply_dense<T, 5, 1>), grid, dim3(128), 0, st, src,
+				   dst, in->scal, M, ta, parts, nblocks);
 		break;
 	case 6:
-		hipLaunchKernelGGL((k_apply_dense<T, 6>), grid, dim3(256), 0, st, src,
-				   dst, in->scal, M, ta, parts);
+		hipLaunchKernelGGL((k_apply_dense<T, 6, 1>), grid, dim3(256), 0, st, src,
+				   dst, in->scal, M, ta, parts, nblocks);
 		break;
 	default:
 		dk_set_error("dense gates act on 4..6 qubits");
@@ -187,12 +213,25 @@ int launch_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 
 } // namespace
 
+unsigned dk_dense_tile_bits(unsigned k, int dtype)
+{
+	const int cw = k4_cw(dtype);
+	const unsigned cwb = k == 4 ? (cw == 4 ? 2 : cw == 2 ? 1 : 0) : 0;
+	return k + 6 + cwb;
+}
+
 int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 			  const unsigned *targets, double **parts, u64 *nparts)
 {
-	// one block per tile of 64 groups: targets plus the six lowest other bits
+	// one block per tile of 64 * CW groups: targets plus the 6 + log2(CW)
+	// lowest other bits
+	const unsigned tb = dk_dense_tile_bits(g->k, in->dtype);
+	if (in->nq < tb) {
+		dk_set_error("dense gate of %u qubits needs %u qubits of state", g->k, tb);
+		return DK_E_ARG;
+	}
 	TileArgs ta;
-	dk_tile_args(in->nq, g->k, targets, ta, g->k + 6);
+	dk_tile_args(in->nq, g->k, targets, ta, (int)tb);
 	const u64 nblocks = in->len >> ta.npos;
 	int rc = dk_partials_reserve(nblocks, parts);
 	if (rc)

@@ -107,6 +107,8 @@ struct dk_ctx {
 	int apply_blocks = 0;   // 0 = auto
 	int time_apply = 0;     // record events[62]/[63] around the gate kernel
 	int dense_mfma = 1;     // dense k>=4 gates on MFMA (else VALU tiles)
+	int apply1_group = 1;   // pair groups / waves per 1-qubit gate block
+	int dense_k4_cw = 0;    // column sets per dense k = 4 block (0 = auto)
 	u64 arena_max_chunks = 0;     // cap on 2 GiB state-arena chunks
 	u64 arena_default_chunks = 0; // the cap at init: free memory less 6 GiB
 };
@@ -147,12 +149,22 @@ bool dk_closed();
 
 // ---- kernel launchers (dk_kernels.hip, dk_dense.hip) --------------------------
 
+// Grid for `nvb` virtual blocks of `threads` threads: a launch holds fewer
+// than 2^32 threads, so beyond 2^31 the kernels loop over virtual blocks.
+static inline unsigned dk_grid_cap(u64 nvb, unsigned threads)
+{
+	const u64 cap = (1ull << 31) / threads;
+	return (unsigned)(nvb < 1 ? 1 : nvb < cap ? nvb : cap);
+}
+
 // Gate application; block partial sums of |out|^2 finish into out's norm on
 // the device.
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm);
 
-// Dense k-qubit gate (4 <= k <= 6, no controls, n >= 12) on MFMA.
+// Dense k-qubit gate (4 <= k <= 6, no controls) on MFMA, for states of at
+// least dk_dense_tile_bits(k) qubits (the index bits one block covers).
+unsigned dk_dense_tile_bits(unsigned k, int dtype);
 int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 			  const unsigned *targets, double **parts, u64 *nparts);
 

@@ -38,23 +38,26 @@ template <typename T> Mat2<T> mat2_of(const dk_gate *g)
 // amplitudes in flight form a sliding window over HBM.  Each block leaves one
 // partial |out|^2 sum.
 //
-// t >= 6: a block is two waves.  Wave 0 reads the lower amplitudes of 64
-// consecutive pairs (one contiguous run), wave 1 the partner run 2^t
-// amplitudes above; partners are exchanged through LDS.
+// t >= 6: a block holds G pair groups of two waves each.  In a group, wave
+// 0 reads the lower amplitudes of 64 consecutive pairs (one contiguous run),
+// wave 1 the partner run 2^t amplitudes above; partners are exchanged
+// through LDS.  One partial per block.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(128) void k_apply1_pairs(
+template <typename T, int G>
+__global__ __launch_bounds__(128 * G) void k_apply1_pairs(
 	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
 {
 	typedef typename real_of<T>::type R;
-	__shared__ T buf[128];
-	__shared__ double red[2];
+	__shared__ T buf[128 * G];
+	__shared__ double red[2 * G];
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	const u64 bit = 1ull << t, lo = bit - 1;
-	const u64 p = (u64)blockIdx.x * 64 + (threadIdx.x & 63);
+	// one pass per virtual block (more than one only beyond 2^31 threads)
+	for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+	const u64 p = (vb * G + (threadIdx.x >> 7)) * 64 + (threadIdx.x & 63);
 	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
-	const bool upper = threadIdx.x >= 64;
+	const bool upper = (threadIdx.x >> 6) & 1;
 	const u64 my = upper ? (i0 | bit) : i0;
 	const T v = cscale(in[my], inv);
 	buf[threadIdx.x] = v;
@@ -71,20 +74,29 @@ __global__ __launch_bounds__(128) void k_apply1_pairs(
 	if ((threadIdx.x & 63) == 0)
 		red[threadIdx.x >> 6] = s;
 	__syncthreads();
-	if (threadIdx.x == 0)
-		partials[blockIdx.x] = red[0] + red[1];
+	if (threadIdx.x == 0) {
+		double tot = 0.0;
+#pragma unroll
+		for (int w = 0; w < 2 * G; ++w)
+			tot += red[w];
+		partials[vb] = tot;
+	}
+	__syncthreads();
+	}
 }
 
-// t < 6: one wave per block over 64 consecutive amplitudes; the partner
-// amplitude comes from lane ^ 2^t.
-template <typename T>
-__global__ __launch_bounds__(64) void k_apply1_lanes(
+// t < 6: W waves per block, each over 64 consecutive amplitudes; the
+// partner amplitude comes from lane ^ 2^t.  One partial per block.
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void k_apply1_lanes(
 	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
 {
 	typedef typename real_of<T>::type R;
+	__shared__ double red[W];
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
-	const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
+	for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+	const u64 i = vb * 64 * W + threadIdx.x;
 	const T v = cscale(in[i], inv);
 	T w;
 	w.x = __shfl_xor(v.x, 1 << t);
@@ -98,8 +110,23 @@ __global__ __launch_bounds__(64) void k_apply1_lanes(
 	}
 	st_stream(out + i, o);
 	const double s = wave_sum(cabs2d(o));
-	if (threadIdx.x == 0)
-		partials[blockIdx.x] = s;
+	if (W == 1) {
+		if (threadIdx.x == 0)
+			partials[vb] = s;
+		continue;
+	}
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double tot = 0.0;
+#pragma unroll
+		for (int w = 0; w < W; ++w)
+			tot += red[w];
+		partials[vb] = tot;
+	}
+	__syncthreads();
+	}
 }
 
 // Small registers (fewer than 128 amplitudes): pairs, grid-stride, guarded.
@@ -445,7 +472,7 @@ __global__ __launch_bounds__(TPB) void k_normalised_copy(
 		TO o;
 		o.x = (decltype(o.x))(a.x * inv);
 		o.y = (decltype(o.y))(a.y * inv);
-		out[i] = o;
+		st_stream(out + i, o);
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		out_scal[DK_SCAL_NORM] = 1.0;
@@ -473,7 +500,7 @@ __global__ __launch_bounds__(TPB) void k_join(
 	     idx += (u64)gridDim.x * TPB) {
 		const T x = cscale(a[idx >> bbits], ia);
 		const T y = cscale(b[idx & bmask], ib);
-		o[idx] = cmul(x, y);
+		st_stream(o + idx, cmul(x, y));
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		o_scal[DK_SCAL_NORM] = 1.0;
@@ -488,7 +515,7 @@ __global__ __launch_bounds__(TPB) void k_fill_zero_state(T *__restrict__ amp,
 		T v = czero<T>();
 		if (i == 0)
 			v.x = 1;
-		amp[i] = v;
+		st_stream(amp + i, v);
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		scal[DK_SCAL_NORM] = 1.0;
@@ -657,6 +684,17 @@ __global__ __launch_bounds__(TPB) void k_density_elements(
 // ---------------------------------------------------------------------------
 // Host-side launch helpers
 // ---------------------------------------------------------------------------
+// One thread per unit (element-wise kernels): every load of the launch is in
+// flight at once, which a capped grid-stride loop cannot match on HBM.  A
+// launch holds fewer than 2^32 threads, so beyond 2^31 units the kernels'
+// loops take over.
+unsigned grid_full(u64 units, u64 per_block)
+{
+	const u64 b = (units + per_block - 1) / per_block;
+	return (unsigned)std::max<u64>(1, std::min<u64>(b, (1ull << 31) / per_block));
+}
+
+
 unsigned grid_for(u64 units, u64 per_block)
 {
 	u64 b = (units + per_block - 1) / per_block;
@@ -682,16 +720,37 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 	if (g->k == 1) {
 		const Mat2<T> M = mat2_of<T>(g);
 		const unsigned t = targets[0];
+		// pair groups / waves per block: 1, 2 or 4 (fewer, larger blocks
+		// leave fewer partials), never more than the state holds
+		int G = g_ctx.apply1_group;
+		while (G > 1 && in->len < 128ull * G)
+			G >>= 1;
 		if (in->len >= 128 && t >= 6) {
-			nblocks = in->len / 128;
-			hipLaunchKernelGGL(k_apply1_pairs<T>, dim3((unsigned)nblocks),
-					   dim3(128), 0, st, src, dst, in->scal, M, t, cm,
-					   acm, parts);
+			nblocks = in->len / (128ull * G);
+#define DK_PAIRS(GG)                                                           \
+	hipLaunchKernelGGL((k_apply1_pairs<T, GG>),                              \
+			   dim3(dk_grid_cap(nblocks, 128 * GG)), dim3(128 * GG), 0, \
+			   st, src, dst, in->scal, M, t, cm, acm, parts, nblocks)
+			if (G == 4)
+				DK_PAIRS(4);
+			else if (G == 2)
+				DK_PAIRS(2);
+			else
+				DK_PAIRS(1);
+#undef DK_PAIRS
 		} else if (in->len >= 128) {
-			nblocks = in->len / 64;
-			hipLaunchKernelGGL(k_apply1_lanes<T>, dim3((unsigned)nblocks),
-					   dim3(64), 0, st, src, dst, in->scal, M, t, cm,
-					   acm, parts);
+			nblocks = in->len / (64ull * G);
+#define DK_LANES(GG)                                                           \
+	hipLaunchKernelGGL((k_apply1_lanes<T, GG>),                              \
+			   dim3(dk_grid_cap(nblocks, 64 * GG)), dim3(64 * GG), 0,   \
+			   st, src, dst, in->scal, M, t, cm, acm, parts, nblocks)
+			if (G == 4)
+				DK_LANES(4);
+			else if (G == 2)
+				DK_LANES(2);
+			else
+				DK_LANES(1);
+#undef DK_LANES
 		} else {
 			nblocks = 1;
 			hipLaunchKernelGGL(k_apply1_small<T>, dim3(1), dim3(TPB), 0, st,
@@ -799,7 +858,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	if (g_ctx.time_apply)
 		DK_HIP(hipEventRecord(dk_cur()->events[62], st));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
-	    in->nq >= g->k + 6) {
+	    in->nq >= dk_dense_tile_bits(g->k, in->dtype)) {
 		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
 	} else {
 		rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
@@ -905,7 +964,10 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 		return DK_OK;
 	}
 	const u64 nch = s->len >> (6 + LU);
-	const unsigned nb = grid_for(nch, WAVES);
+	// four times the usual grid cap: more chunk loads in flight
+	const unsigned nb = (unsigned)std::min<u64>(
+		std::max<u64>(1, (nch + WAVES - 1) / WAVES),
+		4ull * grid_for(~0ull >> 1, 1));
 #define L(T)                                                                  \
 	hipLaunchKernelGGL((k_prob_all<T, LU>), dim3(nb), dim3(TPB), 0,        \
 			    DK_STREAM, (const T *)s->amp, s->scal, s->len,       \
@@ -921,7 +983,7 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
 {
-	const unsigned nb = grid_for(src->len, TPB * 4);
+	const unsigned nb = grid_full(src->len, TPB);
 	hipStream_t st = DK_STREAM;
 	if (src->dtype == DK_C128 && dst->dtype == DK_C128)
 		hipLaunchKernelGGL((k_normalised_copy<double2, double2>), dim3(nb),
@@ -979,7 +1041,7 @@ int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 
 int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
 {
-	const unsigned nb = grid_for(o->len, TPB * 4);
+	const unsigned nb = grid_full(o->len, TPB);
 #define L(T)                                                                  \
 	hipLaunchKernelGGL(k_join<T>, dim3(nb), dim3(TPB), 0, DK_STREAM,     \
 			    (const T *)a->amp, a->scal, (const T *)b->amp, b->scal, \
@@ -1087,7 +1149,7 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 
 int dk_launch_fill_zero_state(dk_state *s)
 {
-	const unsigned nb = grid_for(s->len, TPB * 4);
+	const unsigned nb = grid_full(s->len, TPB);
 #define L(T)                                                                  \
 	hipLaunchKernelGGL(k_fill_zero_state<T>, dim3(nb), dim3(TPB), 0,       \
 			    DK_STREAM, (T *)s->amp, s->scal, s->len)

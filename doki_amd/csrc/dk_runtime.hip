@@ -959,6 +959,22 @@ extern "C" int dk_tune(const char *key, int64_t value)
 		g_ctx.dense_mfma = value != 0;
 		return DK_OK;
 	}
+	if (!strcmp(key, "apply1_group")) {
+		if (value != 1 && value != 2 && value != 4) {
+			dk_set_error("apply1_group is 1, 2 or 4");
+			return DK_E_ARG;
+		}
+		g_ctx.apply1_group = (int)value;
+		return DK_OK;
+	}
+	if (!strcmp(key, "dense_k4_cw")) {
+		if (value != 0 && value != 1 && value != 2 && value != 4) {
+			dk_set_error("dense_k4_cw is 0 (auto), 1, 2 or 4");
+			return DK_E_ARG;
+		}
+		g_ctx.dense_k4_cw = (int)value;
+		return DK_OK;
+	}
 	if (!strcmp(key, "arena_max_chunks")) {
 		g_ctx.arena_max_chunks = value > 0 ? (u64)value
 						   : g_ctx.arena_default_chunks;

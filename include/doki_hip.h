@@ -310,7 +310,10 @@ int dk_event_record(int slot);
 int dk_event_elapsed(int first_slot, int second_slot, float *ms);
 /* Tunables: "apply_blocks" (grid cap of grid-stride kernels, 0 = auto),
  * "time_apply" (events 62/63 bracket every gate kernel), "dense_mfma"
- * (dense k >= 4 gates on MFMA, else on the vector ALUs), "arena_max_chunks"
+ * (dense k >= 4 gates on MFMA, else on the vector ALUs), "apply1_group"
+ * (1, 2 or 4 pair groups per block of the 1-qubit gate kernels), "dense_k4_cw"
+ * (1, 2 or 4 sets of 64 amplitude groups per dense 4-qubit block, 0 =
+ * by amplitude type), "arena_max_chunks"
  * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the
  * free memory at init less 6 GiB). */
 int dk_tune(const char *key, int64_t value);

@@ -127,3 +127,25 @@ def test_circuit_c64_fused(capi, oracle):
     want = oracle_run(oracle, oracle.new(n), steps).values()
     out = circ.run(capi.new_typed(n, capi.C64))
     assert maxabs(out.values(), want) <= C64_TOL
+
+
+@pytest.mark.parametrize("cw", [1, 2, 4])
+def test_dense_k4_column_sets_vs_oracle(capi, oracle, cw):
+    """Every block shape of the 4-qubit dense kernel (64, 128 or 256 groups
+    per block) against the oracle, in both amplitude types."""
+    n = 13
+    rng = W.rng_for(7700 + cw)
+    psi = W.random_normalised_state(77 + cw, n)
+    capi.tune("dense_k4_cw", cw)
+    try:
+        for dtype, tol in ((capi.C128, ATOL), (capi.C64, C64_TOL)):
+            d = capi.convert(capi.from_host(psi), dtype)
+            o = oracle.from_data(n, psi)
+            for rep in range(3):
+                targets = [int(x) for x in rng.choice(n, size=4, replace=False)]
+                u = W.haar_unitary(rng, 4)
+                d = capi.apply(d, capi.Gate(u), targets)
+                o = oracle.apply(o, u, targets)
+                assert maxabs(d.values(), o.values()) <= tol, (dtype, targets)
+    finally:
+        capi.tune("dense_k4_cw", 0)

@@ -212,6 +212,25 @@ def test_one_qubit_targets_vs_oracle(capi, oracle, t):
         assert abs(d.norm - o.norm) <= ATOL
 
 
+@pytest.mark.parametrize("group", [2, 4])
+def test_one_qubit_block_groupings_vs_oracle(capi, oracle, group):
+    """The 1-qubit kernels with 2 or 4 pair groups (waves) per block."""
+    n = 15
+    psi = W.random_normalised_state(40 + group, n)
+    capi.tune("apply1_group", group)
+    try:
+        for t in (0, 4, 5, 6, 9, 14):
+            u = W.gate_u3(0.7 + t, 0.4, 1.9)
+            for ctrl, actrl in [((), ()), (((t + 1) % n,), ((t + 2) % n,))]:
+                d = capi.apply(capi.from_host(psi), capi.Gate(u), [t], ctrl, actrl)
+                o = oracle.apply(oracle.from_data(n, psi), u, [t], list(ctrl),
+                                 list(actrl))
+                assert maxabs(d.values(), o.values()) <= ATOL, (t, ctrl, actrl)
+                assert abs(d.norm - o.norm) <= ATOL
+    finally:
+        capi.tune("apply1_group", 1)
+
+
 def test_in_place_equals_out_of_place(capi):
     n = 14
     psi = W.random_normalised_state(3, n)
