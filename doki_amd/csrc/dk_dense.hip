@@ -57,11 +57,13 @@ template <> struct mfma16<float> {
 	}
 };
 
+// One tile: load it, multiply on the matrix cores, store it, leave its
+// partial |out|^2 sum.
 template <typename T, int K, int CW>
-__global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
+__device__ __forceinline__ void dense_tile(
 	const T *in, T *out, const double *__restrict__ nrm,
-	const T *__restrict__ M, TileArgs ta, double *__restrict__ partials,
-	u64 ntiles)
+	const T *__restrict__ M, const TileArgs &ta, double *__restrict__ partials,
+	u64 tile, T *lds, double *red)
 {
 	typedef typename real_of<T>::type R;
 	typedef mfma16<R> MF;
@@ -72,14 +74,10 @@ __global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 	constexpr int NW = RT * CW;     // waves per block
 	constexpr int NT = 64 * NW;     // threads per block
 	constexpr int TS = 64 * CW * D; // tile: 64 * CW groups of D amplitudes
-	__shared__ T lds[TS];
-	__shared__ double red[NW];
 
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int rt = wv % RT, cw = wv / RT;
 	const int li = lane & 15, lk = lane >> 4;
-	// one pass per tile (more than one per block only beyond 2^31 threads)
-	for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 	const u64 base = tile_base(tile, ta);
 
 	// Slot s = threadIdx.x + j * NT: its index bits are this thread's part
@@ -159,7 +157,25 @@ __global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 			t += red[w];
 		partials[tile] = t;
 	}
-	__syncthreads();
+}
+
+// LOOP: tiles beyond the grid (launches of more than 2^31 threads); the
+// single-tile form keeps the fragment set-up after the tile's loads.
+template <typename T, int K, int CW, bool LOOP>
+__global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
+	const T *in, T *out, const double *__restrict__ nrm,
+	const T *__restrict__ M, TileArgs ta, double *__restrict__ partials,
+	u64 ntiles)
+{
+	__shared__ T lds[64 * CW * (1 << K)];
+	__shared__ double red[(1 << K) / 16 * CW];
+	if constexpr (LOOP) {
+		for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+			dense_tile<T, K, CW>(in, out, nrm, M, ta, partials, tile, lds, red);
+			__syncthreads();
+		}
+	} else {
+		dense_tile<T, K, CW>(in, out, nrm, M, ta, partials, blockIdx.x, lds, red);
 	}
 }
 
@@ -173,6 +189,20 @@ int k4_cw(int dtype)
 	return dtype == DK_C64 ? 1 : 2;
 }
 
+template <typename T, int K, int CW>
+void launch_shape(const T *src, T *dst, const double *scal, const T *M,
+		  const TileArgs &ta, double *parts, u64 nblocks)
+{
+	constexpr unsigned NT = (4u << K) * CW;
+	const dim3 grid(dk_grid_cap(nblocks, NT));
+	if (grid.x < nblocks)
+		hipLaunchKernelGGL((k_apply_dense<T, K, CW, true>), grid, dim3(NT), 0,
+				   DK_STREAM, src, dst, scal, M, ta, parts, nblocks);
+	else
+		hipLaunchKernelGGL((k_apply_dense<T, K, CW, false>), grid, dim3(NT), 0,
+				   DK_STREAM, src, dst, scal, M, ta, parts, nblocks);
+}
+
 template <typename T>
 int launch_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 		 const TileArgs &ta, u64 nblocks, double *parts)
@@ -181,27 +211,20 @@ int launch_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 						      : (const T *)g->dmat32;
 	const T *src = (const T *)in->amp;
 	T *dst = (T *)out->amp;
-	hipStream_t st = DK_STREAM;
-	const dim3 grid(dk_grid_cap(nblocks, (4u << g->k) * (g->k == 4 ? k4_cw(in->dtype) : 1)));
 	switch (g->k) {
 	case 4:
 		if (k4_cw(in->dtype) == 4)
-			hipLaunchKernelGGL((k_apply_dense<T, 4, 4>), grid, dim3(256), 0, st,
-					   src, dst, in->scal, M, ta, parts, nblocks);
+			launch_shape<T, 4, 4>(src, dst, in->scal, M, ta, parts, nblocks);
 		else if (k4_cw(in->dtype) == 2)
-			hipLaunchKernelGGL((k_apply_dense<T, 4, 2>), grid, dim3(128), 0, st,
-					   src, dst, in->scal, M, ta, parts, nblocks);
+			launch_shape<T, 4, 2>(src, dst, in->scal, M, ta, parts, nblocks);
 		else
-			hipLaunchKernelGGL((k_apply_dense<T, 4, 1>), grid, dim3(64), 0, st,
-					   src, dst, in->scal, M, ta, parts, nblocks);
+			launch_shape<T, 4, 1>(src, dst, in->scal, M, ta, parts, nblocks);
 		break;
 	case 5:
-		hipLaunchKernelGGL((k_apply_dense<T, 5, 1>), grid, dim3(128), 0, st, src,
-				   dst, in->scal, M, ta, parts, nblocks);
+		launch_shape<T, 5, 1>(src, dst, in->scal, M, ta, parts, nblocks);
 		break;
 	case 6:
-		hipLaunchKernelGGL((k_apply_dense<T, 6, 1>), grid, dim3(256), 0, st, src,
-				   dst, in->scal, M, ta, parts, nblocks);
+		launch_shape<T, 6, 1>(src, dst, in->scal, M, ta, parts, nblocks);
 		break;
 	default:
 		dk_set_error("dense gates act on 4..6 qubits");

@@ -44,17 +44,12 @@ template <typename T> Mat2<T> mat2_of(const dk_gate *g)
 // through LDS.  One partial per block.
 // ---------------------------------------------------------------------------
 template <typename T, int G>
-__global__ __launch_bounds__(128 * G) void k_apply1_pairs(
-	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+__device__ __forceinline__ void pairs_block(
+	const T *in, T *out, typename real_of<T>::type inv, const Mat2<T> &M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 vb,
+	T *buf, double *red)
 {
-	typedef typename real_of<T>::type R;
-	__shared__ T buf[128 * G];
-	__shared__ double red[2 * G];
-	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	const u64 bit = 1ull << t, lo = bit - 1;
-	// one pass per virtual block (more than one only beyond 2^31 threads)
-	for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
 	const u64 p = (vb * G + (threadIdx.x >> 7)) * 64 + (threadIdx.x & 63);
 	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
 	const bool upper = (threadIdx.x >> 6) & 1;
@@ -81,21 +76,37 @@ __global__ __launch_bounds__(128 * G) void k_apply1_pairs(
 			tot += red[w];
 		partials[vb] = tot;
 	}
-	__syncthreads();
+}
+
+// LOOP: virtual blocks beyond the grid (launches of more than 2^31 threads).
+template <typename T, int G, bool LOOP>
+__global__ __launch_bounds__(128 * G) void k_apply1_pairs(
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+{
+	typedef typename real_of<T>::type R;
+	__shared__ T buf[128 * G];
+	__shared__ double red[2 * G];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	if constexpr (LOOP) {
+		for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+			pairs_block<T, G>(in, out, inv, M, t, cm, acm, partials, vb, buf, red);
+			__syncthreads();
+		}
+	} else {
+		pairs_block<T, G>(in, out, inv, M, t, cm, acm, partials, blockIdx.x, buf,
+				  red);
 	}
 }
 
 // t < 6: W waves per block, each over 64 consecutive amplitudes; the
 // partner amplitude comes from lane ^ 2^t.  One partial per block.
 template <typename T, int W>
-__global__ __launch_bounds__(64 * W) void k_apply1_lanes(
-	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+__device__ __forceinline__ void lanes_block(
+	const T *in, T *out, typename real_of<T>::type inv, const Mat2<T> &M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 vb,
+	double *red)
 {
-	typedef typename real_of<T>::type R;
-	__shared__ double red[W];
-	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
-	for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
 	const u64 i = vb * 64 * W + threadIdx.x;
 	const T v = cscale(in[i], inv);
 	T w;
@@ -113,7 +124,7 @@ __global__ __launch_bounds__(64 * W) void k_apply1_lanes(
 	if (W == 1) {
 		if (threadIdx.x == 0)
 			partials[vb] = s;
-		continue;
+		return;
 	}
 	if ((threadIdx.x & 63) == 0)
 		red[threadIdx.x >> 6] = s;
@@ -125,7 +136,23 @@ __global__ __launch_bounds__(64 * W) void k_apply1_lanes(
 			tot += red[w];
 		partials[vb] = tot;
 	}
-	__syncthreads();
+}
+
+template <typename T, int W, bool LOOP>
+__global__ __launch_bounds__(64 * W) void k_apply1_lanes(
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+{
+	typedef typename real_of<T>::type R;
+	__shared__ double red[W];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	if constexpr (LOOP) {
+		for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+			lanes_block<T, W>(in, out, inv, M, t, cm, acm, partials, vb, red);
+			__syncthreads();
+		}
+	} else {
+		lanes_block<T, W>(in, out, inv, M, t, cm, acm, partials, blockIdx.x, red);
 	}
 }
 
@@ -728,9 +755,17 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		if (in->len >= 128 && t >= 6) {
 			nblocks = in->len / (128ull * G);
 #define DK_PAIRS(GG)                                                           \
-	hipLaunchKernelGGL((k_apply1_pairs<T, GG>),                              \
-			   dim3(dk_grid_cap(nblocks, 128 * GG)), dim3(128 * GG), 0, \
-			   st, src, dst, in->scal, M, t, cm, acm, parts, nblocks)
+	do {                                                                   \
+		const unsigned gr = dk_grid_cap(nblocks, 128 * GG);            \
+		if (gr < nblocks)                                              \
+			hipLaunchKernelGGL((k_apply1_pairs<T, GG, true>), dim3(gr), \
+					   dim3(128 * GG), 0, st, src, dst, in->scal, \
+					   M, t, cm, acm, parts, nblocks);         \
+		else                                                           \
+			hipLaunchKernelGGL((k_apply1_pairs<T, GG, false>), dim3(gr), \
+					   dim3(128 * GG), 0, st, src, dst, in->scal, \
+					   M, t, cm, acm, parts, nblocks);         \
+	} while (0)
 			if (G == 4)
 				DK_PAIRS(4);
 			else if (G == 2)
@@ -741,9 +776,17 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		} else if (in->len >= 128) {
 			nblocks = in->len / (64ull * G);
 #define DK_LANES(GG)                                                           \
-	hipLaunchKernelGGL((k_apply1_lanes<T, GG>),                              \
-			   dim3(dk_grid_cap(nblocks, 64 * GG)), dim3(64 * GG), 0,   \
-			   st, src, dst, in->scal, M, t, cm, acm, parts, nblocks)
+	do {                                                                   \
+		const unsigned gr = dk_grid_cap(nblocks, 64 * GG);             \
+		if (gr < nblocks)                                              \
+			hipLaunchKernelGGL((k_apply1_lanes<T, GG, true>), dim3(gr), \
+					   dim3(64 * GG), 0, st, src, dst, in->scal, \
+					   M, t, cm, acm, parts, nblocks);         \
+		else                                                           \
+			hipLaunchKernelGGL((k_apply1_lanes<T, GG, false>), dim3(gr), \
+					   dim3(64 * GG), 0, st, src, dst, in->scal, \
+					   M, t, cm, acm, parts, nblocks);         \
+	} while (0)
 			if (G == 4)
 				DK_LANES(4);
 			else if (G == 2)
