@@ -383,6 +383,22 @@ def cpu_leg(args, gb_per_gate_unit=True):
                       f"{g} gates at n={n} through the single-threaded oracle port"}
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the 1-qubit gate kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (copy-kernel
+    calibrated); None when no profile covers this register size."""
+    path = os.path.join(ROOT, "profiles", f"apply1_{n}q_c128_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    return int(float(vals["traffic_bytes_per_launch"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -446,7 +462,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.qubits),
         },
         "cpu_baseline": cpu,
         "extras": {

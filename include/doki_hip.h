@@ -4,7 +4,8 @@
  * Plain C: opaque handles, pointers and sizes only.  States live in device
  * memory (HBM) for their whole life; only scalars (probabilities, outcomes,
  * norms) and explicit read-outs cross to the host.  All work is enqueued on
- * the library's HIP stream; calls that return host values synchronise it.
+ * the calling thread's lane (a HIP stream; see dk_lane_new); calls that
+ * return host values synchronise it.
  *
  * Each entry point replaces one function of the reference's internal C API
  * (Mowstyl/Doki 1.6.1, src/doki/qops.h, qstate.h) or one step of its Python
@@ -71,7 +72,9 @@ int dk_synchronize(void);
  * device.  An operation on a state last written on another lane first waits
  * for that lane.  Reads of one state issued concurrently from several lanes
  * are ordered by the caller.  dk_synchronize waits for the calling thread's
- * lane.  dk_lane_free waits for the lane's work and keeps it for a later
+ * lane.  A lane serves one thread at a time: threads that call the library
+ * concurrently each bind a lane of their own (the default lane included).
+ * dk_lane_free waits for the lane's work and keeps it for a later
  * dk_lane_new; a thread still bound to it returns to the default lane.
  * No reference counterpart: they serve concurrent request serving. */
 int dk_lane_new(dk_lane **out);
