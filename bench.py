@@ -119,6 +119,59 @@ def gpu_leg(args):
     }
 
 
+def big_leg(args):
+    """The 33-qubit variant of the brickwork workload (137.4 GB register):
+    one layer through the drop-in module, out of place as registry_apply is
+    (old and new state together: 274.9 GB), and the same layer in place
+    through the C ABI; per-gate HBM rates and the gate kernel's duration."""
+    import doki_amd
+    from doki_amd import capi, workloads as W
+    d = doki_amd.doki
+    n = args.big_qubits
+    bpg = 32 * (1 << n)
+    steps = W.brickwork_layer(W.rng_for(3333), n, 0)
+    out = {"qubits": n, "state_GB": round(16 * (1 << n) / 1e9, 2),
+           "gates": len(steps), "bytes_per_gate": bpg}
+    gates = [(d.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
+    # (loops inline: a caller's reference would keep a third state alive)
+    reg = d.registry_new(n, False)
+    g0, t_, c_ = gates[0]
+    reg = d.registry_apply(reg, g0, t_, c_, None, -1, False)  # second state
+    capi.synchronize()
+    t0 = time.perf_counter()
+    for g, t_, c_ in gates:
+        reg = d.registry_apply(reg, g, t_, c_, None, -1, False)
+    capi.synchronize()
+    el = time.perf_counter() - t0
+    out["out_of_place_GBps"] = round(len(gates) * bpg / el / 1e9, 1)
+    out["out_of_place_ms_per_gate"] = round(el / len(gates) * 1e3, 3)
+    del reg
+    capi.synchronize()
+    s = capi.new(n)
+    cg = [(capi.Gate(m), t, c) for m, t, c, a in steps]
+    capi.apply(s, cg[0][0], cg[0][1], cg[0][2], in_place=True)
+    capi.synchronize()
+    capi.tune("time_apply", 1)
+    kms = []
+    t0 = time.perf_counter()
+    for g, t, c in cg:
+        capi.apply(s, g, t, c, in_place=True)
+        kms.append(capi.event_elapsed_ms(62, 63))
+    capi.synchronize()
+    el = time.perf_counter() - t0
+    capi.tune("time_apply", 0)
+    kavg = sum(kms) / len(kms)
+    out["in_place_GBps"] = round(len(cg) * bpg / el / 1e9, 1)
+    out["kernel_avg_ms"] = round(kavg, 3)
+    out["kernel_GBps"] = round(bpg / (kavg * 1e-3) / 1e9, 1)
+    p = capi.prob_all(s)
+    assert all(0.0 <= x <= 1.0 + 1e-12 for x in p)
+    del s
+    capi.synchronize()
+    log(f"[bench] {n} qubits: {out}")
+    return out
+
+
 def fused_leg(args):
     """Fused dense gates on the matrix cores at the same register size:
     per-k kernel rates (FP64 complex128 and FP32 complex64), the brickwork
@@ -414,6 +467,8 @@ def main():
     ap.add_argument("--batch-registers", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--big-qubits", type=int, default=33,
+                    help="qubits of the large-register leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=60.0)
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -427,6 +482,7 @@ def main():
         sys.exit("--steps must be >= 1")
 
     g = gpu_leg(args)
+    big = big_leg(args) if args.big_qubits else None
     fused = None if args.no_fused else fused_leg(args)
     batched = None if args.no_batch else batch_leg(args)
     served = None if args.no_serve else serve_leg(args)
@@ -470,6 +526,7 @@ def main():
             "state_copy_GBps": round(g["bpg"] / (g["copy_ms"] * 1e-3) / 1e9, 1),
             "per_target_kernel": g["per_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
+            "large_register": big,
             "fused": fused,
             "batched": batched,
             "served": served,
