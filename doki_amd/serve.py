@@ -30,6 +30,7 @@ import time
 import numpy as np
 
 from . import capi
+from . import tune
 
 SMALL_QUBITS = 14
 
@@ -93,7 +94,10 @@ def _finish(h, result=None, error=None):
 
 def build_register(nq, packed, dtype, fuse_k):
     """|0...0> of nq qubits with the packed circuit applied in place, fused
-    into dense blocks when the register is wide enough for them."""
+    into dense blocks when the register is wide enough for them; fuse_k
+    "auto" takes the tuned width for (nq, dtype)."""
+    if fuse_k == "auto":
+        fuse_k = tune.best_fuse_k(nq, dtype)
     st = capi.new_typed(nq, dtype)
     circ = capi.Circuit(nq)
     circ.add_packed(packed)
@@ -124,7 +128,7 @@ def run_operation(spec, states, dtype):
 class Server:
     """Continuous-batching request server on the library's device."""
 
-    def __init__(self, workers=3, small_qubits=SMALL_QUBITS, fuse_k=5,
+    def __init__(self, workers=3, small_qubits=SMALL_QUBITS, fuse_k="auto",
                  budget=None, max_batch=2048, dtype=capi.C128):
         self.dtype = dtype
         self.small_qubits = small_qubits
