@@ -84,6 +84,13 @@ def lib():
         "dk_circuit_block": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_uint),
                                        C.POINTER(C.c_uint), dp]),
         "dk_circuit_run": (C.c_int, [P, P, C.c_int, PP]),
+        "dk_circuit_passes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "dk_circuit_pass": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_uint),
+                                      C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+        "dk_circuit_pass_ops": (C.c_int, [P, C.c_size_t, P, C.c_size_t,
+                                          C.POINTER(C.c_size_t)]),
+        "dk_circuit_plan_step": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_size_t)]),
         "dk_batch_new": (C.c_int, [C.c_int, PP]),
         "dk_batch_free": (None, [P]),
         "dk_batch_set": (C.c_int, [P, C.c_char_p, C.c_int64]),
@@ -281,6 +288,15 @@ def dtype_of(s):
     return lib().dk_state_dtype(s.ptr)
 
 
+# One op of a cache-blocked pass (struct PassOp of doki_amd/csrc/dk_pass.h).
+PASS_OP = np.dtype([
+    ("kind", "<u4"), ("a", "<u4"), ("b", "<u4"), ("rmask", "<u4"),
+    ("tcm", "<u8"), ("tam", "<u8"), ("m", "<f8", (8,)),
+])
+assert PASS_OP.itemsize == 96
+OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH = range(5)
+
+
 class Circuit:
     """A gate list with optional fusion into dense blocks (C ABI dk_circuit_*)."""
 
@@ -320,6 +336,39 @@ class Circuit:
         n = C.c_size_t()
         check(lib().dk_circuit_fuse(self.ptr, k, C.byref(n)))
         return n.value
+
+    def passes(self):
+        """Plans cache-blocked passes (dk_circuit_passes); returns their count."""
+        n = C.c_size_t()
+        check(lib().dk_circuit_passes(self.ptr, C.byref(n)))
+        return n.value
+
+    def plan(self):
+        """The pass plan in execution order: ("pass", i) or ("gate", i)."""
+        out = []
+        L = lib()
+        isp, idx = C.c_int(), C.c_size_t()
+        i = 0
+        while L.dk_circuit_plan_step(self.ptr, i, C.byref(isp), C.byref(idx)) == 0:
+            out.append(("pass" if isp.value else "gate", idx.value))
+            i += 1
+        return out
+
+    def pass_info(self, i):
+        """(gates absorbed, ops, 12 tile qubits) of pass i."""
+        ng, no = C.c_uint(), C.c_uint()
+        tq = (C.c_uint * 12)()
+        check(lib().dk_circuit_pass(self.ptr, i, C.byref(ng), C.byref(no), tq))
+        return ng.value, no.value, list(tq)
+
+    def pass_ops(self, i):
+        """The op records of pass i as a structured numpy array (PASS_OP)."""
+        cnt = C.c_size_t()
+        check(lib().dk_circuit_pass_ops(self.ptr, i, None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=PASS_OP)
+        check(lib().dk_circuit_pass_ops(self.ptr, i, out.ctypes.data_as(C.c_void_p),
+                                        cnt.value, C.byref(cnt)))
+        return out
 
     def block(self, i):
         m = C.c_uint()

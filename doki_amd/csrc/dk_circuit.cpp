@@ -9,12 +9,23 @@
 // b touches Q (so it commutes past every block it moves over) and b's qubit
 // set stays within K.  Among admissible blocks the one gaining the fewest new
 // qubits wins (ties: the earliest); otherwise a new block opens at the end.
+//
+// Cache-blocked passes (dk_circuit_passes) are the other plan: each pass
+// streams the state once and applies every one-qubit gate whose target lies
+// in the pass's 12-qubit tile (qubits 0..5 and six chosen ones), plus every
+// diagonal gate wherever its target is; controls may sit anywhere.  Gates
+// move ahead of gates left for later passes only when they commute with
+// them (checked on the qubits each gate uses non-diagonally).
 #include <algorithm>
 #include <complex>
 #include <cstring>
 #include <vector>
 
 #include "dk_internal.h"
+#include "dk_pass.h"
+
+int dk_launch_pass(const dk_state *in, dk_state *out, const dkp::PassDesc &pd,
+		   const dkp::PassOp *ops_dev);
 
 typedef std::complex<double> cd;
 
@@ -88,11 +99,31 @@ void apply_local(const CGate &g, const std::vector<unsigned> &qubits,
 
 } // namespace
 
+// A step of a pass plan: a pass (index into passes) or one gate run alone.
+struct PlanItem {
+	bool pass;
+	int idx;
+};
+
+struct PassPlan {
+	dkp::PassDesc desc;
+	size_t first_op;
+	unsigned ngates;
+};
+
 struct dk_circuit {
 	unsigned nq;
 	std::vector<CGate> gates;
 	std::vector<CBlock> blocks;
 	unsigned fuse_k = 0;
+	// cache-blocked pass plan
+	bool passes_planned = false;
+	std::vector<PlanItem> plan;
+	std::vector<PassPlan> passes;
+	std::vector<dkp::PassOp> pops;
+	dkp::PassOp *pops_dev = nullptr;
+	size_t pops_dev_bytes = 0;
+	dk_lane *pops_lane = nullptr;
 };
 
 extern "C" int dk_circuit_new(unsigned num_qubits, dk_circuit **out)
@@ -109,6 +140,14 @@ extern "C" int dk_circuit_new(unsigned num_qubits, dk_circuit **out)
 
 static void drop_device_gates(dk_circuit *c)
 {
+	if (c->pops_dev) {
+		if (!dk_closed()) {
+			(void)hipStreamSynchronize(c->pops_lane->stream);
+			dk_device_free(c->pops_dev, c->pops_dev_bytes);
+		}
+		c->pops_dev = nullptr;
+		c->pops_dev_bytes = 0;
+	}
 	for (auto &b : c->blocks)
 		if (b.dev) {
 			dk_gate_free(b.dev);
@@ -169,6 +208,7 @@ extern "C" int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
 	drop_device_gates(c);
 	c->blocks.clear();
 	c->fuse_k = 0;
+	c->passes_planned = false;
 	return DK_OK;
 }
 
@@ -181,6 +221,7 @@ extern "C" int dk_circuit_fuse(dk_circuit *c, unsigned max_qubits,
 	}
 	drop_device_gates(c);
 	c->blocks.clear();
+	c->passes_planned = false;
 	c->fuse_k = max_qubits;
 	if (max_qubits == 0) {
 		if (num_blocks)
@@ -281,6 +322,441 @@ extern "C" int dk_circuit_block(const dk_circuit *c, size_t i,
 	return DK_OK;
 }
 
+// ---- cache-blocked pass plans ---------------------------------------------------
+
+namespace {
+
+using dkp::PassOp;
+
+// A one-qubit gate is diagonal when its off-diagonal entries are exactly 0.
+bool is_diag(const CGate &g)
+{
+	return g.k == 1 && g.m[1] == cd(0, 0) && g.m[2] == cd(0, 0);
+}
+
+// Qubits a gate uses non-diagonally (its targets, unless it is a diagonal
+// one-qubit gate) and diagonally (controls, and a diagonal gate's target).
+// Two gates commute when neither's non-diagonal qubits meet the other's
+// qubits: on every shared qubit both act diagonally.
+void gate_sets(const CGate &g, u64 &N, u64 &D)
+{
+	u64 tm = 0;
+	for (unsigned b = 0; b < g.k; ++b)
+		tm |= 1ull << g.t[b];
+	if (is_diag(g)) {
+		N = 0;
+		D = tm | g.cm | g.am;
+	} else {
+		N = tm;
+		D = g.cm | g.am;
+	}
+}
+
+// 2x2 product a * b (row-major).
+void mul22(const cd *a, const cd *b, cd *r)
+{
+	cd t[4];
+	t[0] = a[0] * b[0] + a[1] * b[2];
+	t[1] = a[0] * b[1] + a[1] * b[3];
+	t[2] = a[2] * b[0] + a[3] * b[2];
+	t[3] = a[2] * b[1] + a[3] * b[3];
+	for (int i = 0; i < 4; ++i)
+		r[i] = t[i];
+}
+
+// Builds the op list of one pass: tile qubits `high` (six of them, qubits
+// >= 6), gates `gs` in an order that respects their commutation.
+struct PassBuilder {
+	const dk_circuit *c;
+	std::vector<PassOp> &ops;
+	int slot_of[64];
+	unsigned rsl[dkp::REG_BITS], wsl[dkp::WAVE_BITS];
+	// the last gate op since the last exchange, for merging
+	long last = -1;
+	unsigned last_q = 0;
+	u64 last_cm = 0, last_am = 0;
+
+	PassBuilder(const dk_circuit *c_, std::vector<PassOp> &ops_) : c(c_), ops(ops_)
+	{
+		for (int q = 0; q < 64; ++q)
+			slot_of[q] = -1;
+	}
+
+	// Location of qubit q under the current mapping: 0 lane (*pos = bit),
+	// 1 register (*pos = j), 2 wave (*pos = b), 3 fixed (*pos = q).
+	int where(unsigned q, unsigned *pos) const
+	{
+		if (q < (unsigned)dkp::LANE_BITS) {
+			*pos = q;
+			return 0;
+		}
+		const int h = slot_of[q];
+		if (h < 0) {
+			*pos = q;
+			return 3;
+		}
+		for (unsigned j = 0; j < (unsigned)dkp::REG_BITS; ++j)
+			if (rsl[j] == (unsigned)h) {
+				*pos = j;
+				return 1;
+			}
+		for (unsigned b = 0; b < (unsigned)dkp::WAVE_BITS; ++b)
+			if (wsl[b] == (unsigned)h) {
+				*pos = b;
+				return 2;
+			}
+		*pos = q;
+		return 3; // not reached: every high slot is a register or wave slot
+	}
+
+	bool executable(const CGate &g) const
+	{
+		if (is_diag(g))
+			return true;
+		unsigned pos;
+		const int w = where(g.t[0], &pos);
+		return w == 0 || w == 1;
+	}
+
+	// Splits a control mask into register bits (returned, 4 bits) and the
+	// qubits of the thread word (*thread).
+	unsigned split_mask(u64 m, uint64_t *thread) const
+	{
+		unsigned reg = 0;
+		*thread = 0;
+		for (; m; m &= m - 1) {
+			const unsigned q = (unsigned)__builtin_ctzll(m);
+			unsigned pos;
+			if (where(q, &pos) == 1)
+				reg |= 1u << pos;
+			else
+				*thread |= 1ull << q;
+		}
+		return reg;
+	}
+
+	void emit_gate(const CGate &g)
+	{
+		const unsigned q = g.t[0];
+		cd m[4] = {g.m[0], g.m[1], g.m[2], g.m[3]};
+		if (last >= 0 && last_q == q && last_cm == g.cm && last_am == g.am) {
+			// same target and controls as the previous op: one product
+			const PassOp &o = ops[(size_t)last];
+			cd prev[4];
+			for (int i = 0; i < 4; ++i)
+				prev[i] = cd(o.m[2 * i], o.m[2 * i + 1]);
+			mul22(m, prev, m);
+			ops.pop_back(); // the previous op is rebuilt below
+		}
+		PassOp o;
+		memset(&o, 0, sizeof(o));
+		for (int i = 0; i < 4; ++i) {
+			o.m[2 * i] = m[i].real();
+			o.m[2 * i + 1] = m[i].imag();
+		}
+		const bool diag = m[1] == cd(0, 0) && m[2] == cd(0, 0);
+		unsigned pos;
+		const int w = where(q, &pos);
+		if (diag) {
+			o.kind = w == 1 ? dkp::OP_DIAG_REG : dkp::OP_DIAG_THREAD;
+			o.a = w == 1 ? pos : q;
+		} else {
+			o.kind = w == 0 ? dkp::OP_LANE : dkp::OP_REG;
+			o.a = pos;
+		}
+		o.rmask = split_mask(g.cm, &o.tcm) | split_mask(g.am, &o.tam) << 8;
+		ops.push_back(o);
+		last = (long)ops.size() - 1;
+		last_q = q;
+		last_cm = g.cm;
+		last_am = g.am;
+	}
+
+	void emit_exchange(const unsigned (&nr)[dkp::REG_BITS])
+	{
+		bool used[dkp::HIGH_BITS] = {false};
+		for (int j = 0; j < dkp::REG_BITS; ++j)
+			used[nr[j]] = true;
+		unsigned nw[dkp::WAVE_BITS], k = 0;
+		for (unsigned h = 0; h < (unsigned)dkp::HIGH_BITS; ++h)
+			if (!used[h])
+				nw[k++] = h;
+		PassOp o;
+		memset(&o, 0, sizeof(o));
+		o.kind = dkp::OP_EXCH;
+		for (int j = 0; j < dkp::REG_BITS; ++j) {
+			rsl[j] = nr[j];
+			o.a |= nr[j] << (3 * j);
+		}
+		for (int b = 0; b < dkp::WAVE_BITS; ++b) {
+			wsl[b] = nw[b];
+			o.b |= nw[b] << (3 * b);
+		}
+		ops.push_back(o);
+		last = -1;
+	}
+};
+
+// Whether ops [first, first+n) are well formed for the pass kernel: every
+// exchange names a permutation of the high slots, every bit index is in
+// range.  The kernel indexes LDS and registers by these fields.
+bool pass_ops_valid(const PassOp *ops, size_t n, unsigned nq)
+{
+	for (size_t i = 0; i < n; ++i) {
+		const PassOp &o = ops[i];
+		switch (o.kind) {
+		case dkp::OP_EXCH: {
+			unsigned seen = 0;
+			for (int j = 0; j < dkp::REG_BITS; ++j)
+				seen |= 1u << ((o.a >> (3 * j)) & 7);
+			for (int b = 0; b < dkp::WAVE_BITS; ++b)
+				seen |= 1u << ((o.b >> (3 * b)) & 7);
+			if (seen != (1u << dkp::HIGH_BITS) - 1 ||
+			    (o.a >> (3 * dkp::REG_BITS)) || (o.b >> (3 * dkp::WAVE_BITS)))
+				return false;
+			continue;
+		}
+		case dkp::OP_REG:
+		case dkp::OP_DIAG_REG:
+			if (o.a >= (unsigned)dkp::REG_BITS)
+				return false;
+			break;
+		case dkp::OP_LANE:
+			if (o.a >= (unsigned)dkp::LANE_BITS)
+				return false;
+			break;
+		case dkp::OP_DIAG_THREAD:
+			if (o.a >= nq)
+				return false;
+			break;
+		default:
+			return false;
+		}
+		const u64 valid = nq >= 64 ? ~0ull : (1ull << nq) - 1;
+		if ((o.rmask & ~0x0f0fu) || (o.tcm & ~valid) || (o.tam & ~valid))
+			return false;
+	}
+	return true;
+}
+
+// One pass over tile qubits `chosen` (high qubits, first-used first) and the
+// gates `gs` (in circuit order, all absorbable).
+void build_pass(dk_circuit *c, std::vector<unsigned> chosen,
+		const std::vector<int> &gs)
+{
+	// fill the tile to six high qubits
+	u64 cmask = 0;
+	for (unsigned q : chosen)
+		cmask |= 1ull << q;
+	for (unsigned q = dkp::LANE_BITS; chosen.size() < (size_t)dkp::HIGH_BITS; ++q)
+		if (!((cmask >> q) & 1)) {
+			chosen.push_back(q);
+			cmask |= 1ull << q;
+		}
+	PassPlan pp;
+	memset(&pp.desc, 0, sizeof(pp.desc));
+	pp.first_op = c->pops.size();
+	pp.ngates = (unsigned)gs.size();
+	PassBuilder B(c, c->pops);
+	// high slot h <- chosen[h]: the first four used start in registers
+	for (int h = 0; h < dkp::HIGH_BITS; ++h) {
+		pp.desc.hpos[h] = (uint8_t)chosen[h];
+		B.slot_of[chosen[h]] = h;
+	}
+	std::vector<unsigned> srt(chosen.begin(), chosen.end());
+	std::sort(srt.begin(), srt.end());
+	for (int h = 0; h < dkp::HIGH_BITS; ++h)
+		pp.desc.hsorted[h] = (uint8_t)srt[h];
+	for (int j = 0; j < dkp::REG_BITS; ++j)
+		B.rsl[j] = (unsigned)j;
+	for (int b = 0; b < dkp::WAVE_BITS; ++b)
+		B.wsl[b] = (unsigned)(dkp::REG_BITS + b);
+
+	std::vector<int> pending(gs.begin(), gs.end());
+	while (!pending.empty()) {
+		// the first gate that can run now and commutes with every pending
+		// gate before it
+		u64 bN = 0, bD = 0;
+		size_t pick = pending.size();
+		for (size_t i = 0; i < pending.size(); ++i) {
+			const CGate &g = c->gates[pending[i]];
+			u64 N, D;
+			gate_sets(g, N, D);
+			if (!(N & (bN | bD)) && !(D & bN) && B.executable(g)) {
+				pick = i;
+				break;
+			}
+			bN |= N;
+			bD |= D;
+		}
+		if (pick < pending.size()) {
+			B.emit_gate(c->gates[pending[pick]]);
+			pending.erase(pending.begin() + (long)pick);
+			continue;
+		}
+		// nothing runs under this mapping: bring the high targets of the
+		// next pending gates into registers (the first pending gate's
+		// target among them)
+		unsigned nr[dkp::REG_BITS];
+		int cnt = 0;
+		bool in_nr[dkp::HIGH_BITS] = {false};
+		for (size_t i = 0; i < pending.size() && cnt < dkp::REG_BITS; ++i) {
+			const CGate &g = c->gates[pending[i]];
+			if (is_diag(g) || g.t[0] < (unsigned)dkp::LANE_BITS)
+				continue;
+			const int h = B.slot_of[g.t[0]];
+			if (!in_nr[h]) {
+				in_nr[h] = true;
+				nr[cnt++] = (unsigned)h;
+			}
+		}
+		// keep current register slots in the remaining places
+		for (int j = 0; j < dkp::REG_BITS && cnt < dkp::REG_BITS; ++j)
+			if (!in_nr[B.rsl[j]]) {
+				in_nr[B.rsl[j]] = true;
+				nr[cnt++] = B.rsl[j];
+			}
+		B.emit_exchange(nr);
+	}
+	pp.desc.nops = (uint32_t)(c->pops.size() - pp.first_op);
+	c->passes.push_back(pp);
+	c->plan.push_back({true, (int)c->passes.size() - 1});
+}
+
+} // namespace
+
+extern "C" int dk_circuit_passes(dk_circuit *c, size_t *num_passes)
+{
+	if (!c) {
+		dk_set_error("NULL circuit");
+		return DK_E_ARG;
+	}
+	drop_device_gates(c);
+	c->blocks.clear();
+	c->fuse_k = 0;
+	c->plan.clear();
+	c->passes.clear();
+	c->pops.clear();
+	c->passes_planned = true;
+	std::vector<int> rem(c->gates.size());
+	for (size_t i = 0; i < rem.size(); ++i)
+		rem[i] = (int)i;
+	if (c->nq < (unsigned)dkp::PASS_TILE_BITS) {
+		// too small for a tile: every gate runs alone
+		for (int gi : rem)
+			c->plan.push_back({false, gi});
+		rem.clear();
+	}
+	while (!rem.empty()) {
+		const CGate &g0 = c->gates[rem[0]];
+		if (g0.k != 1) {
+			c->plan.push_back({false, rem[0]});
+			rem.erase(rem.begin());
+			continue;
+		}
+		std::vector<unsigned> chosen;
+		u64 cmask = 0;
+		std::vector<int> absorbed, deferred;
+		u64 bN = 0, bD = 0;
+		for (int gi : rem) {
+			const CGate &g = c->gates[gi];
+			u64 N, D;
+			gate_sets(g, N, D);
+			bool take = false;
+			if (g.k == 1 && !(N & (bN | bD)) && !(D & bN)) {
+				const unsigned t = g.t[0];
+				if (is_diag(g) || t < (unsigned)dkp::LANE_BITS || ((cmask >> t) & 1)) {
+					take = true;
+				} else if (chosen.size() < (size_t)dkp::HIGH_BITS) {
+					chosen.push_back(t);
+					cmask |= 1ull << t;
+					take = true;
+				}
+			}
+			if (take) {
+				absorbed.push_back(gi);
+			} else {
+				deferred.push_back(gi);
+				bN |= N;
+				bD |= D;
+			}
+		}
+		rem.swap(deferred);
+		if (absorbed.size() == 1 && !is_diag(c->gates[absorbed[0]])) {
+			// one dense gate: the one-qubit kernel does it alone
+			c->plan.push_back({false, absorbed[0]});
+			continue;
+		}
+		build_pass(c, chosen, absorbed);
+	}
+	for (const PassPlan &pp : c->passes)
+		if (!pass_ops_valid(&c->pops[pp.first_op], pp.desc.nops, c->nq)) {
+			dk_set_error("internal error: malformed pass plan");
+			c->passes_planned = false;
+			return DK_E_ARG;
+		}
+	if (num_passes)
+		*num_passes = c->passes.size();
+	return DK_OK;
+}
+
+extern "C" int dk_circuit_pass(const dk_circuit *c, size_t i, unsigned *num_gates,
+			       unsigned *num_ops, unsigned *tile_qubits)
+{
+	if (!c || !c->passes_planned || i >= c->passes.size()) {
+		dk_set_error("pass index out of range");
+		return DK_E_ARG;
+	}
+	const PassPlan &pp = c->passes[i];
+	if (num_gates)
+		*num_gates = pp.ngates;
+	if (num_ops)
+		*num_ops = pp.desc.nops;
+	if (tile_qubits) {
+		for (int b = 0; b < dkp::LANE_BITS; ++b)
+			tile_qubits[b] = (unsigned)b;
+		for (int h = 0; h < dkp::HIGH_BITS; ++h)
+			tile_qubits[dkp::LANE_BITS + h] = pp.desc.hpos[h];
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_circuit_pass_ops(const dk_circuit *c, size_t i, void *ops,
+				   size_t capacity, size_t *count)
+{
+	static_assert(sizeof(PassOp) == 96, "PassOp records are 96 bytes");
+	if (!c || !c->passes_planned || i >= c->passes.size()) {
+		dk_set_error("pass index out of range");
+		return DK_E_ARG;
+	}
+	const PassPlan &pp = c->passes[i];
+	if (count)
+		*count = pp.desc.nops;
+	if (ops) {
+		if (capacity < pp.desc.nops) {
+			dk_set_error("room for %zu ops, the pass has %u", capacity,
+				     (unsigned)pp.desc.nops);
+			return DK_E_ARG;
+		}
+		memcpy(ops, &c->pops[pp.first_op], pp.desc.nops * sizeof(PassOp));
+	}
+	return DK_OK;
+}
+
+extern "C" int dk_circuit_plan_step(const dk_circuit *c, size_t i, int *is_pass,
+				    size_t *index)
+{
+	if (!c || !c->passes_planned || i >= c->plan.size()) {
+		dk_set_error("plan step out of range");
+		return DK_E_ARG;
+	}
+	if (is_pass)
+		*is_pass = c->plan[i].pass ? 1 : 0;
+	if (index)
+		*index = (size_t)c->plan[i].idx;
+	return DK_OK;
+}
+
 static int gate_to_device(unsigned k, const std::vector<cd> &m, dk_gate **out)
 {
 	std::vector<double> flat(2 * m.size());
@@ -328,7 +804,62 @@ extern "C" int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place,
 		first = false;
 		return (int)DK_OK;
 	};
-	if (c->fuse_k > 0) {
+	auto step_pass = [&](const dkp::PassDesc &pd, const PassOp *ops_dev) {
+		dk_state *o = cur;
+		if (first) {
+			int r = dk_alloc_state(cur->nq, cur->dtype, &o);
+			if (r)
+				return r;
+		}
+		int r = dk_launch_pass(cur, o, pd, ops_dev);
+		if (r) {
+			if (o != cur)
+				dk_release_state(o);
+			return r;
+		}
+		o->lane = dk_cur();
+		cur = o;
+		first = false;
+		return (int)DK_OK;
+	};
+	if (c->passes_planned) {
+		if (!c->pops.empty() && !c->pops_dev) {
+			const size_t bytes = c->pops.size() * sizeof(PassOp);
+			rc = dk_device_alloc(bytes, (void **)&c->pops_dev);
+			if (rc)
+				return rc;
+			c->pops_dev_bytes = bytes;
+			c->pops_lane = dk_cur();
+			// the host ops stay untouched until the plan is dropped, which
+			// waits for this lane first
+			hipError_t e = hipMemcpyAsync(c->pops_dev, c->pops.data(), bytes,
+						      hipMemcpyHostToDevice, DK_STREAM);
+			if (e != hipSuccess) {
+				dk_set_error("pass upload failed: %s", hipGetErrorString(e));
+				return DK_E_DEVICE;
+			}
+		} else if (c->pops_dev && c->pops_lane != dk_cur()) {
+			// uploaded (or last read) on another lane
+			DK_HIP(hipStreamSynchronize(c->pops_lane->stream));
+			c->pops_lane = dk_cur();
+		}
+		for (const PlanItem &it : c->plan) {
+			if (it.pass) {
+				const PassPlan &pp = c->passes[(size_t)it.idx];
+				rc = step_pass(pp.desc, c->pops_dev + pp.first_op);
+			} else {
+				CGate &g = c->gates[(size_t)it.idx];
+				if (!g.dev) {
+					rc = gate_to_device(g.k, g.m, &g.dev);
+					if (rc)
+						break;
+				}
+				rc = step(g.dev, g.t, g.cm, g.am);
+			}
+			if (rc)
+				break;
+		}
+	} else if (c->fuse_k > 0) {
 		for (auto &b : c->blocks) {
 			if (!b.dev) {
 				rc = gate_to_device((unsigned)b.qubits.size(), b.U, &b.dev);

@@ -54,7 +54,7 @@ __device__ __forceinline__ void pairs_block(
 	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
 	const bool upper = (threadIdx.x >> 6) & 1;
 	const u64 my = upper ? (i0 | bit) : i0;
-	const T v = cscale(in[my], inv);
+	const T v = cscale(ld_stream(in + my), inv);
 	buf[threadIdx.x] = v;
 	__syncthreads();
 	const T w = buf[threadIdx.x ^ 64];
@@ -108,7 +108,7 @@ __device__ __forceinline__ void lanes_block(
 	double *red)
 {
 	const u64 i = vb * 64 * W + threadIdx.x;
-	const T v = cscale(in[i], inv);
+	const T v = cscale(ld_stream(in + i), inv);
 	T w;
 	w.x = __shfl_xor(v.x, 1 << t);
 	w.y = __shfl_xor(v.y, 1 << t);
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(TPB) void k_prob(
 	double acc = 0.0;
 	for (u64 j = (u64)blockIdx.x * TPB + threadIdx.x; j < half;
 	     j += (u64)gridDim.x * TPB) {
-		const T a = in[((j & ~lo) << 1) | bit | (j & lo)];
+		const T a = ld_stream(in + (((j & ~lo) << 1) | bit | (j & lo)));
 		const double x = a.x * inv, y = a.y * inv;
 		acc += fma(x, x, y * y);
 	}
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 		const T *p = in + (c << (6 + LU)) + lane;
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
-			const T a = p[(u64)u * 64];
+			const T a = ld_stream(p + (u64)u * 64);
 			const double re = a.x * inv, im = a.y * inv;
 			x[u] = fma(re, re, im * im);
 		}
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(TPB) void k_collapse(
 	double acc = 0.0;
 	for (u64 j = (u64)blockIdx.x * TPB + threadIdx.x; j < olen;
 	     j += (u64)gridDim.x * TPB) {
-		const T o = cscale(in[((j & ~lo) << 1) | off | (j & lo)], inv);
+		const T o = cscale(ld_stream(in + (((j & ~lo) << 1) | off | (j & lo))), inv);
 		out[j] = o;
 		if (next_q >= 0 && ((j >> next_q) & 1)) {
 			const double x = o.x * inv_o, y = o.y * inv_o;
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(TPB) void k_normalised_copy(
 	const double inv = 1.0 / nrm[DK_SCAL_NORM];
 	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len;
 	     i += (u64)gridDim.x * TPB) {
-		const TI a = in[i];
+		const TI a = ld_stream(in + i);
 		TO o;
 		o.x = (decltype(o.x))(a.x * inv);
 		o.y = (decltype(o.y))(a.y * inv);

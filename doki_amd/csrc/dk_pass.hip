@@ -1,0 +1,418 @@
+// Cache-blocked multi-gate pass kernel (see dk_pass.h for the tile, the
+// mapping and the op encoding).
+//
+// Per tile: every thread loads its 16 amplitudes (scaled by 1/norm, the
+// reference's state_get, qstate.h:86), runs the pass's ops on them in
+// registers, and stores them back with one partial |out|^2 sum per tile.
+// Each gate keeps registry_apply's semantics (qops.c:176-261): amplitudes
+// whose controls are off keep their value.  The state is renormalised once
+// per pass instead of once per gate; all gates are unitary, so the values
+// agree with per-gate renormalisation to rounding.
+//
+// In place is safe: a workgroup has read its whole tile before it writes.
+#include <algorithm>
+
+#include "dk_device.h"
+#include "dk_pass.h"
+
+using namespace dkd;
+using namespace dkp;
+
+namespace {
+
+// A complex value as two reals (register arrays of these promote to
+// registers reliably; the HIP vector types do not always).
+template <typename R> struct Cx {
+	R re, im;
+};
+
+template <typename R> struct M22 {
+	Cx<R> m00, m01, m10, m11;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cx_sel(bool c, const Cx<R> &a, const Cx<R> &b)
+{
+	return {c ? a.re : b.re, c ? a.im : b.im};
+}
+
+// m * (xr + i xi)
+template <typename R>
+__device__ __forceinline__ void cx_mul(const Cx<R> &m, R xr, R xi, R &orr, R &oi)
+{
+	orr = fma(m.re, xr, -m.im * xi);
+	oi = fma(m.re, xi, m.im * xr);
+}
+
+// m0 * x + m1 * y
+template <typename R>
+__device__ __forceinline__ void row2(const Cx<R> &m0, R xr, R xi, const Cx<R> &m1,
+				     R yr, R yi, R &orr, R &oi)
+{
+	R ar, ai;
+	cx_mul(m0, xr, xi, ar, ai);
+	ar = fma(m1.re, yr, ar);
+	ar = fma(-m1.im, yi, ar);
+	ai = fma(m1.re, yi, ai);
+	ai = fma(m1.im, yr, ai);
+	orr = ar;
+	oi = ai;
+}
+
+// Whether register r passes the register-bit part of a control condition
+// (uniform: r is a compile-time index, rmask an op constant).
+__device__ __forceinline__ bool reg_ok(int r, unsigned rmask)
+{
+	const unsigned rcm = rmask & 0xff, ram = rmask >> 8;
+	return ((unsigned)r & rcm) == rcm && ((unsigned)r & ram) == 0;
+}
+
+// Dense 2x2 gate on register bit J.  SEL: the op has controls, and register
+// r of a thread takes the new value only where ok and its register bits
+// pass; otherwise every amplitude takes it.
+template <int J, bool SEL, typename R>
+__device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
+					 const M22<R> &M, bool ok, unsigned rmask)
+{
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		if (r & (1 << J))
+			continue;
+		constexpr int d = 1 << J;
+		R o0r, o0i, o1r, o1i;
+		row2(M.m00, vr[r], vi[r], M.m01, vr[r | d], vi[r | d], o0r, o0i);
+		row2(M.m10, vr[r], vi[r], M.m11, vr[r | d], vi[r | d], o1r, o1i);
+		if (SEL) {
+			const bool c = ok & reg_ok(r, rmask);
+			o0r = c ? o0r : vr[r];
+			o0i = c ? o0i : vi[r];
+			o1r = c ? o1r : vr[r | d];
+			o1i = c ? o1i : vi[r | d];
+		}
+		vr[r] = o0r;
+		vi[r] = o0i;
+		vr[r | d] = o1r;
+		vi[r | d] = o1i;
+	}
+}
+
+template <bool SEL, typename R>
+__device__ __forceinline__ void gate_reg_any(R (&vr)[REGS], R (&vi)[REGS],
+					     const M22<R> &M, bool ok, unsigned rmask,
+					     unsigned j)
+{
+	// the empty asm markers keep the compiler from merging the four bodies
+	// into one with a variable register index (which would move the tile to
+	// scratch memory)
+	switch (j) {
+	case 0:
+		gate_reg<0, SEL>(vr, vi, M, ok, rmask);
+		asm volatile("; register bit 0");
+		break;
+	case 1:
+		gate_reg<1, SEL>(vr, vi, M, ok, rmask);
+		asm volatile("; register bit 1");
+		break;
+	case 2:
+		gate_reg<2, SEL>(vr, vi, M, ok, rmask);
+		asm volatile("; register bit 2");
+		break;
+	default:
+		gate_reg<3, SEL>(vr, vi, M, ok, rmask);
+		asm volatile("; register bit 3");
+		break;
+	}
+}
+
+// Dense 2x2 gate on lane bit b: the partner amplitude is in lane ^ 2^b.
+template <bool SEL, typename R>
+__device__ __forceinline__ void gate_lane(R (&vr)[REGS], R (&vi)[REGS],
+					  const M22<R> &M, bool ok, unsigned rmask,
+					  unsigned b, unsigned lane)
+{
+	const bool upper = (lane >> b) & 1;
+	const Cx<R> a0 = cx_sel(upper, M.m11, M.m00); // own amplitude
+	const Cx<R> a1 = cx_sel(upper, M.m10, M.m01); // partner's
+	const int d = 1 << b;
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		const R pr = __shfl_xor(vr[r], d), pi = __shfl_xor(vi[r], d);
+		R orr, oi;
+		row2(a0, vr[r], vi[r], a1, pr, pi, orr, oi);
+		if (SEL) {
+			const bool c = ok & reg_ok(r, rmask);
+			orr = c ? orr : vr[r];
+			oi = c ? oi : vi[r];
+		}
+		vr[r] = orr;
+		vi[r] = oi;
+	}
+}
+
+// Diagonal gate: factor m00 where the qubit is 0, m11 where it is 1.  The
+// qubit is register bit a (REG) or bit a of the thread word.
+template <bool SEL, bool REG, typename R>
+__device__ __forceinline__ void gate_diag(R (&vr)[REGS], R (&vi)[REGS],
+					  const M22<R> &M, bool ok, unsigned rmask,
+					  unsigned a, u64 tb)
+{
+	const Cx<R> ft = cx_sel((tb >> a) & 1, M.m11, M.m00);
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		const Cx<R> f = REG ? cx_sel((r >> a) & 1, M.m11, M.m00) : ft;
+		R orr, oi;
+		cx_mul(f, vr[r], vi[r], orr, oi);
+		if (SEL) {
+			const bool c = ok & reg_ok(r, rmask);
+			orr = c ? orr : vr[r];
+			oi = c ? oi : vi[r];
+		}
+		vr[r] = orr;
+		vi[r] = oi;
+	}
+}
+
+template <typename R>
+__device__ __forceinline__ M22<R> op_matrix(const PassOp *__restrict__ op)
+{
+	M22<R> M;
+	M.m00 = {(R)op->m[0], (R)op->m[1]};
+	M.m01 = {(R)op->m[2], (R)op->m[3]};
+	M.m10 = {(R)op->m[4], (R)op->m[5]};
+	M.m11 = {(R)op->m[6], (R)op->m[7]};
+	return M;
+}
+
+template <bool SEL, typename R>
+__device__ __forceinline__ void run_gate(R (&vr)[REGS], R (&vi)[REGS], unsigned kind,
+					 unsigned a, unsigned rmask, const M22<R> &M,
+					 bool ok, u64 tb, unsigned lane)
+{
+	switch (kind) {
+	case OP_REG:
+		gate_reg_any<SEL>(vr, vi, M, ok, rmask, a);
+		break;
+	case OP_LANE:
+		gate_lane<SEL>(vr, vi, M, ok, rmask, a, lane);
+		break;
+	case OP_DIAG_REG:
+		gate_diag<SEL, true>(vr, vi, M, ok, rmask, a, tb);
+		break;
+	default: // OP_DIAG_THREAD
+		gate_diag<SEL, false>(vr, vi, M, ok, rmask, a, tb);
+		break;
+	}
+}
+
+// High slot bits held by register r / by wave w under a mapping.
+__device__ __forceinline__ unsigned reg_slots(int r, const unsigned (&rs)[REG_BITS])
+{
+	unsigned s = 0;
+#pragma unroll
+	for (int j = 0; j < REG_BITS; ++j)
+		if ((r >> j) & 1)
+			s |= 1u << rs[j];
+	return s;
+}
+
+__device__ __forceinline__ unsigned wave_slots(unsigned w, const unsigned (&ws)[WAVE_BITS])
+{
+	unsigned s = 0;
+#pragma unroll
+	for (int b = 0; b < WAVE_BITS; ++b)
+		if ((w >> b) & 1)
+			s |= 1u << ws[b];
+	return s;
+}
+
+// Index bits of the high slot bits in `slots`.
+__device__ __forceinline__ u64 slot_index(unsigned slots, const u64 (&hb)[HIGH_BITS])
+{
+	u64 x = 0;
+#pragma unroll
+	for (int h = 0; h < HIGH_BITS; ++h)
+		if ((slots >> h) & 1)
+			x |= hb[h];
+	return x;
+}
+
+template <typename T>
+__device__ __forceinline__ void pass_tile(
+	const T *in, T *out, typename real_of<T>::type inv, const PassDesc &pd,
+	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 tile,
+	T *lds, double *red)
+{
+	typedef typename real_of<T>::type R;
+	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	u64 base = tile << LANE_BITS;
+#pragma unroll
+	for (int h = 0; h < HIGH_BITS; ++h)
+		base = ins0(base, pd.hsorted[h]);
+	u64 hb[HIGH_BITS];
+#pragma unroll
+	for (int h = 0; h < HIGH_BITS; ++h)
+		hb[h] = 1ull << pd.hpos[h];
+	// mapping at load: register bit j = high slot j, wave bit b = slot 4 + b
+	unsigned rs[REG_BITS], ws[WAVE_BITS];
+#pragma unroll
+	for (int j = 0; j < REG_BITS; ++j)
+		rs[j] = (unsigned)j;
+#pragma unroll
+	for (int b = 0; b < WAVE_BITS; ++b)
+		ws[b] = (unsigned)(REG_BITS + b);
+
+	// the thread word: index bits shared by all of the thread's registers
+	u64 tb = base | lane | slot_index(wave_slots(w, ws), hb);
+	R vr[REGS], vi[REGS];
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		const T x = ld_stream(in + (tb | slot_index(reg_slots(r, rs), hb)));
+		vr[r] = x.x * inv;
+		vi[r] = x.y * inv;
+	}
+
+	for (unsigned i = 0; i < pd.nops; ++i) {
+		const PassOp *__restrict__ op = ops + i;
+		const unsigned kind = op->kind, a = op->a, rmask = op->rmask;
+		const u64 tcm = op->tcm, tam = op->tam;
+		if (kind == OP_EXCH) {
+			__syncthreads(); // the previous exchange's reads are done
+			{
+				const unsigned wsl = wave_slots(w, ws);
+#pragma unroll
+				for (int r = 0; r < REGS; ++r)
+					lds[lane | ((wsl | reg_slots(r, rs)) << LANE_BITS)] =
+						mk<T>(vr[r], vi[r]);
+			}
+			const unsigned b = op->b;
+#pragma unroll
+			for (int j = 0; j < REG_BITS; ++j)
+				rs[j] = (a >> (3 * j)) & 7;
+#pragma unroll
+			for (int k = 0; k < WAVE_BITS; ++k)
+				ws[k] = (b >> (3 * k)) & 7;
+			__syncthreads();
+			{
+				const unsigned wsl = wave_slots(w, ws);
+#pragma unroll
+				for (int r = 0; r < REGS; ++r) {
+					const T x = lds[lane | ((wsl | reg_slots(r, rs)) << LANE_BITS)];
+					vr[r] = x.x;
+					vi[r] = x.y;
+				}
+			}
+			tb = base | lane | slot_index(wave_slots(w, ws), hb);
+			continue;
+		}
+		const M22<R> M = op_matrix<R>(op);
+		if ((tcm | tam | rmask) != 0) {
+			const bool ok = ((tb & tcm) == tcm) & ((tb & tam) == 0);
+			run_gate<true>(vr, vi, kind, a, rmask, M, ok, tb, lane);
+		} else {
+			run_gate<false>(vr, vi, kind, a, rmask, M, true, tb, lane);
+		}
+	}
+
+	double s = 0.0;
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		st_stream(out + (tb | slot_index(reg_slots(r, rs), hb)), mk<T>(vr[r], vi[r]));
+		s += fma((double)vr[r], (double)vr[r], (double)vi[r] * (double)vi[r]);
+	}
+	s = wave_sum(s);
+	if (lane == 0)
+		red[w] = s;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double t = 0.0;
+#pragma unroll
+		for (int ww = 0; ww < THREADS / 64; ++ww)
+			t += red[ww];
+		partials[tile] = t;
+	}
+}
+
+// LOOP: tiles beyond the grid (launches of more than 2^31 threads).
+template <typename T, bool LOOP>
+__global__ __launch_bounds__(THREADS) void k_pass(
+	const T *in, T *out, const double *__restrict__ nrm, PassDesc pd,
+	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 ntiles)
+{
+	typedef typename real_of<T>::type R;
+	__shared__ T lds[1 << PASS_TILE_BITS];
+	__shared__ double red[THREADS / 64];
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	if constexpr (LOOP) {
+		for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+			pass_tile<T>(in, out, inv, pd, ops, partials, t, lds, red);
+			__syncthreads();
+		}
+	} else {
+		pass_tile<T>(in, out, inv, pd, ops, partials, blockIdx.x, lds, red);
+	}
+}
+
+template <typename T>
+int launch_pass_typed(const dk_state *in, dk_state *out, const PassDesc &pd,
+		      const PassOp *ops_dev, double *parts, u64 ntiles)
+{
+	const unsigned gr = dk_grid_cap(ntiles, THREADS);
+	if (gr < ntiles)
+		hipLaunchKernelGGL((k_pass<T, true>), dim3(gr), dim3(THREADS), 0,
+				   DK_STREAM, (const T *)in->amp, (T *)out->amp,
+				   in->scal, pd, ops_dev, parts, ntiles);
+	else
+		hipLaunchKernelGGL((k_pass<T, false>), dim3(gr), dim3(THREADS), 0,
+				   DK_STREAM, (const T *)in->amp, (T *)out->amp,
+				   in->scal, pd, ops_dev, parts, ntiles);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+} // namespace
+
+// One pass over `in` into `out` (which may be `in`): the ops at ops_dev
+// (device memory, pd.nops of them), then out's norm from the tile partials.
+int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
+		   const PassOp *ops_dev)
+{
+	if (in->nq < (unsigned)PASS_TILE_BITS || out->nq != in->nq ||
+	    out->dtype != in->dtype) {
+		dk_set_error("a pass needs matching states of at least %d qubits",
+			     PASS_TILE_BITS);
+		return DK_E_ARG;
+	}
+	u64 seen = 0, sorted = 0;
+	for (int h = 0; h < HIGH_BITS; ++h) {
+		const unsigned q = pd.hpos[h];
+		if (q < (unsigned)LANE_BITS || q >= in->nq || ((seen >> q) & 1)) {
+			dk_set_error("bad pass tile (qubit %u)", q);
+			return DK_E_ARG;
+		}
+		seen |= 1ull << q;
+		if (pd.hsorted[h] >= in->nq ||
+		    (h > 0 && pd.hsorted[h] <= pd.hsorted[h - 1])) {
+			dk_set_error("bad pass tile order");
+			return DK_E_ARG;
+		}
+		sorted |= 1ull << pd.hsorted[h];
+	}
+	if (sorted != seen) {
+		dk_set_error("bad pass tile order");
+		return DK_E_ARG;
+	}
+	const u64 ntiles = in->len >> PASS_TILE_BITS;
+	double *parts = nullptr;
+	DK_TRY(dk_partials_reserve(ntiles, &parts));
+	hipStream_t st = DK_STREAM;
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(dk_cur()->events[62], st));
+	int rc = in->dtype == DK_C64
+			 ? launch_pass_typed<float2>(in, out, pd, ops_dev, parts, ntiles)
+			 : launch_pass_typed<double2>(in, out, pd, ops_dev, parts, ntiles);
+	if (rc)
+		return rc;
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(dk_cur()->events[63], st));
+	return dk_finish_partials(parts, ntiles, out->scal + DK_SCAL_NORM, 1);
+}

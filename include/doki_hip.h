@@ -243,6 +243,36 @@ int dk_circuit_num_gates(const dk_circuit *c, size_t *num_gates);
 int dk_circuit_block(const dk_circuit *c, size_t i, unsigned *num_qubits,
 		     unsigned *qubits, double *matrix);
 
+/* Plans cache-blocked passes instead of fused blocks: each pass streams the
+ * state through registers once and applies every one-qubit gate whose target
+ * lies in the pass's tile (qubits 0..5 plus six chosen qubits), and every
+ * diagonal one-qubit gate wherever its target is; controls may be anywhere.
+ * Gates are moved ahead of gates left for a later pass only where they
+ * commute.  Gates on several qubits run alone between passes.  States of
+ * fewer than 12 qubits run gate by gate.  *num_passes (may be NULL)
+ * receives the pass count.  Results equal gate-by-gate application up to
+ * rounding (the state is renormalised once per pass). */
+int dk_circuit_passes(dk_circuit *c, size_t *num_passes);
+
+/* Pass i of a pass plan: the gates it absorbs, its op count (gates after
+ * merging same-target neighbours, plus register exchanges) and its 12 tile
+ * qubits (tile_qubits[0..5] = 0..5).  Any output may be NULL. */
+int dk_circuit_pass(const dk_circuit *c, size_t i, unsigned *num_gates,
+		    unsigned *num_ops, unsigned *tile_qubits);
+
+/* Step i of a pass plan, in execution order: *is_pass = 1 and *index = the
+ * pass, or *is_pass = 0 and *index = the gate (in order of dk_circuit_add)
+ * that runs alone.  DK_E_ARG past the last step. */
+int dk_circuit_plan_step(const dk_circuit *c, size_t i, int *is_pass,
+			 size_t *index);
+
+/* Copies the op records of pass i (96 bytes each, the PassOp layout of
+ * doki_amd/csrc/dk_pass.h) into ops (room for `capacity` records); for
+ * inspection and host-side checking of plans.  *count receives the number
+ * of records. */
+int dk_circuit_pass_ops(const dk_circuit *c, size_t i, void *ops,
+			size_t capacity, size_t *count);
+
 /* Runs the circuit on st: the first step writes a new state (*out) unless
  * in_place, later steps run in place.  Works on DK_C128 and DK_C64 states. */
 int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place, dk_state **out);

@@ -1,0 +1,107 @@
+"""Host-side check of cache-blocked pass plans (test infrastructure).
+
+Executes a pass plan from dk_circuit_passes with numpy, following the op
+semantics the pass kernel implements: a tile is the 2^12 amplitudes whose
+index bits 0..5 and the pass's six high qubits vary; register, wave and lane
+bits name tile slots under the mapping current at each op; exchanges change
+the mapping only.  This checks the planner (gate order, locations, control
+splitting, merged products) on the CPU; the kernel's register and LDS
+mechanics are checked on the GPU against the oracle.
+"""
+import numpy as np
+
+LANE_BITS, HIGH_BITS, REG_BITS, WAVE_BITS = 6, 6, 4, 2
+OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH = range(5)
+
+
+def _tile_index(n, tile_qubits):
+    """idx[t, s]: state index of slot s of tile t (slot bit b = tile_qubits[b])."""
+    others = [q for q in range(n) if q not in tile_qubits]
+    t = np.arange(1 << len(others), dtype=np.int64)
+    base = np.zeros_like(t)
+    for j, q in enumerate(others):
+        base |= ((t >> j) & 1) << q
+    s = np.arange(1 << len(tile_qubits), dtype=np.int64)
+    off = np.zeros_like(s)
+    for b, q in enumerate(tile_qubits):
+        off |= ((s >> b) & 1) << q
+    return base[:, None] | off[None, :]
+
+
+def run_pass(psi, n, tile_qubits, ops):
+    """One pass on the normalised state psi (complex128); returns the new
+    (unnormalised) state.  Asserts what the kernel relies on: exchanges are
+    permutations of the high slots, and no thread-word field (tcm, tam, a
+    diagonal op's qubit) names a qubit held in registers."""
+    idx = _tile_index(n, tile_qubits)
+    A = psi[idx].copy()
+    hpos = tile_qubits[LANE_BITS:]
+    rs = list(range(REG_BITS))
+
+    def reg_qubit(j):
+        return hpos[rs[j]]
+
+    def bit(q):
+        return ((idx >> q) & 1).astype(bool)
+
+    for op in ops:
+        kind = int(op["kind"])
+        if kind == OP_EXCH:
+            rs = [(int(op["a"]) >> (3 * j)) & 7 for j in range(REG_BITS)]
+            ws = [(int(op["b"]) >> (3 * b)) & 7 for b in range(WAVE_BITS)]
+            assert sorted(rs + ws) == list(range(HIGH_BITS))
+            continue
+        m = op["m"].view(np.complex128).reshape(2, 2)
+        held = sum(1 << reg_qubit(j) for j in range(REG_BITS))
+        tcm, tam, rmask = int(op["tcm"]), int(op["tam"]), int(op["rmask"])
+        assert not (tcm & held) and not (tam & held)
+        ok = np.ones(A.shape, dtype=bool)
+        for q in range(n):
+            if (tcm >> q) & 1:
+                ok &= bit(q)
+            if (tam >> q) & 1:
+                ok &= ~bit(q)
+        for j in range(REG_BITS):
+            if (rmask >> j) & 1:
+                ok &= bit(reg_qubit(j))
+            if (rmask >> (8 + j)) & 1:
+                ok &= ~bit(reg_qubit(j))
+        a = int(op["a"])
+        if kind in (OP_REG, OP_LANE):
+            if kind == OP_REG:
+                q, sb = reg_qubit(a), LANE_BITS + rs[a]
+            else:
+                assert a < LANE_BITS
+                q, sb = a, a
+            slots = np.arange(A.shape[1])
+            partner = slots ^ (1 << sb)
+            upper = bit(q)
+            x = np.where(upper, A[:, partner], A)   # lower amplitude
+            y = np.where(upper, A, A[:, partner])   # upper amplitude
+            o = np.where(upper, m[1, 0] * x + m[1, 1] * y, m[0, 0] * x + m[0, 1] * y)
+            A = np.where(ok, o, A)
+        else:
+            if kind == OP_DIAG_REG:
+                q = reg_qubit(a)
+            else:
+                assert kind == OP_DIAG_THREAD and not ((held >> a) & 1)
+                q = a
+            f = np.where(bit(q), m[1, 1], m[0, 0])
+            A = np.where(ok, A * f, A)
+    out = np.empty_like(psi)
+    out[idx] = A
+    return out
+
+
+def run_plan(circ, steps, n, psi, apply_gate):
+    """Runs circ's pass plan (circ.passes() already called) on psi:
+    passes through run_pass, lone gates through apply_gate(psi, step).
+    The state is renormalised after every step, as the kernels do."""
+    for kind, i in circ.plan():
+        if kind == "pass":
+            _, _, tq = circ.pass_info(i)
+            psi = run_pass(psi, n, tq, circ.pass_ops(i))
+        else:
+            psi = apply_gate(psi, steps[i])
+        psi = psi / np.linalg.norm(psi)
+    return psi
