@@ -174,9 +174,9 @@ def big_leg(args):
 
 def fused_leg(args):
     """Fused dense gates on the matrix cores at the same register size:
-    per-k kernel rates (FP64 complex128 and FP32 complex64), the brickwork
-    circuit plain vs fused (circuits/s), and the complex64 error against the
-    FP64 result."""
+    per-k kernel rates (FP64 complex128 and FP32 complex64), and the
+    brickwork circuit plain (per gate) vs fused dense blocks vs cache-blocked
+    passes (circuits/s), each with its deviation from the plain FP64 result."""
     import numpy as np
 
     from doki_amd import capi, workloads as W
@@ -208,9 +208,10 @@ def fused_leg(args):
     res = {}
     for label, k, dtype in (("plain_f64", 0, capi.C128), ("fused5_f64", 5, capi.C128),
                             ("fused4_f64", 4, capi.C128), ("fused6_f64", 6, capi.C128),
-                            ("fused5_f32", 5, capi.C64)):
+                            ("fused5_f32", 5, capi.C64), ("passes_f64", "passes", capi.C128),
+                            ("passes_f32", "passes", capi.C64)):
         circ = capi.Circuit(n, steps)
-        nb = circ.fuse(k)
+        nb = circ.passes() if k == "passes" else circ.fuse(k)
         st = capi.new_typed(n, dtype)
         circ.run(st, in_place=True)      # warm (gate uploads)
         capi.synchronize()
@@ -222,18 +223,24 @@ def fused_leg(args):
         el = time.perf_counter() - t0
         res[label] = {"passes": nb or len(steps), "seconds": round(el, 4),
                       "circuits_per_s": round(1.0 / el, 3)}
-        if label == "fused5_f64":
+        if label == "plain_f64":
             ref_state = st
-        elif label == "fused5_f32":
+        else:
+            # against the plain FP64 per-gate result, first 4M amplitudes
             a = ref_state.raw(0, 1 << 22)[0] / ref_state.norm
             b = st.raw(0, 1 << 22)[0] / st.norm
-            res[label]["max_abs_err_vs_f64_first_4M"] = float(np.max(np.abs(a - b)))
-        del st
+            res[label]["max_abs_err_vs_plain_f64_first_4M"] = float(np.max(np.abs(a - b)))
+        if label != "plain_f64":
+            del st
         capi.synchronize()
         log(f"[bench] {label}: {res[label]}")
     out["brickwork_circuit"] = {"qubits": n, "depth": depth, "gates": len(steps), **res}
+    del ref_state
+    capi.synchronize()
     out["fused_speedup_vs_plain"] = round(
         res["plain_f64"]["seconds"] / res["fused5_f64"]["seconds"], 2)
+    out["passes_speedup_vs_plain"] = round(
+        res["plain_f64"]["seconds"] / res["passes_f64"]["seconds"], 2)
     return out
 
 

@@ -10,9 +10,10 @@ are admitted by the device memory their states need, against a byte budget:
   launch join the next one (continuous batching);
 * larger registers and joins go to worker threads, each bound to its own
   execution lane (a HIP stream with its own scratch), so that several
-  requests run on the device at once.  Their circuits run fused into dense
-  blocks of up to ``fuse_k`` qubits, in place, and their states stay resident
-  on the device from creation to the final operation.
+  requests run on the device at once.  Their circuits run in place as
+  cache-blocked passes (or fused dense blocks, as the tuned plan says), and
+  their states stay resident on the device from creation to the final
+  operation.
 
 ctypes calls release the GIL, so host threads overlap while the device works.
 Every request's result depends only on the request: the batched engine runs
@@ -92,17 +93,18 @@ def _finish(h, result=None, error=None):
     h._ev.set()
 
 
-def build_register(nq, packed, dtype, fuse_k):
-    """|0...0> of nq qubits with the packed circuit applied in place, fused
-    into dense blocks when the register is wide enough for them; fuse_k
-    "auto" takes the tuned width for (nq, dtype)."""
-    if fuse_k == "auto":
-        fuse_k = tune.best_fuse_k(nq, dtype)
+def build_register(nq, packed, dtype, plan="auto"):
+    """|0...0> of nq qubits with the packed circuit applied in place under
+    `plan`: "passes" (cache-blocked passes), a fusion width k (dense blocks
+    of up to k qubits, when the register is wide enough for them), 0 (gate
+    by gate), or "auto" for the tuned plan of (nq, dtype)."""
+    if plan == "auto":
+        plan = tune.best_plan(nq, dtype)
     st = capi.new_typed(nq, dtype)
     circ = capi.Circuit(nq)
     circ.add_packed(packed)
-    if fuse_k and nq >= fuse_k + 6:
-        circ.fuse(fuse_k)
+    if plan == "passes" or (plan and nq >= plan + 6):
+        tune.prepare(circ, plan)
     circ.run(st, in_place=True)
     return st
 
@@ -128,11 +130,11 @@ def run_operation(spec, states, dtype):
 class Server:
     """Continuous-batching request server on the library's device."""
 
-    def __init__(self, workers=3, small_qubits=SMALL_QUBITS, fuse_k="auto",
+    def __init__(self, workers=3, small_qubits=SMALL_QUBITS, plan="auto",
                  budget=None, max_batch=2048, dtype=capi.C128):
         self.dtype = dtype
         self.small_qubits = small_qubits
-        self.fuse_k = fuse_k
+        self.plan = plan
         self.max_batch = max_batch
         if budget is None:
             b = capi.C.c_uint64()
@@ -268,7 +270,7 @@ class Server:
             _finish(h, res, err)
 
     def _run_large(self, h):
-        states = [build_register(n, pk, self.dtype, self.fuse_k)
+        states = [build_register(n, pk, self.dtype, self.plan)
                   for (n, _), pk in zip(h.spec["regs"], h.packed)]
         res = run_operation(h.spec, states, self.dtype)
         del states  # freed on this thread's lane

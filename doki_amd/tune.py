@@ -1,5 +1,6 @@
-"""Autotuning of the fusion width, kernel block shapes and batched-engine
-pass width, measured on the device and cached per (qubits, amplitude type).
+"""Autotuning of the circuit plan (cache-blocked passes or a fusion width),
+kernel block shapes and batched-engine pass width, measured on the device
+and cached per (qubits, amplitude type).
 
 The cache is a JSON file (``DOKI_TUNE_CACHE``, default
 ``~/.cache/doki_amd/tune-<arch>.json``) mapping ``"<n>:<c128|c64>"`` to the
@@ -16,12 +17,30 @@ from . import workloads as W
 
 DTYPE_NAMES = {capi.C128: "c128", capi.C64: "c64"}
 FUSE_WIDTHS = (0, 3, 4, 5, 6)
+# circuit plans: cache-blocked passes, or dense fusion of width k (0 = none)
+PLANS = ("passes",) + FUSE_WIDTHS
+PASS_MIN_QUBITS = 12
 
 
 def default_fuse_k(n):
     """Fusion width without a tuning entry: 5-qubit blocks once the register
     is wide enough for the dense kernels, none below."""
     return 5 if n >= 11 else 0
+
+
+def default_plan(n):
+    """Plan without a tuning entry: cache-blocked passes once the register
+    holds a whole pass tile, gate by gate below."""
+    return "passes" if n >= PASS_MIN_QUBITS else 0
+
+
+def prepare(circ, plan):
+    """Plans circ (a capi.Circuit) with `plan`: "passes" or a fusion width."""
+    if plan == "passes":
+        return circ.passes()
+    if plan:
+        return circ.fuse(int(plan))
+    return 0
 
 
 def cache_path():
@@ -70,10 +89,18 @@ def best_fuse_k(n, dtype=capi.C128, path=None):
     return default_fuse_k(n)
 
 
-def _time_circuit(n, dtype, steps, k, reps):
+def best_plan(n, dtype=capi.C128, path=None):
+    """The tuned circuit plan for (n, dtype): "passes" or a fusion width."""
+    e = lookup(n, dtype, path)
+    if e and "plan" in e:
+        p = e["plan"]
+        return p if p == "passes" else int(p)
+    return default_plan(n)
+
+
+def _time_circuit(n, dtype, steps, plan, reps):
     circ = capi.Circuit(n, steps)
-    if k:
-        circ.fuse(k)
+    prepare(circ, plan)
     st = capi.new_typed(n, dtype)
     circ.run(st, in_place=True)            # gate uploads, allocation
     capi.synchronize()
@@ -87,14 +114,17 @@ def _time_circuit(n, dtype, steps, k, reps):
 
 
 def tune_fusion(n, dtype=capi.C128, depth=None, seed=7, reps=3):
-    """Times a mixed random circuit (depth 4n by default) unfused and fused
-    into blocks of 3..6 qubits; returns {width: seconds}."""
+    """Times a mixed random circuit (depth 4n by default) gate by gate, fused
+    into blocks of 3..6 qubits and as cache-blocked passes; returns
+    {plan: seconds}."""
     steps, _ = W.random_circuit(seed, n, depth or 4 * n)
     out = {}
-    for k in FUSE_WIDTHS:
-        if k and n < k:
+    for p in PLANS:
+        if p == "passes" and n < PASS_MIN_QUBITS:
             continue
-        out[k] = _time_circuit(n, dtype, steps, k, reps)
+        if p != "passes" and p and n < p:
+            continue
+        out[p] = _time_circuit(n, dtype, steps, p, reps)
     return out
 
 
@@ -158,8 +188,10 @@ def autotune(n, dtype=capi.C128, path=None, batch_specs=None):
     """Measures and caches the settings for registers of n qubits."""
     fus = tune_fusion(n, dtype)
     ker = tune_kernels(n, dtype)
+    widths = {k: v for k, v in fus.items() if k != "passes"}
     entry = {
-        "fuse_k": min(fus, key=fus.get),
+        "plan": min(fus, key=fus.get),
+        "fuse_k": min(widths, key=widths.get),
         "fusion_seconds": {str(k): v for k, v in fus.items()},
         "apply1_group": min(ker["apply1_group"], key=ker["apply1_group"].get),
         "apply1_group_ms": {str(k): v for k, v in ker["apply1_group"].items()},

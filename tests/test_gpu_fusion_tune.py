@@ -16,6 +16,7 @@ def test_autotune_and_use(capi, oracle, tmp_path):
     specs = [(10, *W.random_circuit(60 + i, 10, 30)) for i in range(8)]
     entry = tune.autotune(n, capi.C128, path=path, batch_specs=specs)
     assert entry["fuse_k"] in tune.FUSE_WIDTHS
+    assert entry["plan"] in tune.PLANS
     assert entry["apply1_group"] in (1, 2, 4)
     assert entry["dense_k4_cw"] in (1, 2, 4)
     assert entry["pass_qubits"] in (0, 10, 8, 6)
@@ -23,13 +24,13 @@ def test_autotune_and_use(capi, oracle, tmp_path):
     try:
         tune.apply_kernel_settings(n, capi.C128, path)
         steps, _ = W.random_circuit(99, n, 60)
-        c = capi.Circuit(n, steps)
-        k = tune.best_fuse_k(n, capi.C128, path)
-        if k:
-            c.fuse(k)
-        st = c.run(capi.new(n))
-        want = oracle_run(oracle, oracle.new(n), steps)
-        assert np.max(np.abs(st.values() - want.values())) <= ATOL
+        for plan in (tune.best_fuse_k(n, capi.C128, path),
+                     tune.best_plan(n, capi.C128, path)):
+            c = capi.Circuit(n, steps)
+            tune.prepare(c, plan)
+            st = c.run(capi.new(n))
+            want = oracle_run(oracle, oracle.new(n), steps)
+            assert np.max(np.abs(st.values() - want.values())) <= ATOL
     finally:
         capi.tune("apply1_group", 1)
         capi.tune("dense_k4_cw", 0)
