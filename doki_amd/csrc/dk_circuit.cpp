@@ -533,7 +533,7 @@ bool pass_ops_valid(const PassOp *ops, size_t n, unsigned nq)
 			return false;
 		}
 		const u64 valid = nq >= 64 ? ~0ull : (1ull << nq) - 1;
-		if ((o.rmask & ~0x0f0fu) || (o.tcm & ~valid) || (o.tam & ~valid))
+		if ((o.rmask & ~0x0f0fu) || (o.tcm & ~valid) || (o.tam & ~valid) || o.b)
 			return false;
 	}
 	return true;
@@ -575,20 +575,30 @@ void build_pass(dk_circuit *c, std::vector<unsigned> chosen,
 	std::vector<int> pending(gs.begin(), gs.end());
 	while (!pending.empty()) {
 		// the first gate that can run now and commutes with every pending
-		// gate before it
+		// gate before it; a gate that merges into the previous op (same
+		// target and controls) goes first
 		u64 bN = 0, bD = 0;
-		size_t pick = pending.size();
+		size_t pick = pending.size(), merge = pending.size();
 		for (size_t i = 0; i < pending.size(); ++i) {
 			const CGate &g = c->gates[pending[i]];
 			u64 N, D;
 			gate_sets(g, N, D);
 			if (!(N & (bN | bD)) && !(D & bN) && B.executable(g)) {
-				pick = i;
-				break;
+				if (pick == pending.size())
+					pick = i;
+				if (B.last >= 0 && g.t[0] == B.last_q && g.cm == B.last_cm &&
+				    g.am == B.last_am) {
+					merge = i;
+					break;
+				}
+				if (B.last < 0)
+					break;
 			}
 			bN |= N;
 			bD |= D;
 		}
+		if (merge < pending.size())
+			pick = merge;
 		if (pick < pending.size()) {
 			B.emit_gate(c->gates[pending[pick]]);
 			pending.erase(pending.begin() + (long)pick);
@@ -618,9 +628,167 @@ void build_pass(dk_circuit *c, std::vector<unsigned> chosen,
 			}
 		B.emit_exchange(nr);
 	}
+	// a phase (diagonal with m00 = 1) becomes the factor m11 controlled on
+	// its own target: the kernel then skips the amplitudes it leaves alone
+	// (whole waves or registers where the target is held per wave or per
+	// register)
+	for (size_t i = pp.first_op; i < c->pops.size(); ++i) {
+		PassOp &o = c->pops[i];
+		if ((o.kind != dkp::OP_DIAG_REG && o.kind != dkp::OP_DIAG_THREAD) ||
+		    o.m[0] != 1.0 || o.m[1] != 0.0)
+			continue;
+		o.m[0] = o.m[6];
+		o.m[1] = o.m[7];
+		if (o.kind == dkp::OP_DIAG_REG)
+			o.rmask |= 1u << o.a;
+		else
+			o.tcm |= 1ull << o.a;
+	}
 	pp.desc.nops = (uint32_t)(c->pops.size() - pp.first_op);
 	c->passes.push_back(pp);
 	c->plan.push_back({true, (int)c->passes.size() - 1});
+}
+
+} // namespace
+
+namespace {
+
+// A planned step before op building: a pass (tile qubits, absorbed gates)
+// or a gate run alone.
+struct PlanSpec {
+	bool pass;
+	int gate;                      // lone gate
+	std::vector<unsigned> chosen;  // pass: high tile qubits, first-used first
+	std::vector<int> gates;        // pass: absorbed gates, circuit order
+};
+
+// Splits `rem` into the gates a pass over high tile qubits `hmask` absorbs
+// and the rest (deferred, in order).  A gate is absorbed when it is a
+// one-qubit gate on a tile qubit or diagonal, and commutes with every
+// deferred gate before it.
+void absorb(const dk_circuit *c, const std::vector<int> &rem, u64 hmask,
+	    std::vector<int> *absorbed, std::vector<int> *deferred)
+{
+	u64 bN = 0, bD = 0;
+	for (int gi : rem) {
+		const CGate &g = c->gates[gi];
+		u64 N, D;
+		gate_sets(g, N, D);
+		const bool take = g.k == 1 && !(N & (bN | bD)) && !(D & bN) &&
+				  (is_diag(g) || g.t[0] < (unsigned)dkp::LANE_BITS ||
+				   ((hmask >> g.t[0]) & 1));
+		if (take) {
+			if (absorbed)
+				absorbed->push_back(gi);
+		} else {
+			if (deferred)
+				deferred->push_back(gi);
+			bN |= N;
+			bD |= D;
+		}
+	}
+}
+
+// High tile qubits of the next pass.  First-seen: the targets of the
+// absorbable gates in circuit order, until six are chosen.  Look-ahead:
+// six times, the qubit whose addition absorbs the most gates.
+u64 choose_tile(const dk_circuit *c, const std::vector<int> &rem, bool lookahead)
+{
+	u64 hmask = 0;
+	int nh = 0;
+	if (!lookahead) {
+		u64 bN = 0, bD = 0;
+		for (int gi : rem) {
+			const CGate &g = c->gates[gi];
+			u64 N, D;
+			gate_sets(g, N, D);
+			bool take = false;
+			if (g.k == 1 && !(N & (bN | bD)) && !(D & bN)) {
+				const unsigned t = g.t[0];
+				if (is_diag(g) || t < (unsigned)dkp::LANE_BITS || ((hmask >> t) & 1)) {
+					take = true;
+				} else if (nh < dkp::HIGH_BITS) {
+					hmask |= 1ull << t;
+					++nh;
+					take = true;
+				}
+			}
+			if (!take) {
+				bN |= N;
+				bD |= D;
+			}
+		}
+		return hmask;
+	}
+	u64 cand = 0;
+	for (int gi : rem) {
+		const CGate &g = c->gates[gi];
+		if (g.k == 1 && !is_diag(g) && g.t[0] >= (unsigned)dkp::LANE_BITS)
+			cand |= 1ull << g.t[0];
+	}
+	std::vector<int> ab;
+	for (; nh < dkp::HIGH_BITS && cand; ++nh) {
+		size_t best = 0;
+		unsigned bq = 64;
+		for (u64 m = cand; m; m &= m - 1) {
+			const unsigned q = (unsigned)__builtin_ctzll(m);
+			ab.clear();
+			absorb(c, rem, hmask | (1ull << q), &ab, nullptr);
+			if (bq == 64 || ab.size() > best) {
+				best = ab.size();
+				bq = q;
+			}
+		}
+		hmask |= 1ull << bq;
+		cand &= ~(1ull << bq);
+	}
+	return hmask;
+}
+
+// Plans the whole circuit with one tile rule.
+std::vector<PlanSpec> plan_specs(const dk_circuit *c, bool lookahead)
+{
+	std::vector<PlanSpec> out;
+	std::vector<int> rem(c->gates.size());
+	for (size_t i = 0; i < rem.size(); ++i)
+		rem[i] = (int)i;
+	if (c->nq < (unsigned)dkp::PASS_TILE_BITS) {
+		// too small for a tile: every gate runs alone
+		for (int gi : rem)
+			out.push_back({false, gi, {}, {}});
+		return out;
+	}
+	while (!rem.empty()) {
+		if (c->gates[rem[0]].k != 1) {
+			out.push_back({false, rem[0], {}, {}});
+			rem.erase(rem.begin());
+			continue;
+		}
+		const u64 hmask = choose_tile(c, rem, lookahead);
+		std::vector<int> absorbed, deferred;
+		absorb(c, rem, hmask, &absorbed, &deferred);
+		rem.swap(deferred);
+		if (absorbed.size() == 1 && !is_diag(c->gates[absorbed[0]])) {
+			// one dense gate: the one-qubit kernel does it alone
+			out.push_back({false, absorbed[0], {}, {}});
+			continue;
+		}
+		// high tile qubits in order of first use as a dense target
+		PlanSpec ps{true, -1, {}, std::move(absorbed)};
+		u64 seen = 0;
+		for (int gi : ps.gates) {
+			const CGate &g = c->gates[gi];
+			const unsigned t = g.t[0];
+			if (!is_diag(g) && ((hmask >> t) & 1) && !((seen >> t) & 1)) {
+				seen |= 1ull << t;
+				ps.chosen.push_back(t);
+			}
+		}
+		for (u64 m = hmask & ~seen; m; m &= m - 1)
+			ps.chosen.push_back((unsigned)__builtin_ctzll(m));
+		out.push_back(std::move(ps));
+	}
+	return out;
 }
 
 } // namespace
@@ -638,56 +806,16 @@ extern "C" int dk_circuit_passes(dk_circuit *c, size_t *num_passes)
 	c->passes.clear();
 	c->pops.clear();
 	c->passes_planned = true;
-	std::vector<int> rem(c->gates.size());
-	for (size_t i = 0; i < rem.size(); ++i)
-		rem[i] = (int)i;
-	if (c->nq < (unsigned)dkp::PASS_TILE_BITS) {
-		// too small for a tile: every gate runs alone
-		for (int gi : rem)
-			c->plan.push_back({false, gi});
-		rem.clear();
-	}
-	while (!rem.empty()) {
-		const CGate &g0 = c->gates[rem[0]];
-		if (g0.k != 1) {
-			c->plan.push_back({false, rem[0]});
-			rem.erase(rem.begin());
-			continue;
-		}
-		std::vector<unsigned> chosen;
-		u64 cmask = 0;
-		std::vector<int> absorbed, deferred;
-		u64 bN = 0, bD = 0;
-		for (int gi : rem) {
-			const CGate &g = c->gates[gi];
-			u64 N, D;
-			gate_sets(g, N, D);
-			bool take = false;
-			if (g.k == 1 && !(N & (bN | bD)) && !(D & bN)) {
-				const unsigned t = g.t[0];
-				if (is_diag(g) || t < (unsigned)dkp::LANE_BITS || ((cmask >> t) & 1)) {
-					take = true;
-				} else if (chosen.size() < (size_t)dkp::HIGH_BITS) {
-					chosen.push_back(t);
-					cmask |= 1ull << t;
-					take = true;
-				}
-			}
-			if (take) {
-				absorbed.push_back(gi);
-			} else {
-				deferred.push_back(gi);
-				bN |= N;
-				bD |= D;
-			}
-		}
-		rem.swap(deferred);
-		if (absorbed.size() == 1 && !is_diag(c->gates[absorbed[0]])) {
-			// one dense gate: the one-qubit kernel does it alone
-			c->plan.push_back({false, absorbed[0]});
-			continue;
-		}
-		build_pass(c, chosen, absorbed);
+	// both tile rules; the plan with fewer state passes (passes plus lone
+	// gates) wins, first-seen on ties
+	std::vector<PlanSpec> a = plan_specs(c, false);
+	std::vector<PlanSpec> b = plan_specs(c, true);
+	const std::vector<PlanSpec> &best = b.size() < a.size() ? b : a;
+	for (const PlanSpec &ps : best) {
+		if (ps.pass)
+			build_pass(c, ps.chosen, ps.gates);
+		else
+			c->plan.push_back({false, ps.gate});
 	}
 	for (const PassPlan &pp : c->passes)
 		if (!pass_ops_valid(&c->pops[pp.first_op], pp.desc.nops, c->nq)) {

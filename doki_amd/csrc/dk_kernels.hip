@@ -682,6 +682,90 @@ __global__ __launch_bounds__(TPB) void k_density_sum(
 	}
 }
 
+// Reduced density of at most 6 kept qubits as a Gram matrix: with A the
+// 2^a x 2^m matrix A[i][t] = psi(kept bits = i, traced bits = t),
+// rho = A A^H.  Split over t: a block stages chunks of CH amplitudes (COLS
+// = CH / DIM columns of A, consecutive t, so loads run along the state's
+// low traced bits) in LDS, and each thread accumulates an RB x RB block of
+// rho over every S-th column of the chunk, S = threads per rho block.  The
+// block then adds its S replicas and leaves one DIM x DIM partial.
+constexpr int DENS_CH = 2048;
+
+template <typename T, int DIM, int RB>
+__global__ __launch_bounds__(TPB) void k_density_gram(
+	const T *__restrict__ in, const double *__restrict__ nrm, DensArgs da,
+	u64 nchunks, int cols, double2 *__restrict__ part)
+{
+	constexpr int NBR = DIM / RB;          // rho blocks per row
+	constexpr int NB = NBR * NBR;          // rho blocks
+	constexpr int S = TPB / NB;            // threads per rho block
+	static_assert(NB <= TPB && TPB % NB == 0, "block shape");
+	__shared__ double2 lds[DENS_CH];
+	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const int blk = threadIdx.x % NB, sub = threadIdx.x / NB;
+	const int i0 = (blk / NBR) * RB, j0 = (blk % NBR) * RB;
+	const int ch = cols * DIM;             // amplitudes per chunk
+	double2 acc[RB][RB];
+#pragma unroll
+	for (int a = 0; a < RB; ++a)
+#pragma unroll
+		for (int b = 0; b < RB; ++b)
+			acc[a][b] = make_double2(0.0, 0.0);
+	for (u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const u64 t0 = c * (u64)cols;
+		for (int e = threadIdx.x; e < ch; e += TPB) {
+			const int i = e / cols, t = e - i * cols;
+			const T v = in[keep_index((u64)i, da) | traced_index(t0 + t, da)];
+			lds[e] = make_double2(v.x * inv, v.y * inv);
+		}
+		__syncthreads();
+		for (int t = sub; t < cols; t += S) {
+			double2 x[RB], y[RB];
+#pragma unroll
+			for (int a = 0; a < RB; ++a) {
+				x[a] = lds[(i0 + a) * cols + t];
+				y[a] = lds[(j0 + a) * cols + t];
+			}
+#pragma unroll
+			for (int a = 0; a < RB; ++a)
+#pragma unroll
+				for (int b = 0; b < RB; ++b) {
+					// x * conj(y)
+					acc[a][b].x = fma(x[a].x, y[b].x, acc[a][b].x);
+					acc[a][b].x = fma(x[a].y, y[b].y, acc[a][b].x);
+					acc[a][b].y = fma(x[a].y, y[b].x, acc[a][b].y);
+					acc[a][b].y = fma(-x[a].x, y[b].y, acc[a][b].y);
+				}
+		}
+		__syncthreads();
+	}
+	// add the S replicas of each rho block (in replica order) through LDS,
+	// one row a of the RB x RB blocks at a time (RB * TPB <= DENS_CH
+	// entries); replica r of block bb is thread r * NB + bb
+	static_assert(RB * TPB <= DENS_CH, "reduction buffer");
+	double2 *red = lds;
+#pragma unroll
+	for (int a = 0; a < RB; ++a) {
+#pragma unroll
+		for (int b = 0; b < RB; ++b)
+			red[b * TPB + threadIdx.x] = acc[a][b];
+		__syncthreads();
+		// rho rows i = ib * RB + a, all columns j
+		for (int e = threadIdx.x; e < NBR * DIM; e += TPB) {
+			const int ib = e / DIM, j = e % DIM;
+			const int i = ib * RB + a, bb = ib * NBR + j / RB, b = j % RB;
+			double2 sum = make_double2(0.0, 0.0);
+			for (int r = 0; r < S; ++r) {
+				const double2 v = red[b * TPB + r * NB + bb];
+				sum.x += v.x;
+				sum.y += v.y;
+			}
+			part[(size_t)blockIdx.x * DIM * DIM + (size_t)i * DIM + j] = sum;
+		}
+		__syncthreads();
+	}
+}
+
 // One block per requested element.
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_density_elements(
@@ -1137,10 +1221,50 @@ static void dens_args(const dk_state *s, u64 keep_mask, DensArgs &da)
 	}
 }
 
+// Gram-matrix form for a <= 6 kept qubits (see k_density_gram).
+static int launch_density_gram(const dk_state *s, const DensArgs &da,
+			       double2 *rho_dev)
+{
+	const int dim = 1 << da.a;
+	const u64 ch = std::min<u64>(DENS_CH, s->len);
+	const int cols = (int)(ch / (u64)dim);
+	const u64 nchunks = s->len / ch;
+	const u64 nblk = std::min<u64>(nchunks, (u64)std::max(1, g_ctx.num_cus) * 4);
+	double2 *part = nullptr;
+	const u64 part_bytes = nblk * (u64)dim * dim * sizeof(double2);
+	DK_TRY(dk_device_alloc(part_bytes, (void **)&part));
+#define G(T_, D_, R_)                                                          \
+	hipLaunchKernelGGL((k_density_gram<T_, D_, R_>), dim3((unsigned)nblk),      \
+			   dim3(TPB), 0, DK_STREAM, (const T_ *)s->amp, s->scal, da, \
+			   nchunks, cols, part)
+#define L(T_)                                                                  \
+	switch (dim) {                                                         \
+	case 1: G(T_, 1, 1); break;                                            \
+	case 2: G(T_, 2, 1); break;                                            \
+	case 4: G(T_, 4, 1); break;                                            \
+	case 8: G(T_, 8, 2); break;                                            \
+	case 16: G(T_, 16, 4); break;                                          \
+	case 32: G(T_, 32, 4); break;                                          \
+	default: G(T_, 64, 4); break;                                          \
+	}
+	DK_BY_TYPE(s, L);
+#undef L
+#undef G
+	DK_LAUNCHED();
+	const unsigned nb = grid_for((u64)dim * dim, TPB);
+	hipLaunchKernelGGL(k_density_sum, dim3(nb), dim3(TPB), 0, DK_STREAM,
+			   part, (int)nblk, (u64)dim * dim, rho_dev);
+	DK_LAUNCHED();
+	dk_device_free(part, part_bytes);
+	return DK_OK;
+}
+
 int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
 {
 	DensArgs da;
 	dens_args(s, keep_mask, da);
+	if (da.a <= 6)
+		return launch_density_gram(s, da, rho_dev);
 	const u64 dim = 1ull << da.a;
 	const u64 T = 1ull << da.m;
 	const int tpr = (int)((dim + 15) / 16);

@@ -67,10 +67,17 @@ __device__ __forceinline__ bool reg_ok(int r, unsigned rmask)
 	return ((unsigned)r & rcm) == rcm && ((unsigned)r & ram) == 0;
 }
 
-// Dense 2x2 gate on register bit J.  SEL: the op has controls, and register
-// r of a thread takes the new value only where ok and its register bits
-// pass; otherwise every amplitude takes it.
-template <int J, bool SEL, typename R>
+// Control modes of a gate op:
+//   CM_NONE: no controls, every amplitude takes the new value;
+//   CM_REG: the thread-word part of the condition holds for the whole wave
+//     (checked by the caller) and registers whose register-bit part fails
+//     are skipped (a uniform branch);
+//   CM_LANE: the condition varies across lanes: selects per lane, and
+//     registers whose register-bit part fails are skipped.
+enum { CM_NONE = 0, CM_REG = 1, CM_LANE = 2 };
+
+// Dense 2x2 gate on register bit J.
+template <int J, int CM, typename R>
 __device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
 					 const M22<R> &M, bool ok, unsigned rmask)
 {
@@ -78,16 +85,17 @@ __device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
 	for (int r = 0; r < REGS; ++r) {
 		if (r & (1 << J))
 			continue;
+		if (CM != CM_NONE && !reg_ok(r, rmask))
+			continue;
 		constexpr int d = 1 << J;
 		R o0r, o0i, o1r, o1i;
 		row2(M.m00, vr[r], vi[r], M.m01, vr[r | d], vi[r | d], o0r, o0i);
 		row2(M.m10, vr[r], vi[r], M.m11, vr[r | d], vi[r | d], o1r, o1i);
-		if (SEL) {
-			const bool c = ok & reg_ok(r, rmask);
-			o0r = c ? o0r : vr[r];
-			o0i = c ? o0i : vi[r];
-			o1r = c ? o1r : vr[r | d];
-			o1i = c ? o1i : vi[r | d];
+		if (CM == CM_LANE) {
+			o0r = ok ? o0r : vr[r];
+			o0i = ok ? o0i : vi[r];
+			o1r = ok ? o1r : vr[r | d];
+			o1i = ok ? o1i : vi[r | d];
 		}
 		vr[r] = o0r;
 		vi[r] = o0i;
@@ -96,7 +104,7 @@ __device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
 	}
 }
 
-template <bool SEL, typename R>
+template <int CM, typename R>
 __device__ __forceinline__ void gate_reg_any(R (&vr)[REGS], R (&vi)[REGS],
 					     const M22<R> &M, bool ok, unsigned rmask,
 					     unsigned j)
@@ -106,52 +114,118 @@ __device__ __forceinline__ void gate_reg_any(R (&vr)[REGS], R (&vi)[REGS],
 	// scratch memory)
 	switch (j) {
 	case 0:
-		gate_reg<0, SEL>(vr, vi, M, ok, rmask);
+		gate_reg<0, CM>(vr, vi, M, ok, rmask);
 		asm volatile("; register bit 0");
 		break;
 	case 1:
-		gate_reg<1, SEL>(vr, vi, M, ok, rmask);
+		gate_reg<1, CM>(vr, vi, M, ok, rmask);
 		asm volatile("; register bit 1");
 		break;
 	case 2:
-		gate_reg<2, SEL>(vr, vi, M, ok, rmask);
+		gate_reg<2, CM>(vr, vi, M, ok, rmask);
 		asm volatile("; register bit 2");
 		break;
 	default:
-		gate_reg<3, SEL>(vr, vi, M, ok, rmask);
+		gate_reg<3, CM>(vr, vi, M, ok, rmask);
 		asm volatile("; register bit 3");
 		break;
 	}
 }
 
-// Dense 2x2 gate on lane bit b: the partner amplitude is in lane ^ 2^b.
-template <bool SEL, typename R>
+// The value of lane ^ 2^B.  B < 2: one DPP quad permutation, no LDS
+// traffic.  B >= 2: ds_bpermute, which measured faster on gfx950 than DPP
+// mirror pairs (two dependent moves with hazard waits) and than the
+// permlane swaps (copies, a select and hazard waits per dword).
+template <int B>
+__device__ __forceinline__ int lane_xor_i(int v, unsigned lane)
+{
+	if constexpr (B == 0)
+		return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); // [1,0,3,2]
+	else if constexpr (B == 1)
+		return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); // [2,3,0,1]
+	else
+		return __builtin_amdgcn_ds_bpermute((int)((lane ^ (1u << B)) << 2), v);
+}
+
+template <int B>
+__device__ __forceinline__ double lane_xor(double v, unsigned lane)
+{
+	const int lo = lane_xor_i<B>(__double2loint(v), lane);
+	const int hi = lane_xor_i<B>(__double2hiint(v), lane);
+	return __hiloint2double(hi, lo);
+}
+
+template <int B>
+__device__ __forceinline__ float lane_xor(float v, unsigned lane)
+{
+	return __int_as_float(lane_xor_i<B>(__float_as_int(v), lane));
+}
+
+// Dense 2x2 gate on lane bit B: the partner amplitude is in lane ^ 2^B.
+// Every lane takes part in the exchange; CM_LANE selects afterwards.
+template <int B, int CM, typename R>
 __device__ __forceinline__ void gate_lane(R (&vr)[REGS], R (&vi)[REGS],
 					  const M22<R> &M, bool ok, unsigned rmask,
-					  unsigned b, unsigned lane)
+					  unsigned lane)
 {
-	const bool upper = (lane >> b) & 1;
+	const bool upper = (lane >> B) & 1;
 	const Cx<R> a0 = cx_sel(upper, M.m11, M.m00); // own amplitude
 	const Cx<R> a1 = cx_sel(upper, M.m10, M.m01); // partner's
-	const int d = 1 << b;
 #pragma unroll
 	for (int r = 0; r < REGS; ++r) {
-		const R pr = __shfl_xor(vr[r], d), pi = __shfl_xor(vi[r], d);
+		if (CM != CM_NONE && !reg_ok(r, rmask))
+			continue;
+		const R pr = lane_xor<B>(vr[r], lane), pi = lane_xor<B>(vi[r], lane);
 		R orr, oi;
 		row2(a0, vr[r], vi[r], a1, pr, pi, orr, oi);
-		if (SEL) {
-			const bool c = ok & reg_ok(r, rmask);
-			orr = c ? orr : vr[r];
-			oi = c ? oi : vi[r];
+		if (CM == CM_LANE) {
+			orr = ok ? orr : vr[r];
+			oi = ok ? oi : vi[r];
 		}
 		vr[r] = orr;
 		vi[r] = oi;
 	}
 }
 
+template <int CM, typename R>
+__device__ __forceinline__ void gate_lane_any(R (&vr)[REGS], R (&vi)[REGS],
+					      const M22<R> &M, bool ok, unsigned rmask,
+					      unsigned b, unsigned lane)
+{
+	// distinct asm markers: see gate_reg_any
+	switch (b) {
+	case 0:
+		gate_lane<0, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 0");
+		break;
+	case 1:
+		gate_lane<1, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 1");
+		break;
+	case 2:
+		gate_lane<2, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 2");
+		break;
+	case 3:
+		gate_lane<3, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 3");
+		break;
+	case 4:
+		gate_lane<4, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 4");
+		break;
+	default:
+		gate_lane<5, CM>(vr, vi, M, ok, rmask, lane);
+		asm volatile("; lane bit 5");
+		break;
+	}
+}
+
 // Diagonal gate: factor m00 where the qubit is 0, m11 where it is 1.  The
-// qubit is register bit a (REG) or bit a of the thread word.
-template <bool SEL, bool REG, typename R>
+// qubit is register bit a (REG) or bit a of the thread word.  (The planner
+// sends a phase gate, m00 = 1, as the factor m11 controlled on its own
+// target, so the control modes skip the amplitudes it leaves alone.)
+template <int CM, bool REG, typename R>
 __device__ __forceinline__ void gate_diag(R (&vr)[REGS], R (&vi)[REGS],
 					  const M22<R> &M, bool ok, unsigned rmask,
 					  unsigned a, u64 tb)
@@ -159,13 +233,14 @@ __device__ __forceinline__ void gate_diag(R (&vr)[REGS], R (&vi)[REGS],
 	const Cx<R> ft = cx_sel((tb >> a) & 1, M.m11, M.m00);
 #pragma unroll
 	for (int r = 0; r < REGS; ++r) {
+		if (CM != CM_NONE && !reg_ok(r, rmask))
+			continue;
 		const Cx<R> f = REG ? cx_sel((r >> a) & 1, M.m11, M.m00) : ft;
 		R orr, oi;
 		cx_mul(f, vr[r], vi[r], orr, oi);
-		if (SEL) {
-			const bool c = ok & reg_ok(r, rmask);
-			orr = c ? orr : vr[r];
-			oi = c ? oi : vi[r];
+		if (CM == CM_LANE) {
+			orr = ok ? orr : vr[r];
+			oi = ok ? oi : vi[r];
 		}
 		vr[r] = orr;
 		vi[r] = oi;
@@ -183,23 +258,24 @@ __device__ __forceinline__ M22<R> op_matrix(const PassOp *__restrict__ op)
 	return M;
 }
 
-template <bool SEL, typename R>
+// One gate op.
+template <int CM, typename R>
 __device__ __forceinline__ void run_gate(R (&vr)[REGS], R (&vi)[REGS], unsigned kind,
 					 unsigned a, unsigned rmask, const M22<R> &M,
 					 bool ok, u64 tb, unsigned lane)
 {
 	switch (kind) {
 	case OP_REG:
-		gate_reg_any<SEL>(vr, vi, M, ok, rmask, a);
+		gate_reg_any<CM>(vr, vi, M, ok, rmask, a);
 		break;
 	case OP_LANE:
-		gate_lane<SEL>(vr, vi, M, ok, rmask, a, lane);
+		gate_lane_any<CM>(vr, vi, M, ok, rmask, a, lane);
 		break;
 	case OP_DIAG_REG:
-		gate_diag<SEL, true>(vr, vi, M, ok, rmask, a, tb);
+		gate_diag<CM, true>(vr, vi, M, ok, rmask, a, tb);
 		break;
 	default: // OP_DIAG_THREAD
-		gate_diag<SEL, false>(vr, vi, M, ok, rmask, a, tb);
+		gate_diag<CM, false>(vr, vi, M, ok, rmask, a, tb);
 		break;
 	}
 }
@@ -305,11 +381,16 @@ __device__ __forceinline__ void pass_tile(
 			continue;
 		}
 		const M22<R> M = op_matrix<R>(op);
-		if ((tcm | tam | rmask) != 0) {
-			const bool ok = ((tb & tcm) == tcm) & ((tb & tam) == 0);
-			run_gate<true>(vr, vi, kind, a, rmask, M, ok, tb, lane);
+		const bool ok = ((tb & tcm) == tcm) & ((tb & tam) == 0);
+		if ((tcm | tam | rmask) == 0) {
+			run_gate<CM_NONE>(vr, vi, kind, a, rmask, M, true, tb, lane);
+		} else if (((tcm | tam) & ((1u << LANE_BITS) - 1)) == 0) {
+			// the thread-word condition reads only index bits >= 6: the
+			// same for the whole wave
+			if (__builtin_amdgcn_readfirstlane((int)ok))
+				run_gate<CM_REG>(vr, vi, kind, a, rmask, M, true, tb, lane);
 		} else {
-			run_gate<false>(vr, vi, kind, a, rmask, M, true, tb, lane);
+			run_gate<CM_LANE>(vr, vi, kind, a, rmask, M, ok, tb, lane);
 		}
 	}
 

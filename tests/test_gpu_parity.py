@@ -306,6 +306,21 @@ def test_reduced_density_vs_einsum_and_oracle(capi, oracle, n, keep):
         assert maxabs(rho, oracle.density_matrix(o, chain)) <= ATOL
 
 
+@pytest.mark.parametrize("keep", [[0], [23], [0, 12, 23], [20, 21, 22, 23],
+                                  [1, 3, 8, 13, 18], [1, 2, 3, 4, 5, 6], [2, 7]])
+def test_reduced_density_large_register(capi, keep):
+    """Few kept qubits of a 24-qubit register (the Gram-matrix kernel over
+    many chunks of the state), against the tensor contraction."""
+    n = 24
+    print(f"[density] n={n} keep={keep}", flush=True)
+    psi = W.random_normalised_state(2424, n)
+    d = capi.from_host(psi)
+    rho = capi.reduced_density(d, keep)
+    want = reduced_density_einsum(psi, keep)
+    assert maxabs(rho, want) <= ATOL
+    assert np.allclose(rho, rho.conj().T, atol=1e-15)
+
+
 def test_first_nonzero(capi):
     v = np.zeros(1 << 13, dtype=complex)
     v[5000] = 0.6j
