@@ -10,8 +10,9 @@ reference's apply_gate).  One step = one layer (30 U3 + 15 or 14 CZ).
 value = whole-job HBM GB/s per gate = gates * 32 * 2^n bytes / timed seconds
 (every unfused gate reads and writes each amplitude once: 16 B + 16 B).
 
-Also reported: the gate kernel's own duration from HIP events on the
-library's stream (roofline), per-target kernel GB/s (low and high stride),
+Also reported: the gate kernel's own duration from HIP events recorded on
+the library's stream around every gate of the timed region (roofline),
+per-target kernel GB/s (low and high stride),
 registry_prob on every qubit, and the reference's CPU path timed on the host
 cores over a bounded sample (cpu_baseline).
 """
@@ -64,6 +65,10 @@ def gpu_leg(args):
         reg = run_layer(d, reg, layers[li])
     capi.synchronize()
     log("[bench] warmup done")
+    # every gate kernel of the timed region records a start and an end HIP
+    # event on the library's stream (no synchronisation in between)
+    ntimed = sum(len(layers[li]) for li in range(args.warmup, args.warmup + args.steps))
+    capi.tune("time_ring", ntimed)
     t0 = time.perf_counter()
     ngates = 0
     for li in range(args.warmup, args.warmup + args.steps):
@@ -72,24 +77,20 @@ def gpu_leg(args):
     capi.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    kms = capi.time_ring_read()
+    assert len(kms) == ngates, (len(kms), ngates)
     log(f"[bench] timed {ngates} gates in {elapsed:.3f} s")
 
-    # gate-kernel durations from HIP events on the library stream
-    capi.tune("time_apply", 1)
-    kms = []
-    for g, t, c in layers[-1]:
-        reg = d.registry_apply(reg, g, t, c, None, -1, False)
-        kms.append(capi.event_elapsed_ms(62, 63))
+    # per-target kernel durations: five back-to-back gates per target
     per_target = {}
     u3 = layers[-1][0][0]
     for t in sorted({0, 3, 6, 12, n // 2, n - 10, n - 5, n - 1}):
-        ms = []
         for _ in range(5):
             reg = d.registry_apply(reg, u3, [t], set(), None, -1, False)
-            ms.append(capi.event_elapsed_ms(62, 63))
-        per_target[str(t)] = {"kernel_ms": min(ms),
+        ms = capi.time_ring_read()
+        per_target[str(t)] = {"kernel_ms": min(ms), "avg_ms": sum(ms) / len(ms),
                               "GBps": bpg / (min(ms) * 1e-3) / 1e9}
-    capi.tune("time_apply", 0)
+    capi.tune("time_ring", 0)
     kavg = sum(kms) / len(kms)
     log(f"[bench] gate kernel avg {kavg:.3f} ms -> {bpg / kavg / 1e6:.0f} GB/s")
 
@@ -151,15 +152,14 @@ def big_leg(args):
     cg = [(capi.Gate(m), t, c) for m, t, c, a in steps]
     capi.apply(s, cg[0][0], cg[0][1], cg[0][2], in_place=True)
     capi.synchronize()
-    capi.tune("time_apply", 1)
-    kms = []
+    capi.tune("time_ring", len(cg))
     t0 = time.perf_counter()
     for g, t, c in cg:
         capi.apply(s, g, t, c, in_place=True)
-        kms.append(capi.event_elapsed_ms(62, 63))
     capi.synchronize()
     el = time.perf_counter() - t0
-    capi.tune("time_apply", 0)
+    kms = capi.time_ring_read()
+    capi.tune("time_ring", 0)
     kavg = sum(kms) / len(kms)
     out["in_place_GBps"] = round(len(cg) * bpg / el / 1e9, 1)
     out["kernel_avg_ms"] = round(kavg, 3)
@@ -183,16 +183,15 @@ def fused_leg(args):
     n = args.qubits
     out = {"dense_kernel": {}}
     rng = W.rng_for(4242)
-    capi.tune("time_apply", 1)
+    capi.tune("time_ring", 16)
     for dtype, name, amp_bytes in ((capi.C128, "f64", 16), (capi.C64, "f32", 8)):
         s = capi.new_typed(n, dtype)
         for k in (4, 5, 6):
             g = capi.Gate(W.haar_unitary(rng, k))
-            ms = []
             for rep in range(3):
                 tg = list(range(rep * 7 % (n - k), rep * 7 % (n - k) + k))
                 capi.apply(s, g, tg, in_place=True)
-                ms.append(capi.event_elapsed_ms(62, 63))
+            ms = capi.time_ring_read()
             t = min(ms) * 1e-3
             out["dense_kernel"][f"{name}_k{k}"] = {
                 "kernel_ms": round(t * 1e3, 3),
@@ -200,7 +199,7 @@ def fused_leg(args):
                 "TFLOPs": round(8 * (1 << k) * (1 << n) / t / 1e12, 2)}
         del s
         capi.synchronize()
-    capi.tune("time_apply", 0)
+    capi.tune("time_ring", 0)
     log(f"[bench] dense kernels {out['dense_kernel']}")
 
     depth = args.fused_depth

@@ -71,6 +71,8 @@ def lib():
                                                   C.POINTER(u64), C.POINTER(u64), dp]),
         "dk_event_record": (C.c_int, [C.c_int]),
         "dk_event_elapsed": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "dk_time_ring_read": (C.c_int, [C.POINTER(C.c_float), C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
         "dk_tune": (C.c_int, [C.c_char_p, C.c_int64]),
         "dk_device_info": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
         "dk_state_new_typed": (C.c_int, [C.c_uint, C.c_int, C.c_int, PP]),
@@ -521,6 +523,16 @@ def event_elapsed_ms(a, b):
     v = C.c_float()
     check(lib().dk_event_elapsed(a, b, C.byref(v)))
     return v.value
+
+
+def time_ring_read():
+    """Durations (ms) of the gate-kernel launches recorded in the calling
+    thread's timing ring (tune("time_ring", capacity)) since the last read."""
+    n = C.c_size_t()
+    check(lib().dk_time_ring_read(None, 0, C.byref(n)))
+    out = (C.c_float * max(1, n.value))()
+    check(lib().dk_time_ring_read(out, n.value, C.byref(n)))
+    return [out[i] for i in range(n.value)]
 
 
 def tune(key, value):

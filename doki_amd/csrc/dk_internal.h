@@ -94,6 +94,10 @@ struct dk_lane {
 	double *dec = nullptr;        // measurement decisions scratch (device)
 	double *host_pinned = nullptr;// pinned staging (DK_PINNED doubles)
 	hipEvent_t events[64] = {};
+	// timing ring: event pairs recorded around every gate-kernel launch on
+	// this lane while ring_cap > 0 (dk_tune "time_ring")
+	hipEvent_t *ring = nullptr;   // 2 * ring_cap events
+	size_t ring_cap = 0, ring_used = 0;
 };
 
 struct dk_ctx {
@@ -116,6 +120,11 @@ struct dk_ctx {
 enum { DK_PINNED = 1 << 16 };
 
 extern dk_ctx g_ctx;
+
+// Marks the start (end = 0) or end (end = 1) of a gate-kernel launch on the
+// calling lane: events[62] / [63] when time_apply is set, and the next pair
+// of the lane's timing ring while it has room.
+int dk_time_mark(int end);
 
 // The calling thread's lane, and its stream.
 dk_lane *dk_cur();

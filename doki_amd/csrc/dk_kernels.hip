@@ -972,7 +972,6 @@ void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta,
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm)
 {
-	hipStream_t st = DK_STREAM;
 	double *parts = nullptr;
 	u64 nblocks = 0;
 	int rc;
@@ -982,8 +981,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 			DK_HIP(hipStreamSynchronize(g->lane->stream));
 		g->lane = dk_cur();
 	}
-	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(dk_cur()->events[62], st));
+	DK_TRY(dk_time_mark(0));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
 	    in->nq >= dk_dense_tile_bits(g->k, in->dtype)) {
 		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
@@ -999,8 +997,7 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	}
 	if (rc)
 		return rc;
-	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(dk_cur()->events[63], st));
+	DK_TRY(dk_time_mark(1));
 	return dk_finish_partials(parts, nblocks, out->scal + DK_SCAL_NORM, 1);
 }
 

@@ -485,15 +485,12 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 	const u64 ntiles = in->len >> PASS_TILE_BITS;
 	double *parts = nullptr;
 	DK_TRY(dk_partials_reserve(ntiles, &parts));
-	hipStream_t st = DK_STREAM;
-	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(dk_cur()->events[62], st));
+	DK_TRY(dk_time_mark(0));
 	int rc = in->dtype == DK_C64
 			 ? launch_pass_typed<float2>(in, out, pd, ops_dev, parts, ntiles)
 			 : launch_pass_typed<double2>(in, out, pd, ops_dev, parts, ntiles);
 	if (rc)
 		return rc;
-	if (g_ctx.time_apply)
-		DK_HIP(hipEventRecord(dk_cur()->events[63], st));
+	DK_TRY(dk_time_mark(1));
 	return dk_finish_partials(parts, ntiles, out->scal + DK_SCAL_NORM, 1);
 }

@@ -940,6 +940,65 @@ extern "C" int dk_event_elapsed(int a, int b, float *ms)
 	return DK_OK;
 }
 
+int dk_time_mark(int end)
+{
+	dk_lane *l = dk_cur();
+	if (g_ctx.time_apply)
+		DK_HIP(hipEventRecord(l->events[end ? 63 : 62], l->stream));
+	if (l->ring_used < l->ring_cap) {
+		DK_HIP(hipEventRecord(l->ring[2 * l->ring_used + (end ? 1 : 0)], l->stream));
+		if (end)
+			++l->ring_used;
+	}
+	return DK_OK;
+}
+
+// Sizes the calling lane's timing ring (pairs of events); 0 turns it off.
+static int time_ring_setup(size_t pairs)
+{
+	dk_lane *l = dk_cur();
+	if (pairs > (1u << 16)) {
+		dk_set_error("time_ring holds at most 65536 launches");
+		return DK_E_ARG;
+	}
+	if (l->ring) {
+		DK_HIP(hipStreamSynchronize(l->stream));
+		for (size_t i = 0; i < 2 * l->ring_cap; ++i)
+			(void)hipEventDestroy(l->ring[i]);
+		delete[] l->ring;
+		l->ring = nullptr;
+	}
+	l->ring_cap = l->ring_used = 0;
+	if (!pairs)
+		return DK_OK;
+	l->ring = new hipEvent_t[2 * pairs];
+	for (size_t i = 0; i < 2 * pairs; ++i)
+		DK_HIP(hipEventCreate(&l->ring[i]));
+	l->ring_cap = pairs;
+	return DK_OK;
+}
+
+extern "C" int dk_time_ring_read(float *ms, size_t capacity, size_t *count)
+{
+	DK_INIT();
+	dk_lane *l = dk_cur();
+	const size_t n = l->ring_used;
+	if (count)
+		*count = n;
+	if (!ms)
+		return DK_OK;
+	if (capacity < n) {
+		dk_set_error("room for %zu durations, the ring holds %zu", capacity, n);
+		return DK_E_ARG;
+	}
+	if (n)
+		DK_HIP(hipEventSynchronize(l->ring[2 * n - 1]));
+	for (size_t i = 0; i < n; ++i)
+		DK_HIP(hipEventElapsedTime(&ms[i], l->ring[2 * i], l->ring[2 * i + 1]));
+	l->ring_used = 0;
+	return DK_OK;
+}
+
 extern "C" int dk_tune(const char *key, int64_t value)
 {
 	DK_INIT();
@@ -955,6 +1014,8 @@ extern "C" int dk_tune(const char *key, int64_t value)
 		g_ctx.time_apply = value != 0;
 		return DK_OK;
 	}
+	if (!strcmp(key, "time_ring"))
+		return time_ring_setup(value > 0 ? (size_t)value : 0);
 	if (!strcmp(key, "dense_mfma")) {
 		g_ctx.dense_mfma = value != 0;
 		return DK_OK;

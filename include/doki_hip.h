@@ -341,6 +341,15 @@ int dk_batch_state(const dk_batch *b, size_t i, double *values);
 int dk_event_record(int slot);
 /* Milliseconds between two recorded events (waits for the second). */
 int dk_event_elapsed(int first_slot, int second_slot, float *ms);
+
+/* Durations (ms) of the gate-kernel launches recorded in the calling lane's
+ * timing ring since the last read (dk_tune "time_ring" = capacity in
+ * launches; every gate, dense-block and pass launch records a start and end
+ * event on its own stream, without synchronising).  Waits for the last
+ * recorded launch, then empties the ring.  *count (may be NULL) receives the
+ * number of durations; ms may be NULL to query it.  No reference
+ * counterpart: measurement. */
+int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
 /* Tunables: "apply_blocks" (grid cap of grid-stride kernels, 0 = auto),
  * "time_apply" (events 62/63 bracket every gate kernel), "dense_mfma"
  * (dense k >= 4 gates on MFMA, else on the vector ALUs), "apply1_group"

@@ -125,3 +125,25 @@ def test_30_qubit_passes_equal_per_gate(capi):
         rb, nb = b.raw(int(start), 4096)
         assert maxabs(ra / na, rb / nb) <= ATOL
     assert abs(na - 1.0) <= 1e-12 and abs(nb - 1.0) <= 1e-12
+
+
+def test_time_ring_records_every_launch(capi):
+    """The timing ring holds one duration per gate, block or pass launch,
+    up to its capacity, and empties on read."""
+    n = 16
+    steps, _ = W.random_circuit(8, n, 60)
+    circ = capi.Circuit(n, steps)
+    circ.passes()
+    nsteps = len(circ.plan())
+    st = capi.new(n)
+    capi.tune("time_ring", 64)
+    try:
+        circ.run(st, in_place=True)
+        ms = capi.time_ring_read()
+        assert len(ms) == nsteps and all(m > 0 for m in ms)
+        assert capi.time_ring_read() == []
+        capi.tune("time_ring", 2)
+        circ.run(st, in_place=True)
+        assert len(capi.time_ring_read()) == min(2, nsteps)
+    finally:
+        capi.tune("time_ring", 0)
