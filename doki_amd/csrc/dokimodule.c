@@ -13,7 +13,13 @@
  *  - repeated target qubits raise doki.error (doki.c:866-888 does not check);
  *  - a registry passed to registry_del can no longer be used (raises instead
  *    of dangling, doki.c:400-415);
- *  - num_threads is validated and ignored (work runs on the GPU).
+ *  - num_threads is validated and ignored (work runs on the GPU);
+ *  - funmatrix_create keeps the reference's layout (element (i, j) of an
+ *    r x c input at i * r + j) but drops values that would land past the
+ *    array (r > c; the reference writes them out of bounds), and reads
+ *    there give NaN; funmatrix_mem of a node over a created matrix returns
+ *    its byte count (the reference raises SystemError there);
+ *  - constructors of more than 62 qubits raise doki.error (2^n overflows).
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -870,48 +876,148 @@ static PyObject *doki_registry_prob(PyObject *self, PyObject *args)
 }
 
 /* ------------------------------------------------------------------------ */
-/* funmatrix: density matrices of registries and their partial traces       */
+/* funmatrix: the lazy matrix algebra                                       */
 /* ------------------------------------------------------------------------ */
+/*
+ * A funmatrix is an immutable node: a leaf (custom values, identity, the
+ * |0...0> state or density, Walsh/Hadamard, controlled U, I (x) U (x) I, a
+ * projection, a partial trace, a gate applied to a state, or the density
+ * matrix of a registry) or a binary operation on two funmatrices (sum,
+ * difference, matrix product, element-wise product, Kronecker product).
+ * Every node carries a scalar factor and transpose / conjugate flags.
+ * Element (i, j) is evaluated on demand with the reference's rules
+ * (getitem, funmatrix.c:238-340): the bounds are those of the node's own
+ * shape, which transposition leaves as it was (funmatrix.c:761-839); then
+ * the indices are swapped if the node is transposed, the value computed,
+ * conjugated if asked and multiplied by the scalar.  Evaluation codes: 0 ok,
+ * 1-5 a parent of a sum / difference / product / element-wise product /
+ * Kronecker product failed, 7 out of bounds, 8 NaN, -1 Python error set.
+ * Density leaves of registries are evaluated on the GPU.
+ * funmatrix_mem reports the bytes the reference's structures would take on
+ * LP64 (struct FMatrix 112, Matrix2D 40, Projection 40, DMatrixForTrace 24,
+ * Application 72; funmatrix.c:1301-1566, qops.c:295-447).
+ */
+enum {
+	FM_DENSITY = 1, FM_PTRACE, FM_CUSTOM, FM_IDENTITY, FM_STATEZERO,
+	FM_DENSITYZERO, FM_WALSH, FM_CU, FM_EYEKRON, FM_PROJECTION, FM_APPLY,
+	FM_OP
+};
+enum { FMOP_ADD = 0, FMOP_SUB, FMOP_MATMUL, FMOP_EWMUL, FMOP_KRON };
 
-enum { FM_DENSITY = 1, FM_PTRACE = 2 };
+#define REF_MATRIX2D_BYTES 40
+#define REF_PROJECTION_BYTES 40
+#define REF_APPLICATION_BYTES 72
+
+/* Payload shared by a node and its scaled / transposed copies. */
+typedef struct {
+	Py_ssize_t refs;
+	int64_t len;
+	cplx *v;              /* custom values at flat index i * rows + j */
+	unsigned nt, nc, na;  /* gate application: counts of ... */
+	unsigned *ids;        /* ... targets, then controls, then anticontrols */
+} fm_buf_t;
 
 typedef struct fm_t {
 	int kind;
 	int64_t r, c;
+	cplx s;               /* scalar factor */
+	int transpose, conjugate;
 	size_t mem;           /* bytes the reference would report */
+	int op;               /* FM_OP */
+	PyObject *a_cap;      /* owned: first operand, or the parent / U /
+	                         state of a leaf that reads another matrix */
+	PyObject *b_cap;      /* owned: second operand, or the applied gate */
 	/* FM_DENSITY: reduced density of the kept qubits of a registry */
 	PyObject *state_cap;  /* owned reference */
 	uint64_t keep_mask;
 	unsigned keep_count;
 	unsigned char keep_pos[64];
 	double *cache;        /* materialised matrix, or NULL */
-	/* FM_PTRACE: generic partial trace of a parent funmatrix */
-	PyObject *parent_cap; /* owned reference */
-	int e;
+	int e;                /* FM_PTRACE: index bit traced out */
+	int hadamard;         /* FM_WALSH */
+	int64_t right;        /* FM_EYEKRON: 2^rightQ (the left factor only
+	                         shapes the matrix) */
+	uint64_t qubit;       /* FM_PROJECTION */
+	int value;
+	fm_buf_t *buf;        /* FM_CUSTOM, FM_APPLY */
 } fm_t;
 
 #define FM_MATERIALISE_MAX_QUBITS 10
 
+static void fm_buf_release(fm_buf_t *b)
+{
+	if (b && --b->refs == 0) {
+		PyMem_Free(b->v);
+		PyMem_Free(b->ids);
+		PyMem_Free(b);
+	}
+}
+
+static void fm_free(fm_t *f)
+{
+	Py_XDECREF(f->a_cap);
+	Py_XDECREF(f->b_cap);
+	Py_XDECREF(f->state_cap);
+	fm_buf_release(f->buf);
+	PyMem_Free(f->cache);
+	PyMem_Free(f);
+}
+
 static void fm_capsule_destroy(PyObject *cap)
 {
 	fm_t *f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
-	if (!f)
-		return;
-	Py_XDECREF(f->state_cap);
-	Py_XDECREF(f->parent_cap);
-	PyMem_Free(f->cache);
-	PyMem_Free(f);
+	if (f)
+		fm_free(f);
 }
 
 static PyObject *fm_wrap(fm_t *f)
 {
 	PyObject *cap = PyCapsule_New(f, CAP_FMAT, fm_capsule_destroy);
-	if (!cap) {
-		Py_XDECREF(f->state_cap);
-		Py_XDECREF(f->parent_cap);
-		PyMem_Free(f);
-	}
+	if (!cap)
+		fm_free(f);
 	return cap;
+}
+
+static fm_t *fm_new(int kind, int64_t r, int64_t c)
+{
+	fm_t *f = (fm_t *)PyMem_Calloc(1, sizeof(fm_t));
+	if (!f) {
+		PyErr_NoMemory();
+		return NULL;
+	}
+	f->kind = kind;
+	f->r = r;
+	f->c = c;
+	f->s = 1.0;
+	f->mem = REF_FMATRIX_BYTES;
+	return f;
+}
+
+/* The node of a funmatrix capsule, or NULL (no error set). */
+static fm_t *fm_of(PyObject *cap)
+{
+	fm_t *f = (fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT);
+	if (!f)
+		PyErr_Clear();
+	return f;
+}
+
+/* A copy sharing operands and payload (scaled, transposed, conjugated). */
+static fm_t *fm_clone(const fm_t *f)
+{
+	fm_t *o = (fm_t *)PyMem_Malloc(sizeof(fm_t));
+	if (!o) {
+		PyErr_NoMemory();
+		return NULL;
+	}
+	*o = *f;
+	o->cache = NULL;
+	Py_XINCREF(o->a_cap);
+	Py_XINCREF(o->b_cap);
+	Py_XINCREF(o->state_cap);
+	if (o->buf)
+		o->buf->refs++;
+	return o;
 }
 
 static PyObject *doki_registry_density(PyObject *self, PyObject *args)
@@ -929,11 +1035,9 @@ static PyObject *doki_registry_density(PyObject *self, PyObject *args)
 	r = reg_get(cap, "NULL pointer to registry");
 	if (!r)
 		return NULL;
-	f = (fm_t *)PyMem_Calloc(1, sizeof(fm_t));
+	f = fm_new(FM_DENSITY, (int64_t)1 << r->nq, (int64_t)1 << r->nq);
 	if (!f)
-		return PyErr_NoMemory();
-	f->kind = FM_DENSITY;
-	f->r = f->c = (int64_t)1 << r->nq;
+		return NULL;
 	f->mem = REF_FMATRIX_BYTES + (size_t)dk_state_mem_size(r->st);
 	f->state_cap = cap;
 	Py_INCREF(cap);
@@ -944,8 +1048,6 @@ static PyObject *doki_registry_density(PyObject *self, PyObject *args)
 	return fm_wrap(f);
 }
 
-/* Element (i, j) following getitem's rules (funmatrix.c:238-340): 0 ok,
- * 7 out of bounds, 8 NaN, -1 Python error set. */
 static int fm_get(fm_t *f, int64_t i, int64_t j, cplx *out);
 
 static int64_t ins_bit(int value, int64_t pos, int bit)
@@ -994,49 +1096,251 @@ static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 	}
 }
 
+/* Element of another matrix read by a leaf: NaN when that read fails (the
+ * leaf then reports code 8); -1 on a Python error. */
+static int fm_read(PyObject *cap, int64_t i, int64_t j, cplx *out)
+{
+	fm_t *m = fm_of(cap);
+	int rc;
+	if (!m) {
+		*out = NAN;
+		return 0;
+	}
+	rc = fm_get(m, i, j, out);
+	if (rc < 0)
+		return -1;
+	if (rc)
+		*out = NAN;
+	return 0;
+}
+
+/* Value of a leaf at (i, j), indices already swapped for transposition.
+ * Follows the reference's element functions: _CustomMat, _IdentityFunction,
+ * _StateZeroFunction, _DensityZeroFunction, _WalshFunction, _CUFunction,
+ * _eyeKronFunction, _projectionFunction, _PartialTFunct
+ * (funmatrix.c:680-1566) and _ApplyGateFunction (qops.c:449-563). */
+static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
+{
+	switch (f->kind) {
+	case FM_DENSITY:
+		return fm_density_get(f, i, j, out);
+	case FM_PTRACE: {
+		cplx a, b = 0;
+		if (fm_read(f->a_cap, ins_bit(0, i, f->e), ins_bit(0, j, f->e), &a) ||
+		    fm_read(f->a_cap, ins_bit(1, i, f->e), ins_bit(1, j, f->e), &b))
+			return -1;
+		*out = a + b;
+		return 0;
+	}
+	case FM_CUSTOM: {
+		/* flat index with the row count as stride (funmatrix.c:1514);
+		 * indices past the stored values read NaN */
+		int64_t k = i * f->r + j;
+		*out = (k >= 0 && k < f->buf->len) ? f->buf->v[k] : NAN;
+		return 0;
+	}
+	case FM_IDENTITY:
+		*out = (double)(i == j) + I * 0.0;
+		return 0;
+	case FM_STATEZERO:
+		*out = (double)(i == 0) + I * 0.0;
+		return 0;
+	case FM_DENSITYZERO:
+		*out = (double)(i == 0 && j == 0) + I * 0.0;
+		return 0;
+	case FM_WALSH: {
+		/* Sylvester order: (-1)^popcount(i & j), over sqrt(size) for
+		 * Hadamard */
+		double number = (__builtin_popcountll((uint64_t)(i & j)) & 1) ? -1.0 : 1.0;
+		if (f->hadamard)
+			number /= sqrt((double)f->r);
+		*out = number + I * 0.0;
+		return 0;
+	}
+	case FM_CU: {
+		fm_t *u = fm_of(f->a_cap);
+		if (!u) {
+			*out = NAN;
+			return 0;
+		}
+		if (i < u->r || j < u->c) {
+			*out = (double)(i == j) + I * 0.0;
+			return 0;
+		}
+		return fm_read(f->a_cap, i - u->r, j - u->c, out);
+	}
+	case FM_EYEKRON: {
+		fm_t *u = fm_of(f->a_cap);
+		if (!u) {
+			*out = NAN;
+			return 0;
+		}
+		if (i % f->right != j % f->right) {
+			*out = 0;
+			return 0;
+		}
+		i /= f->right;
+		j /= f->right;
+		if (i / u->r != j / u->c) {
+			*out = 0;
+			return 0;
+		}
+		return fm_read(f->a_cap, i % u->r, j % u->c, out);
+	}
+	case FM_PROJECTION: {
+		const int set = f->qubit < 63 && ((i >> f->qubit) & 1);
+		if (set != f->value) {
+			*out = 0;
+			return 0;
+		}
+		return fm_read(f->a_cap, i, j, out);
+	}
+	case FM_APPLY: {
+		const fm_buf_t *b = f->buf;
+		const unsigned *tg = b->ids, *ct = b->ids + b->nt,
+			       *ac = b->ids + b->nt + b->nc;
+		fm_t *g = fm_of(f->b_cap);
+		int64_t n, reg = i;
+		unsigned k;
+		cplx val = 0, x, y;
+		(void)j;
+		for (k = 0; k < b->nc; k++)
+			if (!((i >> ct[k]) & 1))
+				return fm_read(f->a_cap, i, 0, out);
+		for (k = 0; k < b->na; k++)
+			if ((i >> ac[k]) & 1)
+				return fm_read(f->a_cap, i, 0, out);
+		if (!g) {
+			*out = NAN;
+			return 0;
+		}
+		for (n = 0; n < g->r; n++) {
+			int64_t row = 0;
+			for (k = 0; k < b->nt; k++) {
+				row += (int64_t)((i >> tg[k]) & 1) << k;
+				if ((n >> k) & 1)
+					reg |= (int64_t)1 << tg[k];
+				else
+					reg &= ~((int64_t)1 << tg[k]);
+			}
+			if (fm_read(f->a_cap, reg, 0, &x) || fm_read(f->b_cap, row, n, &y))
+				return -1;
+			if (isnan(creal(x)) || isnan(cimag(x)) || isnan(creal(y)) ||
+			    isnan(cimag(y))) {
+				*out = NAN;
+				return 0;
+			}
+			val = val + x * y;
+		}
+		*out = val;
+		return 0;
+	}
+	default:
+		*out = NAN;
+		return 0;
+	}
+}
+
+/* A binary operation node at (i, j). */
+static int fm_op(fm_t *f, int64_t i, int64_t j, cplx *out)
+{
+	fm_t *a = fm_of(f->a_cap), *b = fm_of(f->b_cap);
+	static const int fail[] = {1, 2, 3, 4, 5};
+	cplx x = NAN, y = NAN;
+	int r1, r2;
+	int64_t k;
+	if (!a || !b)
+		return fail[f->op];
+	switch (f->op) {
+	case FMOP_MATMUL:
+		*out = 0;
+		for (k = 0; k < a->c; k++) {
+			r1 = fm_get(a, i, k, &x);
+			r2 = r1 < 0 ? 0 : fm_get(b, k, j, &y);
+			if (r1 < 0 || r2 < 0)
+				return -1;
+			if (r1 || r2)
+				return fail[f->op];
+			*out = *out + x * y;
+		}
+		return 0;
+	case FMOP_KRON:
+		r1 = fm_get(a, i / b->r, j / b->c, &x);
+		r2 = r1 < 0 ? 0 : fm_get(b, i % b->r, j % b->c, &y);
+		break;
+	default:
+		r1 = fm_get(a, i, j, &x);
+		r2 = r1 < 0 ? 0 : fm_get(b, i, j, &y);
+		break;
+	}
+	if (r1 < 0 || r2 < 0)
+		return -1;
+	if (r1 || r2)
+		return fail[f->op];
+	switch (f->op) {
+	case FMOP_ADD:
+		*out = x + y;
+		break;
+	case FMOP_SUB:
+		*out = x - y;
+		break;
+	default: /* element-wise and Kronecker products */
+		*out = x * y;
+		break;
+	}
+	return 0;
+}
+
 static int fm_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 {
+	int rc;
 	*out = NAN;
 	if (!(i >= 0 && j >= 0 && i < f->r && j < f->c))
 		return 7;
-	if (f->kind == FM_DENSITY) {
-		int rc = fm_density_get(f, i, j, out);
-		if (rc)
-			return rc;
-	} else {
-		/* _PartialTFunct, funmatrix.c:920-944 */
-		fm_t *p = (fm_t *)PyCapsule_GetPointer(f->parent_cap, CAP_FMAT);
-		cplx a = 0, b = 0;
-		int r1, r2 = 0;
-		if (!p) {
-			PyErr_Clear();
-			return 8;
-		}
-		r1 = fm_get(p, ins_bit(0, i, f->e), ins_bit(0, j, f->e), &a);
-		if (r1 < 0)
-			return -1;
-		if (r1 == 0) {
-			r2 = fm_get(p, ins_bit(1, i, f->e), ins_bit(1, j, f->e), &b);
-			if (r2 < 0)
-				return -1;
-		}
-		*out = (r1 == 0 && r2 == 0) ? a + b : a;
+	if (f->transpose) {
+		int64_t t = i;
+		i = j;
+		j = t;
 	}
-	if (isnan(creal(*out)) || isnan(cimag(*out)))
-		return 8;
-	return 0;
+	if (f->kind == FM_OP) {
+		rc = fm_op(f, i, j, out);
+	} else {
+		rc = fm_leaf(f, i, j, out);
+		if (rc == 0 && (isnan(creal(*out)) || isnan(cimag(*out))))
+			rc = 8;
+	}
+	if (rc == 0 && f->conjugate)
+		*out = conj(*out);
+	if (rc == 0)
+		*out = *out * f->s;
+	return rc;
 }
 
 static const char *get_err_text(int code, int trace)
 {
-	switch (code) {
-	case 7:
-		return trace ? "[TRACE] Element out of bounds" : "[GET] Element out of bounds";
-	case 8:
-		return trace ? "[TRACE] f returned NAN" : "[GET] f returned NAN";
-	default:
-		return trace ? "[TRACE] Unknown error code" : "[GET] Unknown error code";
-	}
+	static const char *const get_txt[] = {
+		"[GET] Unknown error code",
+		"[GET] Error adding parent matrices",
+		"[GET] Error substracting parent matrices",
+		"[GET] Error multiplying parent matrices",
+		"[GET] Error multiplying entity-wise parent matrices",
+		"[GET] Error calculating Kronecker product of parent matrices",
+		"[GET] Unknown operation between parent matrices",
+		"[GET] Element out of bounds",
+		"[GET] f returned NAN"};
+	static const char *const trace_txt[] = {
+		"[TRACE] Unknown error code",
+		"[TRACE] Error adding parent matrices",
+		"[TRACE] Error substracting parent matrices",
+		"[TRACE] Error multiplying parent matrices",
+		"[TRACE] Error multiplying entity-wise parent matrices",
+		"[TRACE] Error calculating Kronecker product of parent matrices",
+		"[TRACE] Unknown operation between parent matrices",
+		"[TRACE] Element out of bounds",
+		"[TRACE] f returned NAN"};
+	if (code < 1 || code > 8)
+		code = 0;
+	return trace ? trace_txt[code] : get_txt[code];
 }
 
 static PyObject *doki_funmatrix_get(PyObject *self, PyObject *args)
@@ -1121,15 +1425,15 @@ static PyObject *doki_funmatrix_partialtrace(PyObject *self, PyObject *args)
 		PyErr_SetString(DokiError, "[PTRACE] elem id has to be >= 0");
 		return NULL;
 	}
-	o = (fm_t *)PyMem_Calloc(1, sizeof(fm_t));
+	o = fm_new(FM_PTRACE, f->r >> 1, f->c >> 1);
 	if (!o)
-		return PyErr_NoMemory();
-	o->r = f->r >> 1;
-	o->c = f->c >> 1;
+		return NULL;
 	o->mem = REF_FMATRIX_BYTES + REF_PTRACE_ARGV_BYTES + f->mem;
-	if (f->kind == FM_DENSITY && (unsigned)e < f->keep_count) {
-		/* trace out the e-th remaining qubit: same node kind, one fewer
-		 * kept qubit (higher ones shift down, funmatrix.c:905-918) */
+	if (f->kind == FM_DENSITY && (unsigned)e < f->keep_count &&
+	    f->s == 1.0 && !f->transpose && !f->conjugate) {
+		/* trace out the e-th remaining qubit of a registry density:
+		 * the same node kind with one fewer kept qubit (higher ones
+		 * shift down, funmatrix.c:905-918), evaluated on the GPU */
 		unsigned k, w = 0;
 		o->kind = FM_DENSITY;
 		o->state_cap = f->state_cap;
@@ -1140,8 +1444,7 @@ static PyObject *doki_funmatrix_partialtrace(PyObject *self, PyObject *args)
 		o->keep_count = w;
 		o->keep_mask = f->keep_mask & ~((uint64_t)1 << f->keep_pos[e]);
 	} else {
-		o->kind = FM_PTRACE;
-		o->parent_cap = cap;
+		o->a_cap = cap;
 		Py_INCREF(cap);
 		o->e = e;
 	}
@@ -1201,37 +1504,573 @@ static PyObject *doki_funmatrix_mem(PyObject *self, PyObject *args)
 	return PyLong_FromSize_t(f->mem);
 }
 
-/* The lazy funmatrix algebra of the reference (doki.c:1297-1497,
- * 1581-2055, 2217-2436) is not part of the state-vector core; these entries
- * exist so that the method table matches, and say so when called. */
-#define FM_UNSUPPORTED(name)                                                 \
-	static PyObject *doki_##name(PyObject *self, PyObject *args)         \
-	{                                                                    \
-		(void)self;                                                  \
-		(void)args;                                                  \
-		PyErr_SetString(DokiError, #name ": the lazy funmatrix algebra " \
-					   "is not provided by the device core"); \
-		return NULL;                                                 \
+/* ---- constructors: doki.c:1297-1497 ------------------------------------ */
+
+/* A number argument (complex, float or int) as the reference reads one. */
+static int fm_number(PyObject *o, cplx *out)
+{
+	if (PyComplex_Check(o))
+		*out = PyComplex_RealAsDouble(o) + I * PyComplex_ImagAsDouble(o);
+	else if (PyFloat_Check(o))
+		*out = PyFloat_AsDouble(o) + I * 0.0;
+	else if (PyLong_Check(o))
+		*out = (double)PyLong_AsLong(o) + I * 0.0;
+	else
+		return -1;
+	return 0;
+}
+
+static PyObject *doki_funmatrix_create(PyObject *self, PyObject *args)
+{
+	PyObject *list, *row;
+	Py_ssize_t nr, nc, i, j;
+	int verbose;
+	fm_buf_t *b;
+	fm_t *f;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &list, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_create(matrix, verbose)");
+		return NULL;
+	}
+	if (!PyList_Check(list)) {
+		PyErr_SetString(DokiError, "matrix must be a list of lists (matrix)");
+		return NULL;
+	}
+	nr = PyList_Size(list);
+	if (nr == 0) {
+		PyErr_SetString(DokiError, "there are no rows");
+		return NULL;
+	}
+	row = PyList_GetItem(list, 0);
+	if (!PyList_Check(row)) {
+		PyErr_SetString(DokiError, "rows must be lists");
+		return NULL;
+	}
+	nc = PyList_Size(row);
+	if (nc == 0) {
+		PyErr_SetString(DokiError, "there are no columns");
+		return NULL;
+	}
+	b = (fm_buf_t *)PyMem_Calloc(1, sizeof(fm_buf_t));
+	if (!b)
+		return PyErr_NoMemory();
+	b->refs = 1;
+	b->len = (int64_t)nr * nc;
+	b->v = (cplx *)PyMem_Calloc((size_t)b->len, sizeof(cplx));
+	if (!b->v) {
+		fm_buf_release(b);
+		return PyErr_NoMemory();
+	}
+	for (i = 0; i < nr; i++) {
+		row = PyList_GetItem(list, i);
+		if (!PyList_Check(row) || PyList_Size(row) != nc) {
+			PyErr_SetString(DokiError, "rows must be lists of the same size");
+			fm_buf_release(b);
+			return NULL;
+		}
+		for (j = 0; j < nc; j++) {
+			cplx v;
+			int64_t k = (int64_t)i * nr + j;
+			if (fm_number(PyList_GetItem(row, j), &v)) {
+				PyErr_SetString(DokiError,
+						"matrix elements must be complex numbers");
+				fm_buf_release(b);
+				return NULL;
+			}
+			/* the reference's layout (row stride = row count,
+			 * doki.c:1368); values that would land past the array
+			 * are dropped (the reference writes them out of bounds) */
+			if (k < b->len)
+				b->v[k] = v;
+		}
+	}
+	f = fm_new(FM_CUSTOM, nr, nc);
+	if (!f) {
+		fm_buf_release(b);
+		return NULL;
+	}
+	f->buf = b;
+	f->mem = REF_FMATRIX_BYTES + REF_MATRIX2D_BYTES + (size_t)b->len * sizeof(cplx);
+	return fm_wrap(f);
+}
+
+/* identity, statezero, densityzero, hadamard: 2^n-sized leaves */
+static PyObject *fm_sized_leaf(PyObject *args, const char *syntax, int kind)
+{
+	unsigned int n;
+	int verbose;
+	fm_t *f;
+	int64_t size;
+	if (!PyArg_ParseTuple(args, "Ip", &n, &verbose)) {
+		PyErr_SetString(DokiError, syntax);
+		return NULL;
+	}
+	if (n > 62) {
+		PyErr_SetString(DokiError, "Number of qubits exceeds maximum");
+		return NULL;
+	}
+	size = (int64_t)1 << n;
+	f = fm_new(kind, size, kind == FM_STATEZERO ? 1 : size);
+	if (!f)
+		return NULL;
+	return fm_wrap(f);
+}
+
+static PyObject *doki_funmatrix_identity(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_sized_leaf(args, "Syntax: funmatrix_identity(num_qubits, verbose)",
+			     FM_IDENTITY);
+}
+
+static PyObject *doki_funmatrix_statezero(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_sized_leaf(args, "Syntax: funmatrix_statezero(num_qubits, verbose)",
+			     FM_STATEZERO);
+}
+
+static PyObject *doki_funmatrix_densityzero(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_sized_leaf(args,
+			     "Syntax: funmatrix_densityzero(num_qubits, verbose)",
+			     FM_DENSITYZERO);
+}
+
+static PyObject *doki_funmatrix_hadamard(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	(void)self;
+	cap = fm_sized_leaf(args, "Syntax: funmatrix_hadamard(num_qubits, verbose)",
+			    FM_WALSH);
+	if (cap)
+		((fm_t *)PyCapsule_GetPointer(cap, CAP_FMAT))->hadamard = 1;
+	return cap;
+}
+
+static PyObject *doki_funmatrix_addcontrol(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose;
+	fm_t *u, *f;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_addcontrol(funmatrix, verbose)");
+		return NULL;
+	}
+	u = fm_of(cap);
+	if (!u) {
+		PyErr_SetString(DokiError, "NULL pointer to matrix");
+		return NULL;
+	}
+	f = fm_new(FM_CU, u->r * 2, u->c * 2);
+	if (!f)
+		return NULL;
+	f->a_cap = cap;
+	Py_INCREF(cap);
+	f->mem = REF_FMATRIX_BYTES + u->mem;
+	return fm_wrap(f);
+}
+
+/* ---- operations: doki.c:1581-2055 ---------------------------------------- */
+
+/* Binary operation node; NULL with doki.error set as the reference's
+ * wrappers do (tag: "ADD", "SUB", "MATMUL", "EWMUL", "KRON"). */
+static PyObject *fm_binary(PyObject *ca, PyObject *cb, int op, const char *tag)
+{
+	fm_t *a = fm_of(ca), *b = fm_of(cb), *f;
+	int64_t r, c;
+	char msg[96];
+	if (!a || !b) {
+		snprintf(msg, sizeof(msg), "[%s] The %s operand is NULL", tag,
+			 !a ? "first" : "second");
+		PyErr_SetString(DokiError, msg);
+		return NULL;
+	}
+	switch (op) {
+	case FMOP_MATMUL:
+		if (a->c != b->r)
+			goto misaligned;
+		r = a->r;
+		c = b->c;
+		break;
+	case FMOP_KRON:
+		r = a->r * b->r;
+		c = a->c * b->c;
+		break;
+	case FMOP_EWMUL:
+		if (a->r == b->r && a->c == b->c) {
+			r = a->r;
+			c = a->c;
+			break;
+		}
+		/* row .* column and column .* row are outer products
+		 * (funmatrix.c:580-584) */
+		if (a->r == 1 && b->c == 1)
+			return fm_binary(cb, ca, FMOP_MATMUL, tag);
+		if (b->r == 1 && a->c == 1)
+			return fm_binary(ca, cb, FMOP_MATMUL, tag);
+		goto misaligned;
+	default:
+		if (a->r != b->r || a->c != b->c)
+			goto misaligned;
+		r = a->r;
+		c = a->c;
+		break;
+	}
+	f = fm_new(FM_OP, r, c);
+	if (!f)
+		return NULL;
+	f->op = op;
+	f->a_cap = ca;
+	f->b_cap = cb;
+	Py_INCREF(ca);
+	Py_INCREF(cb);
+	f->mem = REF_FMATRIX_BYTES + a->mem + b->mem;
+	return fm_wrap(f);
+misaligned:
+	snprintf(msg, sizeof(msg), "[%s] The operands are misalligned", tag);
+	PyErr_SetString(DokiError, msg);
+	return NULL;
+}
+
+#define FM_BINARY_ENTRY(name, op, tag)                                        \
+	static PyObject *doki_funmatrix_##name(PyObject *self, PyObject *args)  \
+	{                                                                     \
+		PyObject *a, *b;                                              \
+		int verbose;                                                  \
+		(void)self;                                                   \
+		if (!PyArg_ParseTuple(args, "OOp", &a, &b, &verbose)) {       \
+			PyErr_SetString(DokiError, "Syntax: funmatrix_" #name \
+					"(funmatrix1, funmatrix2, verbose)"); \
+			return NULL;                                          \
+		}                                                             \
+		return fm_binary(a, b, op, tag);                              \
 	}
 
-FM_UNSUPPORTED(funmatrix_create)
-FM_UNSUPPORTED(funmatrix_identity)
-FM_UNSUPPORTED(funmatrix_statezero)
-FM_UNSUPPORTED(funmatrix_densityzero)
-FM_UNSUPPORTED(funmatrix_hadamard)
-FM_UNSUPPORTED(funmatrix_addcontrol)
-FM_UNSUPPORTED(funmatrix_add)
-FM_UNSUPPORTED(funmatrix_sub)
-FM_UNSUPPORTED(funmatrix_scalar_mul)
-FM_UNSUPPORTED(funmatrix_scalar_div)
-FM_UNSUPPORTED(funmatrix_matmul)
-FM_UNSUPPORTED(funmatrix_ewmul)
-FM_UNSUPPORTED(funmatrix_kron)
-FM_UNSUPPORTED(funmatrix_eyekron)
-FM_UNSUPPORTED(funmatrix_transpose)
-FM_UNSUPPORTED(funmatrix_dagger)
-FM_UNSUPPORTED(funmatrix_projection)
-FM_UNSUPPORTED(funmatrix_apply)
+FM_BINARY_ENTRY(add, FMOP_ADD, "ADD")
+FM_BINARY_ENTRY(sub, FMOP_SUB, "SUB")
+FM_BINARY_ENTRY(matmul, FMOP_MATMUL, "MATMUL")
+FM_BINARY_ENTRY(ewmul, FMOP_EWMUL, "EWMUL")
+FM_BINARY_ENTRY(kron, FMOP_KRON, "KRON")
+
+/* scalar_mul (div = 0) and scalar_div (div = 1): a scaled copy */
+static PyObject *fm_scaled(PyObject *args, int div)
+{
+	PyObject *cap, *raw;
+	int verbose;
+	cplx x;
+	fm_t *m, *f;
+	if (!PyArg_ParseTuple(args, "OOp", &cap, &raw, &verbose)) {
+		PyErr_SetString(DokiError, div ? "Syntax: funmatrix_div(funmatrix, scalar, verbose)"
+					       : "Syntax: funmatrix_mul(funmatrix, scalar, verbose)");
+		return NULL;
+	}
+	if (fm_number(raw, &x)) {
+		PyErr_SetString(DokiError, "scalar is not a number");
+		return NULL;
+	}
+	if (div && creal(x) == 0 && cimag(x) == 0) {
+		PyErr_SetString(DokiError, "Dividing by zero");
+		return NULL;
+	}
+	m = fm_of(cap);
+	if (!m) {
+		PyErr_SetString(DokiError, div ? "[SDIV] The matrix operand is NULL"
+					       : "[SPROD] The matrix operand is NULL");
+		return NULL;
+	}
+	f = fm_clone(m);
+	if (!f)
+		return NULL;
+	f->s = div ? m->s / x : m->s * x;
+	return fm_wrap(f);
+}
+
+static PyObject *doki_funmatrix_scalar_mul(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_scaled(args, 0);
+}
+
+static PyObject *doki_funmatrix_scalar_div(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_scaled(args, 1);
+}
+
+static PyObject *doki_funmatrix_eyekron(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned int left, right;
+	int verbose;
+	fm_t *u, *f;
+	int64_t l, rr;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OIIp", &cap, &left, &right, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_eyekron(funmatrix, "
+					   "leftQubits, rightQubits, verbose)");
+		return NULL;
+	}
+	u = fm_of(cap);
+	if (!u) {
+		PyErr_SetString(DokiError, "[EYEKRON] The matrix is NULL");
+		return NULL;
+	}
+	if (left > 62 || right > 62) {
+		PyErr_SetString(DokiError, "Number of qubits exceeds maximum");
+		return NULL;
+	}
+	l = (int64_t)1 << left;
+	rr = (int64_t)1 << right;
+	f = fm_new(FM_EYEKRON, l * u->r * rr, l * u->c * rr);
+	if (!f)
+		return NULL;
+	f->a_cap = cap;
+	Py_INCREF(cap);
+	f->right = rr;
+	f->mem = REF_FMATRIX_BYTES + REF_MATRIX2D_BYTES + u->mem + 2 * sizeof(int64_t);
+	return fm_wrap(f);
+}
+
+/* transpose (conj = 0) and dagger (conj = 1): a flipped copy, same shape */
+static PyObject *fm_flipped(PyObject *args, int conj_too)
+{
+	PyObject *cap;
+	int verbose;
+	fm_t *m, *f;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, conj_too ? "Syntax: funmatrix_dagger(funmatrix, verbose)"
+						    : "Syntax: funmatrix_transpose(funmatrix, verbose)");
+		return NULL;
+	}
+	m = fm_of(cap);
+	if (!m) {
+		PyErr_SetString(DokiError, conj_too ? "[HTRANS] The matrix is NULL"
+						    : "[TRANS] The matrix is NULL");
+		return NULL;
+	}
+	f = fm_clone(m);
+	if (!f)
+		return NULL;
+	f->transpose = !m->transpose;
+	if (conj_too)
+		f->conjugate = !m->conjugate;
+	return fm_wrap(f);
+}
+
+static PyObject *doki_funmatrix_transpose(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_flipped(args, 0);
+}
+
+static PyObject *doki_funmatrix_dagger(PyObject *self, PyObject *args)
+{
+	(void)self;
+	return fm_flipped(args, 1);
+}
+
+static PyObject *doki_funmatrix_projection(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	unsigned int qubit;
+	int value, verbose;
+	fm_t *m, *f;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OIpp", &cap, &qubit, &value, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_projection(funmatrix, "
+					   "qubit_id, value, verbose)");
+		return NULL;
+	}
+	m = fm_of(cap);
+	if (!m) {
+		PyErr_SetString(DokiError, "[PROJ] The matrix is NULL");
+		return NULL;
+	}
+	f = fm_new(FM_PROJECTION, m->r, m->c);
+	if (!f)
+		return NULL;
+	f->a_cap = cap;
+	Py_INCREF(cap);
+	f->qubit = qubit;
+	f->value = value != 0;
+	f->mem = REF_FMATRIX_BYTES + REF_PROJECTION_BYTES + m->mem;
+	return fm_wrap(f);
+}
+
+/* floor(log2(x)) for x >= 1 */
+static unsigned fm_log2(int64_t x)
+{
+	unsigned k = 0;
+	while (x > 1) {
+		x >>= 1;
+		k++;
+	}
+	return k;
+}
+
+static PyObject *doki_funmatrix_apply(PyObject *self, PyObject *args)
+{
+	PyObject *scap, *gcap, *tlist, *cset, *aset;
+	int verbose;
+	fm_t *st, *g, *f;
+	unsigned nq, nqg, nt, nc = 0, na = 0, i;
+	fm_buf_t *b;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "OOOOOp", &scap, &gcap, &tlist, &cset, &aset,
+			      &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: funmatrix_apply(registry, gate, "
+					   "target_list, control_set, anticontrol_set, "
+					   "verbose)");
+		return NULL;
+	}
+	st = fm_of(scap);
+	if (!st) {
+		PyErr_SetString(DokiError, "NULL pointer to registry");
+		return NULL;
+	}
+	g = fm_of(gcap);
+	if (!g) {
+		PyErr_SetString(DokiError, "NULL pointer to gate");
+		return NULL;
+	}
+	if (st->c > 1) {
+		PyErr_SetString(DokiError, "registry is not a column vector");
+		return NULL;
+	}
+	if (g->c != g->r) {
+		PyErr_SetString(DokiError, "gates have to be square matrices");
+		return NULL;
+	}
+	if (g->r > st->r) {
+		PyErr_SetString(DokiError, "gate is too big for this state vector");
+		return NULL;
+	}
+	nq = fm_log2(st->r);
+	if (st->r != (int64_t)1 << nq) {
+		PyErr_SetString(DokiError, "registry needs 2^n rows");
+		return NULL;
+	}
+	nqg = fm_log2(g->r);
+	if (g->r != (int64_t)1 << nqg) {
+		PyErr_SetString(DokiError, "gates need 2^n x 2^n elements");
+		return NULL;
+	}
+	if (!PyList_Check(tlist)) {
+		PyErr_SetString(DokiError, "target_list must be a list");
+		return NULL;
+	}
+	nt = (unsigned)PyList_Size(tlist);
+	if (nt != nqg) {
+		PyErr_SetString(DokiError, "Wrong number of targets specified for that gate");
+		return NULL;
+	}
+	if (PySet_Check(cset))
+		nc = (unsigned)PySet_Size(cset);
+	else if (cset != Py_None) {
+		PyErr_SetString(DokiError, "control_set must be a set or None");
+		return NULL;
+	}
+	if (PySet_Check(aset))
+		na = (unsigned)PySet_Size(aset);
+	else if (aset != Py_None) {
+		PyErr_SetString(DokiError, "anticontrol_set must be a set or None");
+		return NULL;
+	}
+	b = (fm_buf_t *)PyMem_Calloc(1, sizeof(fm_buf_t));
+	if (!b)
+		return PyErr_NoMemory();
+	b->refs = 1;
+	b->nt = nt;
+	b->nc = nc;
+	b->na = na;
+	b->ids = (unsigned *)PyMem_Calloc((size_t)nt + nc + na + 1, sizeof(unsigned));
+	if (!b->ids) {
+		fm_buf_release(b);
+		return PyErr_NoMemory();
+	}
+	if (nc) {
+		unsigned *ct = b->ids + nt;
+		if (pop_qubits(cset, nc, ct,
+			       "control_set must be a set qubit ids (unsigned integers)"))
+			goto fail;
+		for (i = 0; i < nc; i++)
+			if (ct[i] >= nq) {
+				PyErr_SetString(DokiError, "Control qubit out of range");
+				goto fail;
+			}
+	}
+	if (na) {
+		/* in the reference's order: type, then "control and
+		 * anticontrol" (a None control set counts as containing
+		 * everything, doki.c:2370), then range */
+		PyObject *aux = PySet_New(aset);
+		unsigned *ac = b->ids + nt + nc;
+		if (!aux)
+			goto fail;
+		for (i = 0; i < na; i++) {
+			PyObject *v = PySet_Pop(aux);
+			if (!v || !PyLong_Check(v)) {
+				Py_XDECREF(v);
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError, "anticontrol_set must be a set "
+							   "qubit ids (unsigned integers)");
+				goto fail;
+			}
+			if (cset == Py_None || PySet_Contains(cset, v)) {
+				Py_DECREF(v);
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError,
+						"A control cannot also be an anticontrol");
+				goto fail;
+			}
+			ac[i] = (unsigned)PyLong_AsLong(v);
+			Py_DECREF(v);
+			if (ac[i] >= nq) {
+				Py_DECREF(aux);
+				PyErr_SetString(DokiError, "Anticontrol qubit out of range");
+				goto fail;
+			}
+		}
+		Py_DECREF(aux);
+	}
+	for (i = 0; i < nt; i++) {
+		PyObject *v = PyList_GetItem(tlist, i);
+		if (!PyLong_Check(v)) {
+			PyErr_SetString(DokiError, "target_list must be a list of qubit "
+						   "ids (unsigned integers)");
+			goto fail;
+		}
+		if ((nc > 0 && PySet_Contains(cset, v)) ||
+		    (na > 0 && PySet_Contains(aset, v))) {
+			PyErr_SetString(DokiError,
+					"A target cannot also be a control or an anticontrol");
+			goto fail;
+		}
+		b->ids[i] = (unsigned)PyLong_AsLong(v);
+		if (b->ids[i] >= nq) {
+			PyErr_SetString(DokiError, "Target qubit out of range");
+			goto fail;
+		}
+	}
+	f = fm_new(FM_APPLY, st->r, 1);
+	if (!f)
+		goto fail;
+	f->a_cap = scap;
+	f->b_cap = gcap;
+	Py_INCREF(scap);
+	Py_INCREF(gcap);
+	f->buf = b;
+	f->mem = REF_FMATRIX_BYTES + REF_APPLICATION_BYTES + st->mem + g->mem +
+		 (size_t)(nt + nc + na) * sizeof(unsigned);
+	return fm_wrap(f);
+fail:
+	fm_buf_release(b);
+	return NULL;
+}
+
 
 /* ------------------------------------------------------------------------ */
 

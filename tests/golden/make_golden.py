@@ -18,8 +18,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
+sys.path.insert(0, os.path.dirname(HERE))
 from oracle import oracle  # noqa: E402
 from doki_amd import workloads as W  # noqa: E402
+import fm_exprs  # noqa: E402
 
 ref = oracle.load_reference()
 if ref is None:
@@ -279,12 +281,40 @@ def errors_fixture():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def funmatrix_fixture(seeds=(0, 1, 2)):
+    """The lazy matrix algebra: every chain of fm_exprs.build on seeded
+    inputs (shape, mem, all elements, trace), and the edge/error cases.
+    projection and apply are left out: the reference's compiled parsers for
+    them write an int through a 1-byte bool ("p" format,
+    src/doki/doki.c:2020-2023 and 2226-2228), which clobbers a neighbouring
+    argument in this build."""
+    arrays = {}
+    meta = {"seeds": list(seeds), "chains": {}}
+    for seed in seeds:
+        inp = fm_exprs.random_inputs(seed)
+        for name, fm in fm_exprs.build(ref, inp).items():
+            shape, mem, vals, tr = fm_exprs.snapshot(ref, fm)
+            arrays[f"s{seed}_{name}"] = vals
+            meta["chains"][f"s{seed}_{name}"] = {"shape": list(shape),
+                                                 "mem": mem, "trace": tr}
+    meta["errors"] = fm_exprs.run_error_cases(ref)
+    np.savez_compressed(os.path.join(HERE, "funmatrix.npz"), **arrays)
+    with open(os.path.join(HERE, "funmatrix.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+
+
+FIXTURES = {
+    "c0": lambda: ([c0_fixture(s) for s in (0, 1, 2)],
+                   c0_fixture(7, n=6, depth=60)),
+    "multi": multi_fixture,
+    "measure": measure_fixture,
+    "misc": misc_fixture,
+    "errors": errors_fixture,
+    "funmatrix": funmatrix_fixture,
+}
+
 if __name__ == "__main__":
-    for s in (0, 1, 2):
-        c0_fixture(s)
-    c0_fixture(7, n=6, depth=60)
-    multi_fixture()
-    measure_fixture()
-    misc_fixture()
-    errors_fixture()
+    # python tests/golden/make_golden.py [fixture ...]   (default: all)
+    for name in sys.argv[1:] or list(FIXTURES):
+        FIXTURES[name]()
     print("golden fixtures written to", HERE)

@@ -123,6 +123,60 @@ def test_misc_golden(doki):
         assert doki.registry_mem(doki.registry_new(int(n), False), False) == b
 
 
+def test_density_algebra_chains(doki):
+    """The lazy operators over a density-matrix leaf read from the GPU:
+    against numpy on the reference's density of the same 5-qubit state, and
+    projections against registry_prob (trace of the projected density)."""
+    z = load("misc.npz")
+    reg = doki_run(doki, doki.registry_new(5, False), unpack_steps(z, "dens_"))
+    rho = doki.registry_density(reg, False)
+    R = z["dens_full"]
+
+    def F(name, *a):
+        return getattr(doki, "funmatrix_" + name)(*a, False)
+
+    def dense(fm):
+        r, c = F("shape", fm)
+        return np.array([[F("get", fm, i, k) for k in range(c)]
+                         for i in range(r)])
+
+    h = np.array([[1, 1], [1, -1]]) / math.sqrt(2)
+    for q in range(5):
+        p = doki.registry_prob(reg, q, 1, False)
+        t1 = F("trace", F("projection", rho, q, True))
+        t0 = F("trace", F("projection", rho, q, False))
+        assert abs(t1 - p) <= ATOL and abs(t0 - (1 - p)) <= ATOL
+        keep = ((np.arange(32) >> q) & 1) == 1
+        assert maxabs(dense(F("projection", rho, q, True)),
+                      np.where(keep[:, None], R, 0)) <= ATOL
+    assert maxabs(dense(F("matmul", rho, rho)), R) <= ATOL   # pure state
+    assert maxabs(dense(F("dagger", rho)), R) <= ATOL         # Hermitian
+    # partial traces as in the reference's chain [2, 3] (golden)
+    ci = json.load(open(f"{GOLDEN}/misc.json"))["dens_chains"].index([2, 3])
+    pt = F("partialtrace", F("partialtrace", rho, 2), 3)
+    want_pt = z[f"dens_chain{ci}"]
+    assert maxabs(dense(pt), want_pt) <= ATOL
+    kr = F("kron", pt, F("hadamard", 1))
+    assert maxabs(dense(kr), np.kron(want_pt, h)) <= ATOL
+    comb = F("sub", F("scalar_mul", rho, 2.5 - 1j),
+             F("ewmul", rho, F("densityzero", 5)))
+    dz = np.zeros((32, 32))
+    dz[0, 0] = 1
+    assert maxabs(dense(comb), (2.5 - 1j) * R - R * dz) <= ATOL
+    # a gate applied to the first column of rho (psi * conj(psi_0))
+    col = F("matmul", rho, F("statezero", 5))
+    app = F("apply", col, F("hadamard", 2), [3, 1], {0}, {4})
+    U = np.eye(32, dtype=complex)
+    for i in range(32):
+        if (i & 1) and not (i >> 4) & 1:
+            U[i] = 0
+            row = ((i >> 3) & 1) | (((i >> 1) & 1) << 1)
+            for m in range(4):
+                j = (i & ~0b1010) | ((m & 1) << 3) | (((m >> 1) & 1) << 1)
+                U[i, j] = np.kron(h, h)[row, m]
+    assert maxabs(dense(app)[:, 0], U @ R[:, 0]) <= ATOL
+
+
 ERRORS = json.load(open(f"{GOLDEN}/errors.json"))
 
 
