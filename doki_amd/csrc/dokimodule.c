@@ -1096,22 +1096,19 @@ static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 	}
 }
 
-/* Element of another matrix read by a leaf: NaN when that read fails (the
- * leaf then reports code 8); -1 on a Python error. */
+/* Element of another matrix read by a leaf: the code fm_get returns (0,
+ * a positive getitem code, or -1 on a Python error) with *out as fm_get
+ * left it -- NaN for an index out of bounds, the partial sum of a failed
+ * matrix product -- because the reference's leaf functions go on with
+ * whatever value their nested getitem produced. */
 static int fm_read(PyObject *cap, int64_t i, int64_t j, cplx *out)
 {
 	fm_t *m = fm_of(cap);
-	int rc;
 	if (!m) {
 		*out = NAN;
-		return 0;
+		return 8;
 	}
-	rc = fm_get(m, i, j, out);
-	if (rc < 0)
-		return -1;
-	if (rc)
-		*out = NAN;
-	return 0;
+	return fm_get(m, i, j, out);
 }
 
 /* Value of a leaf at (i, j), indices already swapped for transposition.
@@ -1125,11 +1122,22 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 	case FM_DENSITY:
 		return fm_density_get(f, i, j, out);
 	case FM_PTRACE: {
-		cplx a, b = 0;
-		if (fm_read(f->a_cap, ins_bit(0, i, f->e), ins_bit(0, j, f->e), &a) ||
-		    fm_read(f->a_cap, ins_bit(1, i, f->e), ins_bit(1, j, f->e), &b))
+		/* _PartialTFunct: the sum of both reads when both succeed; the
+		 * first value alone when only the second fails (its index lies
+		 * past the matrix); whatever the failed first read left
+		 * otherwise */
+		cplx a, b;
+		int r1 = fm_read(f->a_cap, ins_bit(0, i, f->e), ins_bit(0, j, f->e), &a), r2;
+		if (r1 < 0)
 			return -1;
-		*out = a + b;
+		*out = a;
+		if (r1)
+			return 0;
+		r2 = fm_read(f->a_cap, ins_bit(1, i, f->e), ins_bit(1, j, f->e), &b);
+		if (r2 < 0)
+			return -1;
+		if (r2 == 0)
+			*out = a + b;
 		return 0;
 	}
 	case FM_CUSTOM: {
@@ -1167,7 +1175,7 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 			*out = (double)(i == j) + I * 0.0;
 			return 0;
 		}
-		return fm_read(f->a_cap, i - u->r, j - u->c, out);
+		return fm_read(f->a_cap, i - u->r, j - u->c, out) < 0 ? -1 : 0;
 	}
 	case FM_EYEKRON: {
 		fm_t *u = fm_of(f->a_cap);
@@ -1185,7 +1193,7 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 			*out = 0;
 			return 0;
 		}
-		return fm_read(f->a_cap, i % u->r, j % u->c, out);
+		return fm_read(f->a_cap, i % u->r, j % u->c, out) < 0 ? -1 : 0;
 	}
 	case FM_PROJECTION: {
 		const int set = f->qubit < 63 && ((i >> f->qubit) & 1);
@@ -1193,7 +1201,7 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 			*out = 0;
 			return 0;
 		}
-		return fm_read(f->a_cap, i, j, out);
+		return fm_read(f->a_cap, i, j, out) < 0 ? -1 : 0;
 	}
 	case FM_APPLY: {
 		const fm_buf_t *b = f->buf;
@@ -1203,13 +1211,15 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 		int64_t n, reg = i;
 		unsigned k;
 		cplx val = 0, x, y;
+		int r1, r2;
 		(void)j;
+		/* a failed read makes the element NaN (_ApplyGateFunction) */
 		for (k = 0; k < b->nc; k++)
 			if (!((i >> ct[k]) & 1))
-				return fm_read(f->a_cap, i, 0, out);
+				goto passthrough;
 		for (k = 0; k < b->na; k++)
 			if ((i >> ac[k]) & 1)
-				return fm_read(f->a_cap, i, 0, out);
+				goto passthrough;
 		if (!g) {
 			*out = NAN;
 			return 0;
@@ -1223,16 +1233,24 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 				else
 					reg &= ~((int64_t)1 << tg[k]);
 			}
-			if (fm_read(f->a_cap, reg, 0, &x) || fm_read(f->b_cap, row, n, &y))
+			r1 = fm_read(f->a_cap, reg, 0, &x);
+			r2 = r1 ? 0 : fm_read(f->b_cap, row, n, &y);
+			if (r1 < 0 || r2 < 0)
 				return -1;
-			if (isnan(creal(x)) || isnan(cimag(x)) || isnan(creal(y)) ||
-			    isnan(cimag(y))) {
+			if (r1 || r2) {
 				*out = NAN;
 				return 0;
 			}
 			val = val + x * y;
 		}
 		*out = val;
+		return 0;
+	passthrough:
+		r1 = fm_read(f->a_cap, i, 0, out);
+		if (r1 < 0)
+			return -1;
+		if (r1)
+			*out = NAN;
 		return 0;
 	}
 	default:

@@ -97,7 +97,28 @@ def error_cases(mod):
     M2 = F("create", [[1, 2], [3, 4]])
     M4 = F("create", [[1, 2, 3, 4]] * 4)
     R = F("create", [[1, 2, 3, 4]])
+    # a product whose element (0, 0) fails at k = 1 (NaN factor) after a
+    # partial sum: leaves above it carry that partial value on
+    N = F("create", [[1, 2], [float("nan"), 4]])
+    MN = F("matmul", F("scalar_mul", M2, 1.5 - 0.5j), N)
+
+    def dense(fm):
+        r, c = F("shape", fm)
+        return [[cx(F("get", fm, i, j)) for j in range(c)] for i in range(r)]
+
     return {
+        "mm_fail": lambda: cx(F("get", MN, 0, 0)),
+        "mm_fail_trace": lambda: cx(F("trace", MN)),
+        "pt_partial": lambda: cx(F("get", F("partialtrace", MN, 0), 0, 0)),
+        "cu_partial": lambda: cx(F("get", F("addcontrol", MN), 2, 2)),
+        "cu_partial_ok": lambda: cx(F("get", F("addcontrol", MN), 2, 3)),
+        "ek_partial": lambda: cx(F("get", F("eyekron", MN, 1, 1), 0, 0)),
+        "pt_oob_values": lambda: dense(F("partialtrace", M4, 2)),
+        "pt_oob_T_values": lambda: dense(F("partialtrace",
+                                           F("transpose", M4), 3)),
+        "pt_oob_trace": lambda: cx(F("trace", F("partialtrace", M4, 2))),
+        "pt_nan_first": lambda: cx(F("get", F("partialtrace", N, 0), 0, 0)),
+        "kron_nan": lambda: cx(F("get", F("kron", M2, N), 2, 0)),
         "get_row_oob": lambda: cx(F("get", M2, 2, 0)),
         "get_col_oob": lambda: cx(F("get", M2, 0, 2)),
         "get_T_oob": lambda: cx(F("get", F("transpose", R), 3, 0)),
