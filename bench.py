@@ -80,6 +80,16 @@ def gpu_leg(args):
     kms = capi.time_ring_read()
     assert len(kms) == ngates, (len(kms), ngates)
     log(f"[bench] timed {ngates} gates in {elapsed:.3f} s")
+    # the same durations by gate kind and target
+    by_target = {}
+    gi = 0
+    for li in range(args.warmup, args.warmup + args.steps):
+        for g, t, c in layers[li]:
+            key = ("cz" if c else "u3") + str(t[0])
+            by_target.setdefault(key, []).append(kms[gi])
+            gi += 1
+    by_target = {k: round(bpg / (sum(v) / len(v) * 1e-3) / 1e9)
+                 for k, v in sorted(by_target.items(), key=lambda kv: (kv[0][:2], int(kv[0][2:])))}
 
     # per-target kernel durations: five back-to-back gates per target
     per_target = {}
@@ -115,7 +125,7 @@ def gpu_leg(args):
 
     return {
         "elapsed": elapsed, "ngates": ngates, "bpg": bpg, "kavg_ms": kavg,
-        "kernel_ms_layer": kms, "per_target": per_target,
+        "kernel_GBps_timed_by_target": by_target, "per_target": per_target,
         "prob_all_ms": (tp1 - tp0) * 1e3, "copy_ms": min(copy_ms),
     }
 
@@ -531,6 +541,7 @@ def main():
             "gate_kernel_avg_ms": round(g["kavg_ms"], 4),
             "state_copy_GBps": round(g["bpg"] / (g["copy_ms"] * 1e-3) / 1e9, 1),
             "per_target_kernel": g["per_target"],
+            "timed_region_kernel_GBps_by_gate": g["kernel_GBps_timed_by_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "large_register": big,
             "fused": fused,
