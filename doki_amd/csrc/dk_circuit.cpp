@@ -472,11 +472,30 @@ struct PassBuilder {
 		last_am = g.am;
 	}
 
-	void emit_exchange(const unsigned (&nr)[dkp::REG_BITS])
+	// New mapping with register slots `want` (any order).  A slot that
+	// stays in registers keeps its register bit: with 4 of 6 slots in
+	// registers before and after, at least two do, and the kernel splits
+	// the exchange into two halves on one of them.
+	void emit_exchange(const unsigned (&want)[dkp::REG_BITS])
 	{
-		bool used[dkp::HIGH_BITS] = {false};
+		bool used[dkp::HIGH_BITS] = {false}, placed[dkp::HIGH_BITS] = {false};
 		for (int j = 0; j < dkp::REG_BITS; ++j)
-			used[nr[j]] = true;
+			used[want[j]] = true;
+		unsigned nr[dkp::REG_BITS];
+		bool filled[dkp::REG_BITS] = {false};
+		for (int j = 0; j < dkp::REG_BITS; ++j)
+			if (used[rsl[j]]) {
+				nr[j] = rsl[j];
+				filled[j] = placed[rsl[j]] = true;
+			}
+		for (int j = 0, k = 0; j < dkp::REG_BITS; ++j) {
+			if (filled[j])
+				continue;
+			while (placed[want[k]])
+				++k;
+			nr[j] = want[k];
+			placed[want[k]] = true;
+		}
 		unsigned nw[dkp::WAVE_BITS], k = 0;
 		for (unsigned h = 0; h < (unsigned)dkp::HIGH_BITS; ++h)
 			if (!used[h])
@@ -502,16 +521,24 @@ struct PassBuilder {
 // range.  The kernel indexes LDS and registers by these fields.
 bool pass_ops_valid(const PassOp *ops, size_t n, unsigned nq)
 {
+	unsigned rs[dkp::REG_BITS]; // register slots of the current mapping
+	for (int j = 0; j < dkp::REG_BITS; ++j)
+		rs[j] = (unsigned)j;
 	for (size_t i = 0; i < n; ++i) {
 		const PassOp &o = ops[i];
 		switch (o.kind) {
 		case dkp::OP_EXCH: {
 			unsigned seen = 0;
-			for (int j = 0; j < dkp::REG_BITS; ++j)
-				seen |= 1u << ((o.a >> (3 * j)) & 7);
+			bool kept = false; // a register bit keeps its slot
+			for (int j = 0; j < dkp::REG_BITS; ++j) {
+				const unsigned s = (o.a >> (3 * j)) & 7;
+				seen |= 1u << s;
+				kept |= s == rs[j];
+				rs[j] = s;
+			}
 			for (int b = 0; b < dkp::WAVE_BITS; ++b)
 				seen |= 1u << ((o.b >> (3 * b)) & 7);
-			if (seen != (1u << dkp::HIGH_BITS) - 1 ||
+			if (seen != (1u << dkp::HIGH_BITS) - 1 || !kept ||
 			    (o.a >> (3 * dkp::REG_BITS)) || (o.b >> (3 * dkp::WAVE_BITS)))
 				return false;
 			continue;

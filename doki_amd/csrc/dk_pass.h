@@ -11,7 +11,9 @@
 // amplitudes.  Of the six high slot bits, four are register bits (the 16
 // amplitudes of a thread differ in them) and two are wave bits; which is
 // which is the pass's current mapping.  An exchange op moves the tile
-// through LDS into another mapping.  Gates act
+// through LDS into another mapping; one register bit keeps its slot across
+// it, so it runs as two rounds over half the tile each (a half-tile LDS
+// buffer per workgroup).  Gates act
 //   - on a register bit: pairs of a thread's own registers;
 //   - on a lane bit: the partner amplitude comes from lane ^ 2^b;
 //   - diagonally on any qubit: a per-amplitude factor, no partner at all.

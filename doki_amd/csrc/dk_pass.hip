@@ -301,6 +301,12 @@ __device__ __forceinline__ unsigned wave_slots(unsigned w, const unsigned (&ws)[
 	return s;
 }
 
+// A 6-bit slot set without slot bit h (higher slot bits shift down).
+__device__ __forceinline__ unsigned squeeze(unsigned slots, unsigned h)
+{
+	return (slots & ((1u << h) - 1)) | ((slots >> (h + 1)) << h);
+}
+
 // Index bits of the high slot bits in `slots`.
 __device__ __forceinline__ u64 slot_index(unsigned slots, const u64 (&hb)[HIGH_BITS])
 {
@@ -312,6 +318,16 @@ __device__ __forceinline__ u64 slot_index(unsigned slots, const u64 (&hb)[HIGH_B
 	return x;
 }
 
+// Exchange buffer.  complex128: half a tile (32 KiB; exchanges run in two
+// rounds), so that LDS admits the three workgroups per CU the kernel's
+// registers allow; complex64: the whole tile (32 KiB; one round), since its
+// registers already cap the CU at four workgroups.
+template <typename T> constexpr bool exch_halves() { return sizeof(T) == 16; }
+template <typename T> constexpr int exch_entries()
+{
+	return 1 << (PASS_TILE_BITS - (exch_halves<T>() ? 1 : 0));
+}
+
 template <typename T>
 __device__ __forceinline__ void pass_tile(
 	const T *in, T *out, typename real_of<T>::type inv, const PassDesc &pd,
@@ -319,6 +335,7 @@ __device__ __forceinline__ void pass_tile(
 	T *lds, double *red)
 {
 	typedef typename real_of<T>::type R;
+	constexpr bool HALVES = exch_halves<T>();
 	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	u64 base = tile << LANE_BITS;
 #pragma unroll
@@ -352,31 +369,66 @@ __device__ __forceinline__ void pass_tile(
 		const unsigned kind = op->kind, a = op->a, rmask = op->rmask;
 		const u64 tcm = op->tcm, tam = op->tam;
 		if (kind == OP_EXCH) {
-			__syncthreads(); // the previous exchange's reads are done
-			{
-				const unsigned wsl = wave_slots(w, ws);
-#pragma unroll
-				for (int r = 0; r < REGS; ++r)
-					lds[lane | ((wsl | reg_slots(r, rs)) << LANE_BITS)] =
-						mk<T>(vr[r], vi[r]);
-			}
 			const unsigned b = op->b;
+			unsigned nrs[REG_BITS], nws[WAVE_BITS];
 #pragma unroll
 			for (int j = 0; j < REG_BITS; ++j)
-				rs[j] = (a >> (3 * j)) & 7;
+				nrs[j] = (a >> (3 * j)) & 7;
 #pragma unroll
 			for (int k = 0; k < WAVE_BITS; ++k)
-				ws[k] = (b >> (3 * k)) & 7;
-			__syncthreads();
-			{
-				const unsigned wsl = wave_slots(w, ws);
+				nws[k] = (b >> (3 * k)) & 7;
+			const unsigned wsl0 = wave_slots(w, ws), wsl1 = wave_slots(w, nws);
+			if constexpr (HALVES) {
+				// Register bit js holds the same high slot hs before
+				// and after (the planner guarantees one): the
+				// amplitudes with slot hs = 0 and those with hs = 1
+				// are exchanged in two rounds, each by the 8
+				// registers whose bit js is that value, before and
+				// after.
+				unsigned js = 0;
+#pragma unroll
+				for (int j = REG_BITS - 1; j >= 0; --j)
+					if (rs[j] == nrs[j])
+						js = (unsigned)j;
+				const unsigned hs = rs[js];
+#pragma unroll
+				for (unsigned half = 0; half < 2; ++half) {
+					__syncthreads(); // earlier reads of the buffer are done
+#pragma unroll
+					for (int r = 0; r < REGS; ++r)
+						if (((r >> js) & 1) == half)
+							lds[lane | (squeeze(wsl0 | reg_slots(r, rs), hs)
+								    << LANE_BITS)] = mk<T>(vr[r], vi[r]);
+					__syncthreads();
+#pragma unroll
+					for (int r = 0; r < REGS; ++r)
+						if (((r >> js) & 1) == half) {
+							const T x = lds[lane | (squeeze(wsl1 | reg_slots(r, nrs), hs)
+										<< LANE_BITS)];
+							vr[r] = x.x;
+							vi[r] = x.y;
+						}
+				}
+			} else {
+				__syncthreads(); // the previous exchange's reads are done
+#pragma unroll
+				for (int r = 0; r < REGS; ++r)
+					lds[lane | ((wsl0 | reg_slots(r, rs)) << LANE_BITS)] =
+						mk<T>(vr[r], vi[r]);
+				__syncthreads();
 #pragma unroll
 				for (int r = 0; r < REGS; ++r) {
-					const T x = lds[lane | ((wsl | reg_slots(r, rs)) << LANE_BITS)];
+					const T x = lds[lane | ((wsl1 | reg_slots(r, nrs)) << LANE_BITS)];
 					vr[r] = x.x;
 					vi[r] = x.y;
 				}
 			}
+#pragma unroll
+			for (int j = 0; j < REG_BITS; ++j)
+				rs[j] = nrs[j];
+#pragma unroll
+			for (int k = 0; k < WAVE_BITS; ++k)
+				ws[k] = nws[k];
 			tb = base | lane | slot_index(wave_slots(w, ws), hb);
 			continue;
 		}
@@ -420,7 +472,7 @@ __global__ __launch_bounds__(THREADS) void k_pass(
 	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 ntiles)
 {
 	typedef typename real_of<T>::type R;
-	__shared__ T lds[1 << PASS_TILE_BITS];
+	__shared__ T lds[exch_entries<T>()];
 	__shared__ double red[THREADS / 64];
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	if constexpr (LOOP) {

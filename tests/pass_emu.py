@@ -31,8 +31,9 @@ def _tile_index(n, tile_qubits):
 def run_pass(psi, n, tile_qubits, ops):
     """One pass on the normalised state psi (complex128); returns the new
     (unnormalised) state.  Asserts what the kernel relies on: exchanges are
-    permutations of the high slots, and no thread-word field (tcm, tam, a
-    diagonal op's qubit) names a qubit held in registers."""
+    permutations of the high slots under which a register bit keeps its
+    slot, and no thread-word field (tcm, tam, a diagonal op's qubit) names a
+    qubit held in registers."""
     idx = _tile_index(n, tile_qubits)
     A = psi[idx].copy()
     hpos = tile_qubits[LANE_BITS:]
@@ -47,9 +48,15 @@ def run_pass(psi, n, tile_qubits, ops):
     for op in ops:
         kind = int(op["kind"])
         if kind == OP_EXCH:
-            rs = [(int(op["a"]) >> (3 * j)) & 7 for j in range(REG_BITS)]
+            new_rs = [(int(op["a"]) >> (3 * j)) & 7 for j in range(REG_BITS)]
             ws = [(int(op["b"]) >> (3 * b)) & 7 for b in range(WAVE_BITS)]
-            assert sorted(rs + ws) == list(range(HIGH_BITS))
+            assert sorted(new_rs + ws) == list(range(HIGH_BITS))
+            # the kernel splits an exchange in two on a register bit that
+            # keeps its slot; every slot kept in registers keeps its bit
+            assert any(new_rs[j] == rs[j] for j in range(REG_BITS))
+            assert all(new_rs[j] == rs[j] for j in range(REG_BITS)
+                       if rs[j] in new_rs)
+            rs = new_rs
             continue
         m = op["m"].view(np.complex128).reshape(2, 2)
         held = sum(1 << reg_qubit(j) for j in range(REG_BITS))
