@@ -76,6 +76,11 @@ __device__ __forceinline__ bool reg_ok(int r, unsigned rmask)
 //     registers whose register-bit part fails are skipped.
 enum { CM_NONE = 0, CM_REG = 1, CM_LANE = 2 };
 
+// Body of a register that a control may skip: the empty volatile asm keeps
+// the uniform skip a branch (if-converted, every register would be computed
+// and the old values selected back).
+#define DK_KEEP_BRANCH() asm volatile("")
+
 // Dense 2x2 gate on register bit J.
 template <int J, int CM, typename R>
 __device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
@@ -101,6 +106,8 @@ __device__ __forceinline__ void gate_reg(R (&vr)[REGS], R (&vi)[REGS],
 		vi[r] = o0i;
 		vr[r | d] = o1r;
 		vi[r | d] = o1i;
+		if (CM != CM_NONE)
+			DK_KEEP_BRANCH();
 	}
 }
 
@@ -184,6 +191,8 @@ __device__ __forceinline__ void gate_lane(R (&vr)[REGS], R (&vi)[REGS],
 		}
 		vr[r] = orr;
 		vi[r] = oi;
+		if (CM != CM_NONE)
+			DK_KEEP_BRANCH();
 	}
 }
 
@@ -221,30 +230,52 @@ __device__ __forceinline__ void gate_lane_any(R (&vr)[REGS], R (&vi)[REGS],
 	}
 }
 
-// Diagonal gate: factor m00 where the qubit is 0, m11 where it is 1.  The
-// qubit is register bit a (REG) or bit a of the thread word.  (The planner
-// sends a phase gate, m00 = 1, as the factor m11 controlled on its own
-// target, so the control modes skip the amplitudes it leaves alone.)
-template <int CM, bool REG, typename R>
-__device__ __forceinline__ void gate_diag(R (&vr)[REGS], R (&vi)[REGS],
-					  const M22<R> &M, bool ok, unsigned rmask,
-					  unsigned a, u64 tb)
+// One diagonal op: a sign change (NEG) or a complex factor per register.
+template <int CM, bool REG, bool NEG, typename R>
+__device__ __forceinline__ void gate_diag_impl(R (&vr)[REGS], R (&vi)[REGS],
+					       const M22<R> &M, bool ok, unsigned rmask,
+					       unsigned a, u64 tb)
 {
 	const Cx<R> ft = cx_sel((tb >> a) & 1, M.m11, M.m00);
 #pragma unroll
 	for (int r = 0; r < REGS; ++r) {
 		if (CM != CM_NONE && !reg_ok(r, rmask))
 			continue;
-		const Cx<R> f = REG ? cx_sel((r >> a) & 1, M.m11, M.m00) : ft;
 		R orr, oi;
-		cx_mul(f, vr[r], vi[r], orr, oi);
+		if (NEG) {
+			orr = -vr[r];
+			oi = -vi[r];
+		} else {
+			const Cx<R> f = REG ? cx_sel((r >> a) & 1, M.m11, M.m00) : ft;
+			cx_mul(f, vr[r], vi[r], orr, oi);
+		}
 		if (CM == CM_LANE) {
 			orr = ok ? orr : vr[r];
 			oi = ok ? oi : vi[r];
 		}
 		vr[r] = orr;
 		vi[r] = oi;
+		if (CM != CM_NONE)
+			DK_KEEP_BRANCH();
 	}
+}
+
+// Diagonal gate: factor m00 where the qubit is 0, m11 where it is 1.  The
+// qubit is register bit a (REG) or bit a of the thread word.  (The planner
+// sends a phase gate, m00 = 1, as the factor m11 controlled on its own
+// target, so the control modes skip the amplitudes it leaves alone; a
+// factor of -1 on both sides, as Z and CZ become, is a sign change.)
+template <int CM, bool REG, typename R>
+__device__ __forceinline__ void gate_diag(R (&vr)[REGS], R (&vi)[REGS],
+					  const M22<R> &M, bool ok, unsigned rmask,
+					  unsigned a, u64 tb)
+{
+	const bool neg = M.m00.re == (R)-1 && M.m00.im == (R)0 && M.m11.re == (R)-1 &&
+			 M.m11.im == (R)0;
+	if (neg)
+		gate_diag_impl<CM, REG, true>(vr, vi, M, ok, rmask, a, tb);
+	else
+		gate_diag_impl<CM, REG, false>(vr, vi, M, ok, rmask, a, tb);
 }
 
 template <typename R>
@@ -328,23 +359,43 @@ template <typename T> constexpr int exch_entries()
 	return 1 << (PASS_TILE_BITS - (exch_halves<T>() ? 1 : 0));
 }
 
-template <typename T>
-__device__ __forceinline__ void pass_tile(
-	const T *in, T *out, typename real_of<T>::type inv, const PassDesc &pd,
-	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 tile,
-	T *lds, double *red)
+// Index bits of a tile's fixed part: the tile number spread over the index
+// bits outside the tile.
+__device__ __forceinline__ u64 tile_base(u64 tile, const PassDesc &pd)
 {
-	typedef typename real_of<T>::type R;
-	constexpr bool HALVES = exch_halves<T>();
-	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	u64 base = tile << LANE_BITS;
 #pragma unroll
 	for (int h = 0; h < HIGH_BITS; ++h)
 		base = ins0(base, pd.hsorted[h]);
-	u64 hb[HIGH_BITS];
+	return base;
+}
+
+// A thread's 16 amplitudes of the tile at `base` in the load mapping
+// (register bit j = high slot j, wave bit b = slot 4 + b), as stored.
+template <typename T, typename R>
+__device__ __forceinline__ void tile_fetch(const T *in, u64 base,
+					   const u64 (&hb)[HIGH_BITS], unsigned lane,
+					   unsigned w, R (&xr)[REGS], R (&xi)[REGS])
+{
+	const u64 tb = base | lane | slot_index(w << REG_BITS, hb);
 #pragma unroll
-	for (int h = 0; h < HIGH_BITS; ++h)
-		hb[h] = 1ull << pd.hpos[h];
+	for (int r = 0; r < REGS; ++r) {
+		const T x = ld_stream(in + (tb | slot_index((unsigned)r, hb)));
+		xr[r] = x.x;
+		xi[r] = x.y;
+	}
+}
+
+// The pass's ops on one tile held in vr/vi (normalised values, load
+// mapping), then the store and the tile's partial |out|^2 sum.
+template <typename T, typename R>
+__device__ __forceinline__ void tile_run(
+	T *out, const PassDesc &pd, const PassOp *__restrict__ ops,
+	double *__restrict__ partials, u64 tile, u64 base,
+	const u64 (&hb)[HIGH_BITS], unsigned lane, unsigned w, R (&vr)[REGS],
+	R (&vi)[REGS], T *lds, double *red)
+{
+	constexpr bool HALVES = exch_halves<T>();
 	// mapping at load: register bit j = high slot j, wave bit b = slot 4 + b
 	unsigned rs[REG_BITS], ws[WAVE_BITS];
 #pragma unroll
@@ -356,13 +407,6 @@ __device__ __forceinline__ void pass_tile(
 
 	// the thread word: index bits shared by all of the thread's registers
 	u64 tb = base | lane | slot_index(wave_slots(w, ws), hb);
-	R vr[REGS], vi[REGS];
-#pragma unroll
-	for (int r = 0; r < REGS; ++r) {
-		const T x = ld_stream(in + (tb | slot_index(reg_slots(r, rs), hb)));
-		vr[r] = x.x * inv;
-		vi[r] = x.y * inv;
-	}
 
 	for (unsigned i = 0; i < pd.nops; ++i) {
 		const PassOp *__restrict__ op = ops + i;
@@ -463,6 +507,30 @@ __device__ __forceinline__ void pass_tile(
 			t += red[ww];
 		partials[tile] = t;
 	}
+}
+
+// One tile, loaded on the spot.
+template <typename T>
+__device__ __forceinline__ void pass_tile(
+	const T *in, T *out, typename real_of<T>::type inv, const PassDesc &pd,
+	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 tile,
+	T *lds, double *red)
+{
+	typedef typename real_of<T>::type R;
+	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	u64 hb[HIGH_BITS];
+#pragma unroll
+	for (int h = 0; h < HIGH_BITS; ++h)
+		hb[h] = 1ull << pd.hpos[h];
+	const u64 base = tile_base(tile, pd);
+	R vr[REGS], vi[REGS];
+	tile_fetch(in, base, hb, lane, w, vr, vi);
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		vr[r] *= inv;
+		vi[r] *= inv;
+	}
+	tile_run<T>(out, pd, ops, partials, tile, base, hb, lane, w, vr, vi, lds, red);
 }
 
 // LOOP: tiles beyond the grid (launches of more than 2^31 threads).
