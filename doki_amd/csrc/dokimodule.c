@@ -401,6 +401,59 @@ static PyObject *doki_registry_get(PyObject *self, PyObject *args)
 	return PyComplex_FromDoubles(creal(v), cimag(v));
 }
 
+/* registry_get_all(registry, canonical, verbose): every amplitude as a 1-D
+ * numpy complex128 array whose element i equals registry_get(registry, i,
+ * canonical, verbose) bit for bit (same division by the norm constant, same
+ * canonical rotation), read with one device-to-host copy.  The reference
+ * has no such entry; its callers build the vector with one registry_get
+ * call per amplitude (tests/reg_creation_tests.py:24-28). */
+static PyObject *doki_registry_get_all(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	PyArrayObject *arr;
+	int canonical, verbose, rc;
+	reg_t *r;
+	npy_intp len;
+	double *v;
+	cplx rot = 1.0;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Opp", &cap, &canonical, &verbose)) {
+		PyErr_SetString(DokiError,
+				"Syntax: registry_get_all(registry, canonical, verbose)");
+		return NULL;
+	}
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	if (reg_norm(r))
+		return NULL;
+	if (canonical) {
+		double ph;
+		if (reg_phase(r, &ph))
+			return NULL;
+		rot = cos(ph) + I * -sin(ph);
+	}
+	len = (npy_intp)1 << r->nq;
+	arr = (PyArrayObject *)PyArray_SimpleNew(1, &len, NPY_CDOUBLE);
+	if (!arr)
+		return NULL;
+	v = (double *)PyArray_DATA(arr);
+	rc = dk_read_raw(r->st, 0, (uint64_t)len, v, NULL);
+	if (rc) {
+		Py_DECREF(arr);
+		dk_fail(rc, "Failed to read state");
+		return NULL;
+	}
+	for (npy_intp i = 0; i < len; i++) {
+		cplx x = (v[2 * i] + I * v[2 * i + 1]) / r->norm;
+		if (canonical)
+			x = x * rot;
+		v[2 * i] = creal(x);
+		v[2 * i + 1] = cimag(x);
+	}
+	return (PyObject *)arr;
+}
+
 static PyObject *doki_registry_mem(PyObject *self, PyObject *args)
 {
 	PyObject *cap;
@@ -2101,6 +2154,8 @@ static PyMethodDef DokiMethods[] = {
 	{"registry_clone", doki_registry_clone, METH_VARARGS, "Clone a registry"},
 	{"registry_del", doki_registry_del, METH_VARARGS, "Destroy a registry"},
 	{"registry_get", doki_registry_get, METH_VARARGS, "Get value from registry"},
+	{"registry_get_all", doki_registry_get_all, METH_VARARGS,
+	 "Get every value of a registry as a numpy array (one copy)"},
 	{"registry_apply", doki_registry_apply, METH_VARARGS, "Apply a gate"},
 	{"registry_join", doki_registry_join, METH_VARARGS, "Merges two registries"},
 	{"registry_measure", doki_registry_measure, METH_VARARGS,

@@ -112,3 +112,13 @@ def test_no_device_fails_loudly(built):
     d = doki_amd.doki
     with pytest.raises(d.error):
         d.registry_new(3, False)
+
+
+def test_bulk_read_entry_validates_on_host(built):
+    """registry_get_all (bulk form of registry_get, not in the reference's
+    table) checks its arguments before any device work."""
+    d = doki_amd.doki
+    assert callable(d.registry_get_all)
+    assert _err(lambda: d.registry_get_all(5, False, False)) == "NULL pointer to registry"
+    assert _err(lambda: d.registry_get_all(5, False)) == (
+        "Syntax: registry_get_all(registry, canonical, verbose)")

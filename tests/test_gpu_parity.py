@@ -177,6 +177,34 @@ def test_density_algebra_chains(doki):
     assert maxabs(dense(app)[:, 0], U @ R[:, 0]) <= ATOL
 
 
+def _bits(a):
+    return np.asarray(a, dtype=np.complex128).view(np.float64).tobytes()
+
+
+def test_registry_get_all_equals_get(doki):
+    """The bulk read equals registry_get element by element, bit for bit:
+    circuit states, unnormalised input, collapsed states (norm constant
+    != 1), with and without the canonical phase; and the reference's
+    canonical-phase vectors."""
+    z = load("c0_n6_seed7.npz")
+    regs = [doki_run(doki, doki.registry_new(6, False), unpack_steps(z))]
+    m = load("misc.npz")
+    regs.append(doki.registry_new_data(4, m["unnorm_in"], False))
+    st, _ = doki.registry_measure(regs[0], 0b100101, [0.3, 0.8, 0.1], 1, False)
+    regs.append(st)
+    regs.append(doki.registry_new(9, False))
+    for reg in regs:
+        n = int(np.log2(len(doki.registry_get_all(reg, False, False))))
+        for canonical in (False, True):
+            want = [doki.registry_get(reg, i, canonical, False) for i in range(1 << n)]
+            got = doki.registry_get_all(reg, canonical, False)
+            assert got.dtype == np.complex128 and got.shape == (1 << n,)
+            assert _bits(got) == _bits(want)
+    for i in range(5):
+        reg = doki.registry_new_data(2, m[f"canon_in{i}"], False)
+        assert maxabs(doki.registry_get_all(reg, True, False), m[f"canon_out{i}"]) <= ATOL
+
+
 ERRORS = json.load(open(f"{GOLDEN}/errors.json"))
 
 
