@@ -296,7 +296,9 @@ PASS_OP = np.dtype([
     ("tcm", "<u8"), ("tam", "<u8"), ("m", "<f8", (8,)),
 ])
 assert PASS_OP.itemsize == 96
-OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH = range(5)
+(OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH, OP_REG_MUX, OP_LANE_MUX,
+ OP_MDATA) = range(8)
+SEL_REG = 0x100   # mux selector: register bit (low bits), else a thread-word qubit
 
 
 class Circuit:

@@ -375,8 +375,21 @@ struct PassBuilder {
 	long last = -1;
 	unsigned last_q = 0;
 	u64 last_cm = 0, last_am = 0;
+	// Per op of this pass (ops[first + i] <-> meta[i]): the qubits it acts
+	// on non-diagonally, and for a phase on |11> of two qubits (a diagonal
+	// diag(1, phase) with one control) those qubits, which a later dense
+	// op on either of them may absorb.
+	struct OpMeta {
+		u64 N = 0;
+		bool phase2 = false;
+		unsigned qa = 0, qb = 0;
+		cd phase = 1.0;
+	};
+	size_t first;
+	std::vector<OpMeta> meta;
 
-	PassBuilder(const dk_circuit *c_, std::vector<PassOp> &ops_) : c(c_), ops(ops_)
+	PassBuilder(const dk_circuit *c_, std::vector<PassOp> &ops_)
+		: c(c_), ops(ops_), first(ops_.size())
 	{
 		for (int q = 0; q < 64; ++q)
 			slot_of[q] = -1;
@@ -435,6 +448,57 @@ struct PassBuilder {
 		return reg;
 	}
 
+	// Dense uncontrolled gate m on q absorbing an earlier phase on |11> of
+	// (q, s): m where s is 0, m * diag(1, phase) where s is 1, as one mux
+	// op.  The phase moves past the ops in between, which act on neither
+	// q nor s non-diagonally.  The selector s must be a register bit (for a
+	// register target) or a thread-word bit >= 6 (one value per wave).
+	bool try_fold(unsigned q, const cd (&m)[4])
+	{
+		unsigned tpos, spos;
+		const int tw = where(q, &tpos);
+		u64 B = 0; // qubits acted on non-diagonally after the candidate
+		for (size_t i = ops.size(); i-- > first && ops.size() - i <= 64;) {
+			const OpMeta &mt = meta[i - first];
+			if (mt.phase2 && (mt.qa == q || mt.qb == q)) {
+				const unsigned sq = mt.qa == q ? mt.qb : mt.qa;
+				const int sw = where(sq, &spos);
+				const bool sel_ok = (sw == 1 && tw == 1) || sw == 2 || sw == 3;
+				if (sel_ok && !(B & ((1ull << q) | (1ull << sq)))) {
+					const cd ph = mt.phase;
+					ops.erase(ops.begin() + (long)i);
+					meta.erase(meta.begin() + (long)(i - first));
+					PassOp o, d;
+					memset(&o, 0, sizeof(o));
+					memset(&d, 0, sizeof(d));
+					const cd m1[4] = {m[0], m[1] * ph, m[2], m[3] * ph};
+					for (int k = 0; k < 4; ++k) {
+						o.m[2 * k] = m[k].real();
+						o.m[2 * k + 1] = m[k].imag();
+						d.m[2 * k] = m1[k].real();
+						d.m[2 * k + 1] = m1[k].imag();
+					}
+					o.kind = tw == 1 ? dkp::OP_REG_MUX : dkp::OP_LANE_MUX;
+					o.a = tpos;
+					o.b = sw == 1 ? (dkp::SEL_REG | spos) : sq;
+					d.kind = dkp::OP_MDATA;
+					ops.push_back(o);
+					ops.push_back(d);
+					OpMeta mo;
+					mo.N = 1ull << q;
+					meta.push_back(mo);
+					meta.push_back(OpMeta());
+					last = -1;
+					return true;
+				}
+			}
+			B |= mt.N;
+			if ((B >> q) & 1)
+				break;
+		}
+		return false;
+	}
+
 	void emit_gate(const CGate &g)
 	{
 		const unsigned q = g.t[0];
@@ -447,6 +511,10 @@ struct PassBuilder {
 				prev[i] = cd(o.m[2 * i], o.m[2 * i + 1]);
 			mul22(m, prev, m);
 			ops.pop_back(); // the previous op is rebuilt below
+			meta.pop_back();
+		} else if (g.cm == 0 && g.am == 0 &&
+			   !(m[1] == cd(0, 0) && m[2] == cd(0, 0)) && try_fold(q, m)) {
+			return;
 		}
 		PassOp o;
 		memset(&o, 0, sizeof(o));
@@ -466,6 +534,16 @@ struct PassBuilder {
 		}
 		o.rmask = split_mask(g.cm, &o.tcm) | split_mask(g.am, &o.tam) << 8;
 		ops.push_back(o);
+		OpMeta mt;
+		if (!diag)
+			mt.N = 1ull << q;
+		else if (m[0] == cd(1, 0) && g.am == 0 && g.cm && !(g.cm & (g.cm - 1))) {
+			mt.phase2 = true;
+			mt.qa = q;
+			mt.qb = (unsigned)__builtin_ctzll(g.cm);
+			mt.phase = m[3];
+		}
+		meta.push_back(mt);
 		last = (long)ops.size() - 1;
 		last_q = q;
 		last_cm = g.cm;
@@ -512,6 +590,7 @@ struct PassBuilder {
 			o.b |= nw[b] << (3 * b);
 		}
 		ops.push_back(o);
+		meta.push_back(OpMeta());
 		last = -1;
 	}
 };
@@ -541,6 +620,25 @@ bool pass_ops_valid(const PassOp *ops, size_t n, unsigned nq)
 			if (seen != (1u << dkp::HIGH_BITS) - 1 || !kept ||
 			    (o.a >> (3 * dkp::REG_BITS)) || (o.b >> (3 * dkp::WAVE_BITS)))
 				return false;
+			continue;
+		}
+		case dkp::OP_REG_MUX:
+		case dkp::OP_LANE_MUX: {
+			// a second-matrix record follows; no controls; the selector
+			// is another register bit or a thread-word bit >= 6
+			const bool reg = o.kind == dkp::OP_REG_MUX;
+			if (i + 1 >= n || ops[i + 1].kind != dkp::OP_MDATA ||
+			    o.a >= (unsigned)(reg ? dkp::REG_BITS : dkp::LANE_BITS) ||
+			    o.rmask || o.tcm || o.tam)
+				return false;
+			if (o.b & dkp::SEL_REG) {
+				const unsigned j = o.b & ~dkp::SEL_REG;
+				if (!reg || j >= (unsigned)dkp::REG_BITS || j == o.a)
+					return false;
+			} else if (o.b < (unsigned)dkp::LANE_BITS || o.b >= nq) {
+				return false;
+			}
+			++i; // the data record
 			continue;
 		}
 		case dkp::OP_REG:

@@ -44,11 +44,21 @@ enum OpKind : uint32_t {
 	OP_DIAG_THREAD = 3, // diagonal on qubit a, a bit of the thread word
 	OP_EXCH = 4,        // new mapping: a = 4 x 3-bit register slots, b = 2 x
 	                    // 3-bit wave slots (high slot bit indices)
+	OP_REG_MUX = 5,     // dense 2x2 on register bit a, matrix chosen per
+	                    // amplitude by a selector qubit (b): m if it is 0,
+	                    // the next record's m if it is 1
+	OP_LANE_MUX = 6,    // the same on lane bit (qubit) a
+	OP_MDATA = 7,       // second matrix of the preceding mux op
 };
+
+// Selector of a mux op (field b): SEL_REG | j for register bit j (register
+// targets only), or the qubit of a thread-word bit >= 6, the same for the
+// whole wave.
+constexpr uint32_t SEL_REG = 0x100;
 
 // One op of a pass (96 bytes).  rmask: register-bit controls in bits 0..3,
 // register-bit anticontrols in bits 8..11; tcm / tam: control and
-// anticontrol qubits of the thread word.
+// anticontrol qubits of the thread word.  Mux ops carry no controls.
 struct alignas(16) PassOp {
 	uint32_t kind, a, b, rmask;
 	uint64_t tcm, tam;

@@ -139,6 +139,85 @@ __device__ __forceinline__ void gate_reg_any(R (&vr)[REGS], R (&vi)[REGS],
 	}
 }
 
+// Dense 2x2 gate on register bit J whose matrix is M0 where register bit K
+// is 0 and M1 where it is 1 (a phase on |11> absorbed into a gate on one of
+// its qubits).
+template <int J, int K, typename R>
+__device__ __forceinline__ void gate_reg_mux(R (&vr)[REGS], R (&vi)[REGS],
+					     const M22<R> &M0, const M22<R> &M1)
+{
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		if (r & (1 << J))
+			continue;
+		constexpr int d = 1 << J;
+		const M22<R> &M = ((r >> K) & 1) ? M1 : M0;
+		R o0r, o0i, o1r, o1i;
+		row2(M.m00, vr[r], vi[r], M.m01, vr[r | d], vi[r | d], o0r, o0i);
+		row2(M.m10, vr[r], vi[r], M.m11, vr[r | d], vi[r | d], o1r, o1i);
+		vr[r] = o0r;
+		vi[r] = o0i;
+		vr[r | d] = o1r;
+		vi[r | d] = o1i;
+	}
+}
+
+template <int J, typename R>
+__device__ __forceinline__ void gate_reg_mux_k(R (&vr)[REGS], R (&vi)[REGS],
+					       const M22<R> &M0, const M22<R> &M1,
+					       unsigned k)
+{
+	// k != J (checked by the planner's validator); distinct asm markers:
+	// see gate_reg_any
+	switch (k) {
+	case 0:
+		if constexpr (J != 0)
+			gate_reg_mux<J, 0>(vr, vi, M0, M1);
+		asm volatile("; mux selector bit 0");
+		break;
+	case 1:
+		if constexpr (J != 1)
+			gate_reg_mux<J, 1>(vr, vi, M0, M1);
+		asm volatile("; mux selector bit 1");
+		break;
+	case 2:
+		if constexpr (J != 2)
+			gate_reg_mux<J, 2>(vr, vi, M0, M1);
+		asm volatile("; mux selector bit 2");
+		break;
+	default:
+		if constexpr (J != 3)
+			gate_reg_mux<J, 3>(vr, vi, M0, M1);
+		asm volatile("; mux selector bit 3");
+		break;
+	}
+}
+
+template <typename R>
+__device__ __forceinline__ void gate_reg_mux_any(R (&vr)[REGS], R (&vi)[REGS],
+						 const M22<R> &M0, const M22<R> &M1,
+						 unsigned j, unsigned k)
+{
+	switch (j) {
+	case 0:
+		gate_reg_mux_k<0>(vr, vi, M0, M1, k);
+		asm volatile("; mux register bit 0");
+		break;
+	case 1:
+		gate_reg_mux_k<1>(vr, vi, M0, M1, k);
+		asm volatile("; mux register bit 1");
+		break;
+	case 2:
+		gate_reg_mux_k<2>(vr, vi, M0, M1, k);
+		asm volatile("; mux register bit 2");
+		break;
+	default:
+		gate_reg_mux_k<3>(vr, vi, M0, M1, k);
+		asm volatile("; mux register bit 3");
+		break;
+	}
+}
+
 // The value of lane ^ 2^B.  B < 2: one DPP quad permutation, no LDS
 // traffic.  B >= 2: ds_bpermute, which measured faster on gfx950 than DPP
 // mirror pairs (two dependent moves with hazard waits) and than the
@@ -474,6 +553,25 @@ __device__ __forceinline__ void tile_run(
 			for (int k = 0; k < WAVE_BITS; ++k)
 				ws[k] = nws[k];
 			tb = base | lane | slot_index(wave_slots(w, ws), hb);
+			continue;
+		}
+		if (kind == OP_REG_MUX || kind == OP_LANE_MUX) {
+			// two matrices (this record and the next), chosen by a
+			// selector qubit; no controls
+			const M22<R> M0 = op_matrix<R>(op), M1 = op_matrix<R>(op + 1);
+			const unsigned sel = op->b;
+			++i;
+			if (sel & SEL_REG) {
+				gate_reg_mux_any(vr, vi, M0, M1, a, sel & (REG_BITS - 1));
+			} else {
+				// a thread-word bit >= 6: one value for the whole wave
+				const bool one = __builtin_amdgcn_readfirstlane((int)((tb >> sel) & 1));
+				const M22<R> M = one ? M1 : M0;
+				if (kind == OP_REG_MUX)
+					gate_reg_any<CM_NONE>(vr, vi, M, true, 0, a);
+				else
+					gate_lane_any<CM_NONE>(vr, vi, M, true, 0, a, lane);
+			}
 			continue;
 		}
 		const M22<R> M = op_matrix<R>(op);

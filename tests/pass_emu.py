@@ -11,7 +11,9 @@ mechanics are checked on the GPU against the oracle.
 import numpy as np
 
 LANE_BITS, HIGH_BITS, REG_BITS, WAVE_BITS = 6, 6, 4, 2
-OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH = range(5)
+(OP_REG, OP_LANE, OP_DIAG_REG, OP_DIAG_THREAD, OP_EXCH, OP_REG_MUX, OP_LANE_MUX,
+ OP_MDATA) = range(8)
+SEL_REG = 0x100
 
 
 def _tile_index(n, tile_qubits):
@@ -45,8 +47,13 @@ def run_pass(psi, n, tile_qubits, ops):
     def bit(q):
         return ((idx >> q) & 1).astype(bool)
 
-    for op in ops:
+    ops = list(ops)
+    for i, op in enumerate(ops):
         kind = int(op["kind"])
+        if kind == OP_MDATA:
+            # read by the mux op before it
+            assert i > 0 and int(ops[i - 1]["kind"]) in (OP_REG_MUX, OP_LANE_MUX)
+            continue
         if kind == OP_EXCH:
             new_rs = [(int(op["a"]) >> (3 * j)) & 7 for j in range(REG_BITS)]
             ws = [(int(op["b"]) >> (3 * b)) & 7 for b in range(WAVE_BITS)]
@@ -74,8 +81,8 @@ def run_pass(psi, n, tile_qubits, ops):
             if (rmask >> (8 + j)) & 1:
                 ok &= ~bit(reg_qubit(j))
         a = int(op["a"])
-        if kind in (OP_REG, OP_LANE):
-            if kind == OP_REG:
+        if kind in (OP_REG, OP_LANE, OP_REG_MUX, OP_LANE_MUX):
+            if kind in (OP_REG, OP_REG_MUX):
                 q, sb = reg_qubit(a), LANE_BITS + rs[a]
             else:
                 assert a < LANE_BITS
@@ -85,6 +92,23 @@ def run_pass(psi, n, tile_qubits, ops):
             upper = bit(q)
             x = np.where(upper, A[:, partner], A)   # lower amplitude
             y = np.where(upper, A, A[:, partner])   # upper amplitude
+            if kind in (OP_REG_MUX, OP_LANE_MUX):
+                # m where the selector qubit is 0, the next record's where 1
+                assert not (tcm | tam | rmask)
+                m1 = ops[i + 1]["m"].view(np.complex128).reshape(2, 2)
+                assert int(ops[i + 1]["kind"]) == OP_MDATA
+                b = int(op["b"])
+                if b & SEL_REG:
+                    assert kind == OP_REG_MUX and (b & 0xff) < REG_BITS and (b & 0xff) != a
+                    s = reg_qubit(b & 0xff)
+                else:
+                    assert LANE_BITS <= b < n and not ((held >> b) & 1)
+                    s = b
+                sel = bit(s)
+                o0 = np.where(sel, m1[0, 0] * x + m1[0, 1] * y, m[0, 0] * x + m[0, 1] * y)
+                o1 = np.where(sel, m1[1, 0] * x + m1[1, 1] * y, m[1, 0] * x + m[1, 1] * y)
+                A = np.where(upper, o1, o0)
+                continue
             o = np.where(upper, m[1, 0] * x + m[1, 1] * y, m[0, 0] * x + m[0, 1] * y)
             A = np.where(ok, o, A)
         else:

@@ -104,6 +104,31 @@ def test_brickwork_plan(oracle, capi):
     assert npass * 8 < len(steps)
 
 
+def test_phase_pairs_fold_into_dense_ops(oracle, capi):
+    """A phase on |11> of two qubits (CZ, controlled phase) and a later
+    uncontrolled dense gate on one of them become one two-matrix op chosen
+    by the other qubit, when that qubit is a register bit or a thread-word
+    bit >= 6; every such op is followed by its data record."""
+    n = 14
+    steps = W.brickwork_circuit(21, n, 6)
+    circ, npass = check_plan(oracle, capi, n, steps)
+    kinds = np.concatenate([circ.pass_ops(i)["kind"] for i in range(npass)])
+    mux = np.isin(kinds, [capi.OP_REG_MUX, capi.OP_LANE_MUX])
+    assert mux.sum() > 0
+    assert np.array_equal(np.flatnonzero(kinds == capi.OP_MDATA), np.flatnonzero(mux) + 1)
+    # both selector forms: a register bit, and a qubit outside the tile
+    n = 22
+    u = W.gate_u3(0.4, 1.3, -0.2)
+    cp = np.diag([1, np.exp(0.7j)])
+    steps = [(u, [q], set(), set()) for q in (8, 9, 10, 11)]
+    steps += [(cp, [8], {9}, set()), (u, [9], set(), set()),
+              (W.gate_z(), [10], {20}, set()), (u, [10], set(), set())]
+    circ, npass = check_plan(oracle, capi, n, steps)
+    ops = np.concatenate([circ.pass_ops(i) for i in range(npass)])
+    sels = [int(o["b"]) for o in ops if int(o["kind"]) == capi.OP_REG_MUX]
+    assert any(b & capi.SEL_REG for b in sels) and 20 in sels
+
+
 def test_controls_everywhere(oracle, capi):
     """Every control/anticontrol location (lane, register, wave, outside
     the tile) on dense and diagonal gates."""
