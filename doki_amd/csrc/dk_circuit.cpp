@@ -382,8 +382,9 @@ struct PassBuilder {
 	struct OpMeta {
 		u64 N = 0;
 		bool phase2 = false;
+		bool diag1 = false;  // an uncontrolled diagonal diag(d0, d1) on qa
 		unsigned qa = 0, qb = 0;
-		cd phase = 1.0;
+		cd phase = 1.0, d0 = 1.0, d1 = 1.0;
 	};
 	size_t first;
 	std::vector<OpMeta> meta;
@@ -448,18 +449,33 @@ struct PassBuilder {
 		return reg;
 	}
 
-	// Dense uncontrolled gate m on q absorbing an earlier phase on |11> of
-	// (q, s): m where s is 0, m * diag(1, phase) where s is 1, as one mux
-	// op.  The phase moves past the ops in between, which act on neither
+	// Dense uncontrolled gate m on q absorbing an earlier diagonal: an
+	// uncontrolled one on q (m becomes m * diag(d0, d1); returns FOLD_MORE,
+	// the caller tries again and emits m), or a phase on |11> of (q, s): m
+	// where s is 0, m * diag(1, phase) where s is 1, as one mux op (emitted
+	// here; returns FOLD_DONE).  FOLD_NONE: nothing to absorb.  The phase moves past the ops in between, which act on neither
 	// q nor s non-diagonally.  The selector s must be a register bit (for a
 	// register target) or a thread-word bit >= 6 (one value per wave).
-	bool try_fold(unsigned q, const cd (&m)[4])
+	enum { FOLD_NONE, FOLD_MORE, FOLD_DONE };
+	int try_fold(unsigned q, cd (&m)[4])
 	{
 		unsigned tpos, spos;
 		const int tw = where(q, &tpos);
 		u64 B = 0; // qubits acted on non-diagonally after the candidate
 		for (size_t i = ops.size(); i-- > first && ops.size() - i <= 64;) {
 			const OpMeta &mt = meta[i - first];
+			if (mt.diag1 && mt.qa == q && !((B >> q) & 1)) {
+				// an uncontrolled diagonal on q: m * diag(d0, d1),
+				// emitted as a plain op by the caller
+				m[0] *= mt.d0;
+				m[2] *= mt.d0;
+				m[1] *= mt.d1;
+				m[3] *= mt.d1;
+				ops.erase(ops.begin() + (long)i);
+				meta.erase(meta.begin() + (long)(i - first));
+				last = -1;
+				return FOLD_MORE;
+			}
 			if (mt.phase2 && (mt.qa == q || mt.qb == q)) {
 				const unsigned sq = mt.qa == q ? mt.qb : mt.qa;
 				const int sw = where(sq, &spos);
@@ -489,14 +505,14 @@ struct PassBuilder {
 					meta.push_back(mo);
 					meta.push_back(OpMeta());
 					last = -1;
-					return true;
+					return FOLD_DONE;
 				}
 			}
 			B |= mt.N;
 			if ((B >> q) & 1)
 				break;
 		}
-		return false;
+		return FOLD_NONE;
 	}
 
 	void emit_gate(const CGate &g)
@@ -512,9 +528,12 @@ struct PassBuilder {
 			mul22(m, prev, m);
 			ops.pop_back(); // the previous op is rebuilt below
 			meta.pop_back();
-		} else if (g.cm == 0 && g.am == 0 &&
-			   !(m[1] == cd(0, 0) && m[2] == cd(0, 0)) && try_fold(q, m)) {
-			return;
+		} else if (g.cm == 0 && g.am == 0 && !(m[1] == cd(0, 0) && m[2] == cd(0, 0))) {
+			int f;
+			while ((f = try_fold(q, m)) == FOLD_MORE) {
+			}
+			if (f == FOLD_DONE)
+				return;
 		}
 		PassOp o;
 		memset(&o, 0, sizeof(o));
@@ -535,9 +554,14 @@ struct PassBuilder {
 		o.rmask = split_mask(g.cm, &o.tcm) | split_mask(g.am, &o.tam) << 8;
 		ops.push_back(o);
 		OpMeta mt;
-		if (!diag)
+		if (!diag) {
 			mt.N = 1ull << q;
-		else if (m[0] == cd(1, 0) && g.am == 0 && g.cm && !(g.cm & (g.cm - 1))) {
+		} else if (g.cm == 0 && g.am == 0) {
+			mt.diag1 = true;
+			mt.qa = q;
+			mt.d0 = m[0];
+			mt.d1 = m[3];
+		} else if (m[0] == cd(1, 0) && g.am == 0 && g.cm && !(g.cm & (g.cm - 1))) {
 			mt.phase2 = true;
 			mt.qa = q;
 			mt.qb = (unsigned)__builtin_ctzll(g.cm);
