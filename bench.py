@@ -184,9 +184,11 @@ def big_leg(args):
 
 def fused_leg(args):
     """Fused dense gates on the matrix cores at the same register size:
-    per-k kernel rates (FP64 complex128 and FP32 complex64), and the
-    brickwork circuit plain (per gate) vs fused dense blocks vs cache-blocked
-    passes (circuits/s), each with its deviation from the plain FP64 result."""
+    per-k kernel rates (FP64 complex128 and FP32 complex64), circuits of
+    Haar-random k-qubit blocks in both precisions (FP32 error and fidelity
+    against FP64), and the brickwork circuit plain (per gate) vs fused dense
+    blocks vs cache-blocked passes (circuits/s), each with its error and
+    fidelity against the plain FP64 result."""
     import numpy as np
 
     from doki_amd import capi, workloads as W
@@ -212,6 +214,39 @@ def fused_leg(args):
     capi.tune("time_ring", 0)
     log(f"[bench] dense kernels {out['dense_kernel']}")
 
+    # circuits of Haar-random k-qubit blocks (four layers of blocks across
+    # the register, shifted per layer), in FP64 and in complex64/FP32; the
+    # FP32 result against the FP64 one over the whole state
+    haar = {}
+    for k in (4, 5, 6):
+        blocks = []
+        for layer in range(4):
+            off = (layer * (k // 2 + 1)) % k
+            for j in range(off, n - k + 1, k):
+                blocks.append((capi.Gate(W.haar_unitary(rng, k)), list(range(j, j + k))))
+        rk = {"blocks": len(blocks)}
+        states = {}
+        for dtype, name in ((capi.C128, "f64"), (capi.C64, "f32")):
+            s = capi.new_typed(n, dtype)
+            for g, tg in blocks[:2]:
+                capi.apply(s, g, tg, in_place=True)    # warm (gate uploads)
+            s = capi.new_typed(n, dtype)
+            capi.synchronize()
+            t0 = time.perf_counter()
+            for g, tg in blocks:
+                capi.apply(s, g, tg, in_place=True)
+            capi.synchronize()
+            rk[f"{name}_seconds"] = round(time.perf_counter() - t0, 4)
+            states[name] = s
+        inner, mx = capi.compare(states["f64"], states["f32"])
+        rk["f32_max_abs_err_vs_f64"] = mx
+        rk["f32_fidelity_vs_f64"] = abs(inner) ** 2
+        del states, s
+        capi.synchronize()
+        haar[f"k{k}"] = rk
+        log(f"[bench] haar blocks k={k}: {rk}")
+    out["haar_block_circuits"] = haar
+
     depth = args.fused_depth
     steps = W.brickwork_circuit(77, n, depth)
     res = {}
@@ -235,10 +270,11 @@ def fused_leg(args):
         if label == "plain_f64":
             ref_state = st
         else:
-            # against the plain FP64 per-gate result, first 4M amplitudes
-            a = ref_state.raw(0, 1 << 22)[0] / ref_state.norm
-            b = st.raw(0, 1 << 22)[0] / st.norm
-            res[label]["max_abs_err_vs_plain_f64_first_4M"] = float(np.max(np.abs(a - b)))
+            # against the plain FP64 per-gate result, whole state (one pass
+            # over both on the GPU)
+            inner, mx = capi.compare(ref_state, st)
+            res[label]["max_abs_err_vs_plain_f64"] = mx
+            res[label]["fidelity_vs_plain_f64"] = abs(inner) ** 2
         if label != "plain_f64":
             del st
         capi.synchronize()

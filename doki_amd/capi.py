@@ -66,6 +66,7 @@ def lib():
         "dk_first_nonzero": (C.c_int, [P, C.POINTER(u64)]),
         "dk_read_raw": (C.c_int, [P, u64, u64, dp, dp]),
         "dk_read": (C.c_int, [P, u64, u64, dp]),
+        "dk_state_compare": (C.c_int, [P, P, dp, dp]),
         "dk_reduced_density": (C.c_int, [P, u64, dp]),
         "dk_reduced_density_elements": (C.c_int, [P, u64, C.c_size_t,
                                                   C.POINTER(u64), C.POINTER(u64), dp]),
@@ -245,6 +246,15 @@ def join(a, b):
     p = C.c_void_p()
     check(lib().dk_join(a.ptr, b.ptr, C.byref(p)))
     return State(p.value)
+
+
+def compare(a, b):
+    """(<a|b>, max_i |a_i - b_i|) over normalised amplitudes of two states of
+    one size (either amplitude type); |<a|b>|^2 is the fidelity."""
+    inner = np.zeros(2)
+    mx = np.zeros(1)
+    check(lib().dk_state_compare(a.ptr, b.ptr, _dp(inner), _dp(mx)))
+    return complex(inner[0], inner[1]), float(mx[0])
 
 
 def first_nonzero(s):

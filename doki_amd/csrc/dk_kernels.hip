@@ -798,6 +798,95 @@ __global__ __launch_bounds__(TPB) void k_density_elements(
 }
 
 // ---------------------------------------------------------------------------
+// Overlap and distance of two states of one size (normalised amplitudes):
+// per-block FP64 partials of sum conj(a_i) b_i and max |a_i - b_i|, then one
+// block combines them in block order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v = fmax(v, __shfl_xor(v, o));
+	return v;
+}
+
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(TPB) void k_compare(const Ta *a, const Tb *b,
+						 const double *__restrict__ na,
+						 const double *__restrict__ nb, u64 len,
+						 double *__restrict__ parts)
+{
+	__shared__ double red[3][WAVES];
+	const double ia = 1.0 / na[DK_SCAL_NORM], ib = 1.0 / nb[DK_SCAL_NORM];
+	double sr = 0.0, si = 0.0, mx = 0.0;
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len;
+	     i += (u64)gridDim.x * TPB) {
+		const Ta x = ld_stream(a + i);
+		const Tb y = ld_stream(b + i);
+		const double xr = (double)x.x * ia, xi = (double)x.y * ia;
+		const double yr = (double)y.x * ib, yi = (double)y.y * ib;
+		sr = fma(xr, yr, fma(xi, yi, sr));
+		si = fma(xr, yi, fma(-xi, yr, si));
+		const double dr = xr - yr, di = xi - yi;
+		mx = fmax(mx, sqrt(fma(dr, dr, di * di)));
+	}
+	sr = wave_sum(sr);
+	si = wave_sum(si);
+	mx = wave_max(mx);
+	if ((threadIdx.x & 63) == 0) {
+		red[0][threadIdx.x >> 6] = sr;
+		red[1][threadIdx.x >> 6] = si;
+		red[2][threadIdx.x >> 6] = mx;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double r = 0.0, im = 0.0, m = 0.0;
+#pragma unroll
+		for (int w = 0; w < WAVES; ++w) {
+			r += red[0][w];
+			im += red[1][w];
+			m = fmax(m, red[2][w]);
+		}
+		parts[3 * (u64)blockIdx.x] = r;
+		parts[3 * (u64)blockIdx.x + 1] = im;
+		parts[3 * (u64)blockIdx.x + 2] = m;
+	}
+}
+
+__global__ __launch_bounds__(TPB) void k_compare_finish(const double *__restrict__ parts,
+							int nblocks, double *__restrict__ out)
+{
+	__shared__ double red[3][WAVES];
+	double sr = 0.0, si = 0.0, mx = 0.0;
+	for (int k = threadIdx.x; k < nblocks; k += TPB) {
+		sr += parts[3 * k];
+		si += parts[3 * k + 1];
+		mx = fmax(mx, parts[3 * k + 2]);
+	}
+	sr = wave_sum(sr);
+	si = wave_sum(si);
+	mx = wave_max(mx);
+	if ((threadIdx.x & 63) == 0) {
+		red[0][threadIdx.x >> 6] = sr;
+		red[1][threadIdx.x >> 6] = si;
+		red[2][threadIdx.x >> 6] = mx;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double r = 0.0, im = 0.0, m = 0.0;
+#pragma unroll
+		for (int w = 0; w < WAVES; ++w) {
+			r += red[0][w];
+			im += red[1][w];
+			m = fmax(m, red[2][w]);
+		}
+		out[0] = r;
+		out[1] = im;
+		out[2] = m;
+	}
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launch helpers
 // ---------------------------------------------------------------------------
 // One thread per unit (element-wise kernels): every load of the launch is in
@@ -1333,6 +1422,34 @@ int dk_launch_set_norm(dk_state *s, double v)
 {
 	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, DK_STREAM,
 			   s->scal + DK_SCAL_NORM, v);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+// dst[0..2] (device) <- re and im of sum conj(a_i) b_i, max |a_i - b_i|, over
+// normalised amplitudes of two states of one size.
+int dk_launch_compare(const dk_state *a, const dk_state *b, double *dst)
+{
+	const int nblocks = (int)std::min<u64>(std::max(1, g_ctx.num_cus) * 8ull,
+					       std::max<u64>(1, a->len / TPB));
+	double *parts = nullptr;
+	DK_TRY(dk_partials_reserve(3ull * (u64)nblocks, &parts));
+	hipStream_t st = DK_STREAM;
+#define DK_CMP(TA, TB)                                                          \
+	hipLaunchKernelGGL((k_compare<TA, TB>), dim3(nblocks), dim3(TPB), 0, st,  \
+			   (const TA *)a->amp, (const TB *)b->amp, a->scal, b->scal, \
+			   a->len, parts)
+	if (a->dtype == DK_C64 && b->dtype == DK_C64)
+		DK_CMP(float2, float2);
+	else if (a->dtype == DK_C64)
+		DK_CMP(float2, double2);
+	else if (b->dtype == DK_C64)
+		DK_CMP(double2, float2);
+	else
+		DK_CMP(double2, double2);
+#undef DK_CMP
+	DK_LAUNCHED();
+	hipLaunchKernelGGL(k_compare_finish, dim3(1), dim3(TPB), 0, st, parts, nblocks, dst);
 	DK_LAUNCHED();
 	return DK_OK;
 }

@@ -614,6 +614,35 @@ extern "C" int dk_prob(const dk_state *s, unsigned q, double *p)
 	return copy_to_host(p, dst, 1);
 }
 
+int dk_launch_compare(const dk_state *a, const dk_state *b, double *dst);
+
+extern "C" int dk_state_compare(const dk_state *a, const dk_state *b, double *inner,
+				double *max_abs)
+{
+	DK_INIT();
+	if (!a || !b) {
+		dk_set_error("NULL state");
+		return DK_E_ARG;
+	}
+	if (a->len != b->len) {
+		dk_set_error("states of different sizes");
+		return DK_E_ARG;
+	}
+	DK_SETTLE(a);
+	DK_SETTLE(b);
+	double *dst = dk_cur()->dec + DEC_PROBS;
+	DK_TRY(dk_launch_compare(a, b, dst));
+	double r[3];
+	DK_TRY(copy_to_host(r, dst, 3));
+	if (inner) {
+		inner[0] = r[0];
+		inner[1] = r[1];
+	}
+	if (max_abs)
+		*max_abs = r[2];
+	return DK_OK;
+}
+
 extern "C" int dk_prob_all(const dk_state *s, double *p)
 {
 	DK_INIT();

@@ -190,6 +190,15 @@ int dk_join(const dk_state *most, const dk_state *least, dk_state **out);
  * Device half of get_global_phase, qops.c:46-70. */
 int dk_first_nonzero(const dk_state *s, uint64_t *index);
 
+/* Overlap and distance of two states of the same size (either amplitude
+ * type on each side), over normalised amplitudes: inner[0..1] = re, im of
+ * sum_i conj(a_i) * b_i (|inner|^2 is the fidelity) and *max_abs =
+ * max_i |a_i - b_i|.  One pass over both states, FP64 partials combined in a
+ * fixed order.  No counterpart in the reference; for checking fused and
+ * complex64 results against complex128 ones.  Either output may be NULL. */
+int dk_state_compare(const dk_state *a, const dk_state *b, double *inner,
+		     double *max_abs);
+
 /* Raw stored amplitudes [first, first+count) and the norm constant, so the
  * caller divides exactly as state_get does (qstate.h:86).  Read-out of
  * registry_get, doki.c:546-595. */

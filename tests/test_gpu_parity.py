@@ -444,3 +444,23 @@ def test_large_register_two_level_norm(capi, oracle, t):
         o = oracle.apply(oracle.from_data(n, psi), u, [t], list(ctrl))
         assert abs(d.norm - o.norm) <= ATOL
         assert maxabs(d.values(), o.values()) <= ATOL
+
+
+@pytest.mark.parametrize("n", [3, 8, 14, 22])
+def test_state_compare(capi, n):
+    """dk_state_compare: <a|b> and max |a_i - b_i| over the values a read
+    returns (raw / norm constant), against numpy; complex64 against
+    complex128 of the same data."""
+    a = W.random_normalised_state(70 + n, n)
+    b = W.random_normalised_state(90 + n, n) * 1.5
+    da, db = capi.from_host(a), capi.from_host(b)
+    inner, mx = capi.compare(da, da)
+    assert abs(inner - 1) <= ATOL and mx == 0.0
+    inner, mx = capi.compare(da, db)
+    assert abs(inner - np.vdot(a, b)) <= ATOL
+    assert abs(mx - np.max(np.abs(a - b))) <= ATOL
+    d32 = capi.convert(da, capi.C64)
+    inner, mx = capi.compare(da, d32)
+    a32 = a.astype(np.complex64).astype(np.complex128)
+    assert abs(inner - np.vdot(a, a32)) <= 1e-6
+    assert abs(mx - np.max(np.abs(a - a32))) <= 1e-7
