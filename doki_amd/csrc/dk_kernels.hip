@@ -79,14 +79,10 @@ __device__ __forceinline__ void pairs_block(
 }
 
 // LOOP: virtual blocks beyond the grid (launches of more than 2^31 threads).
-// xcd: block b takes virtual block (b mod 8) * (nvb / 8) + b / 8, so each
-// XCD (workgroups are dealt to the 8 XCDs round robin) streams one
-// contiguous eighth of the state; used for target 6, where it measured 3-5%
-// faster at 30 qubits (for targets 7..19 it measured the same or slower).
 template <typename T, int G, bool LOOP>
 __global__ __launch_bounds__(128 * G) void k_apply1_pairs(
 	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb, bool xcd)
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
 {
 	typedef typename real_of<T>::type R;
 	__shared__ T buf[128 * G];
@@ -98,9 +94,55 @@ __global__ __launch_bounds__(128 * G) void k_apply1_pairs(
 			__syncthreads();
 		}
 	} else {
-		const u64 b = blockIdx.x;
-		const u64 vb = xcd ? (b & 7) * (nvb >> 3) + (b >> 3) : b;
-		pairs_block<T, G>(in, out, inv, M, t, cm, acm, partials, vb, buf, red);
+		pairs_block<T, G>(in, out, inv, M, t, cm, acm, partials, blockIdx.x, buf,
+				  red);
+	}
+}
+
+// t = 6..9: one wave per block holds 32 pairs, lanes 0..31 the lower
+// amplitudes (a contiguous 512-byte run) and lanes 32..63 their partners
+// 2^t above; the partner comes from lane ^ 32 (no LDS buffer, no barrier).
+// Measured at 30 qubits: 6.5-6.6 TB/s for these targets against 6.2-6.5
+// with the two-wave LDS exchange; slower than it for t >= 12 (5.9 at 20-21).
+template <typename T>
+__device__ __forceinline__ void halves_block(
+	const T *in, T *out, typename real_of<T>::type inv, const Mat2<T> &M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 vb)
+{
+	const u64 bit = 1ull << t, lo = bit - 1;
+	const unsigned lane = threadIdx.x & 63;
+	const u64 p = vb * 32 + (lane & 31);
+	const u64 i0 = ((p & ~lo) << 1) | (p & lo);
+	const bool upper = lane >= 32;
+	const u64 my = upper ? (i0 | bit) : i0;
+	const T v = cscale(ld_stream(in + my), inv);
+	T w;
+	w.x = __shfl_xor(v.x, 32);
+	w.y = __shfl_xor(v.y, 32);
+	const T x = upper ? w : v, y = upper ? v : w;
+	T o = v;
+	if ((i0 & cm) == cm && (i0 & acm) == 0) {
+		o = upper ? cmac(cmac(czero<T>(), M.m10, x), M.m11, y)
+			  : cmac(cmac(czero<T>(), M.m00, x), M.m01, y);
+	}
+	st_stream(out + my, o);
+	const double s = wave_sum(cabs2d(o));
+	if (threadIdx.x == 0)
+		partials[vb] = s;
+}
+
+template <typename T, bool LOOP>
+__global__ __launch_bounds__(64) void k_apply1_halves(
+	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+{
+	typedef typename real_of<T>::type R;
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	if constexpr (LOOP) {
+		for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+			halves_block<T>(in, out, inv, M, t, cm, acm, partials, vb);
+	} else {
+		halves_block<T>(in, out, inv, M, t, cm, acm, partials, blockIdx.x);
 	}
 }
 
@@ -930,7 +972,18 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		int G = g_ctx.apply1_group;
 		while (G > 1 && in->len < 128ull * G)
 			G >>= 1;
-		if (in->len >= 128 && t >= 6) {
+		if (in->len >= 128 && t >= 6 && t <= 9) {
+			nblocks = in->len / 64;
+			const unsigned gr = dk_grid_cap(nblocks, 64);
+			if (gr < nblocks)
+				hipLaunchKernelGGL((k_apply1_halves<T, true>), dim3(gr), dim3(64), 0,
+						   st, src, dst, in->scal, M, t, cm, acm, parts,
+						   nblocks);
+			else
+				hipLaunchKernelGGL((k_apply1_halves<T, false>), dim3(gr), dim3(64),
+						   0, st, src, dst, in->scal, M, t, cm, acm, parts,
+						   nblocks);
+		} else if (in->len >= 128 && t >= 6) {
 			nblocks = in->len / (128ull * G);
 #define DK_PAIRS(GG)                                                           \
 	do {                                                                   \
@@ -938,12 +991,11 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		if (gr < nblocks)                                              \
 			hipLaunchKernelGGL((k_apply1_pairs<T, GG, true>), dim3(gr), \
 					   dim3(128 * GG), 0, st, src, dst, in->scal, \
-					   M, t, cm, acm, parts, nblocks, false);  \
+					   M, t, cm, acm, parts, nblocks);         \
 		else                                                           \
 			hipLaunchKernelGGL((k_apply1_pairs<T, GG, false>), dim3(gr), \
 					   dim3(128 * GG), 0, st, src, dst, in->scal, \
-					   M, t, cm, acm, parts, nblocks,          \
-					   t == 6 && nblocks % 8 == 0);            \
+					   M, t, cm, acm, parts, nblocks);         \
 	} while (0)
 			if (G == 4)
 				DK_PAIRS(4);
