@@ -131,19 +131,14 @@ __device__ __forceinline__ void halves_block(
 		partials[vb] = s;
 }
 
-template <typename T, bool LOOP>
+template <typename T>
 __global__ __launch_bounds__(64) void k_apply1_halves(
 	const T *in, T *out, const double *__restrict__ nrm, Mat2<T> M,
-	unsigned t, u64 cm, u64 acm, double *__restrict__ partials, u64 nvb)
+	unsigned t, u64 cm, u64 acm, double *__restrict__ partials)
 {
 	typedef typename real_of<T>::type R;
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
-	if constexpr (LOOP) {
-		for (u64 vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-			halves_block<T>(in, out, inv, M, t, cm, acm, partials, vb);
-	} else {
-		halves_block<T>(in, out, inv, M, t, cm, acm, partials, blockIdx.x);
-	}
+	halves_block<T>(in, out, inv, M, t, cm, acm, partials, blockIdx.x);
 }
 
 // t < 6: W waves per block, each over 64 consecutive amplitudes; the
@@ -972,17 +967,13 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		int G = g_ctx.apply1_group;
 		while (G > 1 && in->len < 128ull * G)
 			G >>= 1;
-		if (in->len >= 128 && t >= 6 && t <= 9) {
+		if (in->len >= 128 && t >= 6 && t <= 9 &&
+		    dk_grid_cap(in->len / 64, 64) == in->len / 64) {
+			// one launch covers the state (beyond 2^31 threads the
+			// looping pair kernel measured faster)
 			nblocks = in->len / 64;
-			const unsigned gr = dk_grid_cap(nblocks, 64);
-			if (gr < nblocks)
-				hipLaunchKernelGGL((k_apply1_halves<T, true>), dim3(gr), dim3(64), 0,
-						   st, src, dst, in->scal, M, t, cm, acm, parts,
-						   nblocks);
-			else
-				hipLaunchKernelGGL((k_apply1_halves<T, false>), dim3(gr), dim3(64),
-						   0, st, src, dst, in->scal, M, t, cm, acm, parts,
-						   nblocks);
+			hipLaunchKernelGGL((k_apply1_halves<T>), dim3((unsigned)nblocks), dim3(64),
+					   0, st, src, dst, in->scal, M, t, cm, acm, parts);
 		} else if (in->len >= 128 && t >= 6) {
 			nblocks = in->len / (128ull * G);
 #define DK_PAIRS(GG)                                                           \
