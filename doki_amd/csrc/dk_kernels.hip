@@ -99,11 +99,11 @@ __global__ __launch_bounds__(128 * G) void k_apply1_pairs(
 	}
 }
 
-// t = 6..9: one wave per block holds 32 pairs, lanes 0..31 the lower
+// t = 6..17: one wave per block holds 32 pairs, lanes 0..31 the lower
 // amplitudes (a contiguous 512-byte run) and lanes 32..63 their partners
 // 2^t above; the partner comes from lane ^ 32 (no LDS buffer, no barrier).
-// Measured at 30 qubits: 6.5-6.6 TB/s for these targets against 6.2-6.5
-// with the two-wave LDS exchange; slower than it for t >= 12 (5.9 at 20-21).
+// Measured at 30 qubits: 1-5% faster than the two-wave LDS exchange for
+// these targets (6.5-6.8 TB/s), slower from t = 18 on (5.9 TB/s at 20-21).
 template <typename T>
 __device__ __forceinline__ void halves_block(
 	const T *in, T *out, typename real_of<T>::type inv, const Mat2<T> &M,
@@ -967,7 +967,7 @@ int launch_apply_typed(const dk_state *in, dk_state *out, const dk_gate *g,
 		int G = g_ctx.apply1_group;
 		while (G > 1 && in->len < 128ull * G)
 			G >>= 1;
-		if (in->len >= 128 && t >= 6 && t <= 9 &&
+		if (in->len >= 128 && t >= 6 && t <= 17 &&
 		    dk_grid_cap(in->len / 64, 64) == in->len / 64) {
 			// one launch covers the state (beyond 2^31 threads the
 			// looping pair kernel measured faster)
