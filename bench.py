@@ -130,6 +130,39 @@ def gpu_leg(args):
     }
 
 
+def join_leg(n):
+    """A join to n qubits (registers of n/2 and n - n/2 qubits) followed by
+    the marginal probabilities of every qubit of the result: as a join and a
+    prob_all pass over the joined state, and in one pass (dk_join_prob_all:
+    the joined state written once and not read back)."""
+    import numpy as np
+
+    from doki_amd import capi
+    from doki_amd import workloads as W
+    a = capi.from_host(W.random_normalised_state(5, n // 2))
+    b = capi.from_host(W.random_normalised_state(6, n - n // 2))
+    two, one = [], []
+    for _ in range(3):
+        capi.event_record(0)
+        j = capi.join(a, b)
+        p2 = capi.prob_all(j)
+        capi.event_record(1)
+        two.append(capi.event_elapsed_ms(0, 1))
+        del j
+        capi.event_record(0)
+        j, p1 = capi.join_prob_all(a, b)
+        capi.event_record(1)
+        one.append(capi.event_elapsed_ms(0, 1))
+        del j
+    assert np.array_equal(p1, p2)
+    written = 16 << n
+    out = {"qubits": n, "join_then_prob_all_ms": round(min(two), 3),
+           "one_pass_ms": round(min(one), 3),
+           "one_pass_written_GBps": round(written / (min(one) * 1e-3) / 1e9, 1)}
+    log(f"[bench] join + marginals at {n} qubits: {out}")
+    return out
+
+
 def big_leg(args):
     """The 33-qubit variant of the brickwork workload (137.4 GB register):
     one layer through the drop-in module, out of place as registry_apply is
@@ -535,6 +568,7 @@ def main():
 
     g = gpu_leg(args)
     big = big_leg(args) if args.big_qubits else None
+    joins = join_leg(args.qubits + 2)
     fused = None if args.no_fused else fused_leg(args)
     batched = None if args.no_batch else batch_leg(args)
     served = None if args.no_serve else serve_leg(args)
@@ -580,6 +614,7 @@ def main():
             "timed_region_kernel_GBps_by_gate": g["kernel_GBps_timed_by_target"],
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "large_register": big,
+            "join_prob_all": joins,
             "fused": fused,
             "batched": batched,
             "served": served,

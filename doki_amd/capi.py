@@ -63,6 +63,7 @@ def lib():
         "dk_measure": (C.c_int, [P, u64, dp, C.c_size_t,
                                  C.POINTER(C.c_int8), dp, PP]),
         "dk_join": (C.c_int, [P, P, PP]),
+        "dk_join_prob_all": (C.c_int, [P, P, PP, dp]),
         "dk_first_nonzero": (C.c_int, [P, C.POINTER(u64)]),
         "dk_read_raw": (C.c_int, [P, u64, u64, dp, dp]),
         "dk_read": (C.c_int, [P, u64, u64, dp]),
@@ -246,6 +247,14 @@ def join(a, b):
     p = C.c_void_p()
     check(lib().dk_join(a.ptr, b.ptr, C.byref(p)))
     return State(p.value)
+
+
+def join_prob_all(a, b):
+    """join(a, b) and prob_all of the result, made in one pass over it."""
+    p = C.c_void_p()
+    out = np.empty(a.nq + b.nq)
+    check(lib().dk_join_prob_all(a.ptr, b.ptr, C.byref(p), _dp(out)))
+    return State(p.value), out
 
 
 def compare(a, b):

@@ -198,6 +198,10 @@ int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 		       dk_state *o, int next_q, int *nparts_out);
 
 int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o);
+// join into o and the marginal probabilities of o's qubits into dst (device,
+// o->nq doubles), in one pass where 9 <= o->nq <= 40
+int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
+			    double *dst);
 
 int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev);
 

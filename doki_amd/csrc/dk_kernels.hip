@@ -395,20 +395,69 @@ __global__ __launch_bounds__(TPB) void k_prob(
 		partials[blockIdx.x] = tot;
 }
 
-// All single-qubit probabilities in one pass.  A wave reads chunks of 64*U
+// All single-qubit probabilities in one pass.  A wave takes chunks of 64*U
 // consecutive amplitudes: index bits 0..5 are the lane, bits 6..6+LU-1 the
-// unroll slot, higher bits the chunk number.
+// unroll slot, higher bits the chunk number.  The amplitudes come from a
+// source: a stored state (ReadAmps), or the product amplitudes of a join,
+// stored as they are made (JoinAmps).
 constexpr int PQ = 40; // qubits covered by the one-pass kernel
 
-template <typename T, int LU>
+template <typename T> struct ReadAmps {
+	const T *in;
+	const double *nrm;
+	struct Ready {
+		const T *__restrict__ in;
+		double inv;
+		__device__ double2 operator()(u64 i) const
+		{
+			const T a = ld_stream(in + i);
+			return make_double2(a.x * inv, a.y * inv);
+		}
+	};
+	__device__ Ready ready() const { return {in, 1.0 / nrm[DK_SCAL_NORM]}; }
+};
+
+// r[i] = (a[i >> bbits] / |a|) (b[i & bmask] / |b|), written to o; the value
+// returned is the stored one (the joined state has norm 1).
+template <typename T> struct JoinAmps {
+	typedef typename real_of<T>::type R;
+	const T *a;
+	const double *na;
+	const T *b;
+	const double *nb;
+	T *o;
+	unsigned bbits;
+	struct Ready {
+		const T *__restrict__ a;
+		const T *__restrict__ b;
+		T *__restrict__ o;
+		R ia, ib;
+		unsigned bbits;
+		u64 bmask;
+		__device__ double2 operator()(u64 i) const
+		{
+			const T r = cmul(cscale(a[i >> bbits], ia),
+					 cscale(b[i & bmask], ib));
+			st_stream(o + i, r);
+			return make_double2(r.x, r.y);
+		}
+	};
+	__device__ Ready ready() const
+	{
+		return {a, b, o, (R)(1.0 / na[DK_SCAL_NORM]),
+			(R)(1.0 / nb[DK_SCAL_NORM]), bbits, (1ull << bbits) - 1};
+	}
+};
+
+template <int LU, typename Src>
 __global__ __launch_bounds__(TPB) void k_prob_all(
-	const T *__restrict__ in, const double *__restrict__ nrm, u64 len,
-	double *__restrict__ partials)
+	Src src, u64 len, double *__restrict__ partials,
+	double *__restrict__ set_norm)
 {
 	constexpr int U = 1 << LU;
 	constexpr int NH = PQ - 6 - LU;
 	__shared__ double red[WAVES][PQ];
-	const double inv = 1.0 / nrm[DK_SCAL_NORM];
+	const auto amp = src.ready();
 	const int lane = threadIdx.x & 63;
 	const u64 wid = (u64)blockIdx.x * WAVES + (threadIdx.x >> 6);
 	const u64 nw = (u64)gridDim.x * WAVES;
@@ -422,12 +471,11 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 		acch[b] = 0.0;
 	for (u64 c = wid; c < nch; c += nw) {
 		double x[U];
-		const T *p = in + (c << (6 + LU)) + lane;
+		const u64 first = (c << (6 + LU)) + lane;
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
-			const T a = ld_stream(p + (u64)u * 64);
-			const double re = a.x * inv, im = a.y * inv;
-			x[u] = fma(re, re, im * im);
+			const double2 a = amp(first + (u64)u * 64);
+			x[u] = fma(a.x, a.x, a.y * a.y);
 		}
 		double s = 0.0;
 #pragma unroll
@@ -469,6 +517,8 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 			s += red[ww][threadIdx.x];
 		partials[(size_t)blockIdx.x * PQ + threadIdx.x] = s;
 	}
+	if (set_norm && blockIdx.x == 0 && threadIdx.x == 0)
+		set_norm[DK_SCAL_NORM] = 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1214,9 +1264,27 @@ int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
 	return DK_OK;
 }
 
+// Grid of the one-pass marginals kernel over 2^nq amplitudes: four times the
+// usual grid cap, for more chunk loads in flight.  The join-and-marginals
+// launch uses the same grid, so its sums are made in the same order.
+static unsigned prob_all_grid(u64 len, int lu)
+{
+	const u64 nch = len >> (6 + lu);
+	return (unsigned)std::min<u64>(std::max<u64>(1, (nch + WAVES - 1) / WAVES),
+				       4ull * grid_for(~0ull >> 1, 1));
+}
+
+static void launch_finish_marginals(unsigned nq, unsigned nb, double *dst)
+{
+	hipLaunchKernelGGL(k_finish_cols, dim3(nq), dim3(TPB), 0, DK_STREAM,
+			   dk_cur()->partials, (int)nb, PQ, dst);
+}
+
+constexpr int PROB_LU = 3;
+
 int dk_launch_prob_all(const dk_state *s, double *dst)
 {
-	constexpr int LU = 3;
+	constexpr int LU = PROB_LU;
 	if (s->nq < 6 + LU || s->nq > PQ) {
 		for (unsigned q = 0; q < s->nq; ++q) {
 			int rc = dk_launch_prob(s, q, dst + q);
@@ -1225,20 +1293,39 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 		}
 		return DK_OK;
 	}
-	const u64 nch = s->len >> (6 + LU);
-	// four times the usual grid cap: more chunk loads in flight
-	const unsigned nb = (unsigned)std::min<u64>(
-		std::max<u64>(1, (nch + WAVES - 1) / WAVES),
-		4ull * grid_for(~0ull >> 1, 1));
+	const unsigned nb = prob_all_grid(s->len, LU);
 #define L(T)                                                                  \
-	hipLaunchKernelGGL((k_prob_all<T, LU>), dim3(nb), dim3(TPB), 0,        \
-			    DK_STREAM, (const T *)s->amp, s->scal, s->len,       \
-			    dk_cur()->partials)
+	hipLaunchKernelGGL((k_prob_all<LU, ReadAmps<T>>), dim3(nb), dim3(TPB), \
+			   0, DK_STREAM, ReadAmps<T>{(const T *)s->amp, s->scal}, \
+			   s->len, dk_cur()->partials, (double *)nullptr)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
-	hipLaunchKernelGGL(k_finish_cols, dim3(s->nq), dim3(TPB), 0, DK_STREAM,
-			   dk_cur()->partials, (int)nb, PQ, dst);
+	launch_finish_marginals(s->nq, nb, dst);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
+			    double *dst)
+{
+	constexpr int LU = PROB_LU;
+	if (o->nq < 6 + LU || o->nq > PQ) {
+		int rc = dk_launch_join(a, b, o);
+		return rc ? rc : dk_launch_prob_all(o, dst);
+	}
+	const unsigned nb = prob_all_grid(o->len, LU);
+#define L(T)                                                                  \
+	hipLaunchKernelGGL((k_prob_all<LU, JoinAmps<T>>), dim3(nb), dim3(TPB), \
+			   0, DK_STREAM,                                       \
+			   JoinAmps<T>{(const T *)a->amp, a->scal,              \
+				       (const T *)b->amp, b->scal, (T *)o->amp, \
+				       b->nq},                                  \
+			   o->len, dk_cur()->partials, o->scal)
+	DK_BY_TYPE(o, L);
+#undef L
+	DK_LAUNCHED();
+	launch_finish_marginals(o->nq, nb, dst);
 	DK_LAUNCHED();
 	return DK_OK;
 }

@@ -795,6 +795,36 @@ extern "C" int dk_join(const dk_state *most, const dk_state *least,
 	return DK_OK;
 }
 
+extern "C" int dk_join_prob_all(const dk_state *most, const dk_state *least,
+				dk_state **out, double *p)
+{
+	DK_INIT();
+	if (!most || !least || !out || !p) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	if (most->dtype != least->dtype) {
+		dk_set_error("joined states must share an amplitude type");
+		return DK_E_ARG;
+	}
+	DK_SETTLE(most);
+	DK_SETTLE(least);
+	dk_state *o;
+	int rc = dk_alloc_state(most->nq + least->nq, most->dtype, &o);
+	if (rc)
+		return rc;
+	double *dst = dk_cur()->dec + DEC_PROBS;
+	rc = dk_launch_join_prob_all(most, least, o, dst);
+	if (!rc)
+		rc = copy_to_host(p, dst, o->nq);
+	if (rc) {
+		dk_release_state(o);
+		return rc;
+	}
+	*out = o;
+	return DK_OK;
+}
+
 extern "C" int dk_read_raw(const dk_state *s, uint64_t first, uint64_t count,
 			   double *values, double *norm)
 {

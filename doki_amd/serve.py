@@ -13,7 +13,8 @@ are admitted by the device memory their states need, against a byte budget:
   requests run on the device at once.  Their circuits run in place as
   cache-blocked passes (or fused dense blocks, as the tuned plan says), and
   their states stay resident on the device from creation to the final
-  operation.
+  operation.  A join and the marginals read from it are made in one pass
+  (the joined state is written once and not read back).
 
 ctypes calls release the GIL, so host threads overlap while the device works.
 Every request's result depends only on the request: the batched engine runs
@@ -109,8 +110,10 @@ def build_register(nq, packed, dtype, plan="auto"):
     return st
 
 
-def run_operation(spec, states, dtype):
-    """The request's final operation on its resident register(s)."""
+def run_operation(spec, states, dtype, one_pass_join=False):
+    """The request's final operation on its resident register(s).  With
+    one_pass_join a join and its marginals are made in one pass over the
+    joined state (dk_join_prob_all), else as a join and then prob_all."""
     op = spec["op"]
     st = states[0]
     if op == "prob":
@@ -122,6 +125,9 @@ def run_operation(spec, states, dtype):
     if op == "density":
         return {"rho": capi.reduced_density(st, spec["keep"])}
     if op == "join":
+        if one_pass_join:
+            _, probs = capi.join_prob_all(states[0], states[1])
+            return {"probs": probs}
         j = capi.join(states[0], states[1])
         return {"probs": capi.prob_all(j)}
     raise ValueError("unknown operation %r" % op)
@@ -272,7 +278,7 @@ class Server:
     def _run_large(self, h):
         states = [build_register(n, pk, self.dtype, self.plan)
                   for (n, _), pk in zip(h.spec["regs"], h.packed)]
-        res = run_operation(h.spec, states, self.dtype)
+        res = run_operation(h.spec, states, self.dtype, one_pass_join=True)
         del states  # freed on this thread's lane
         return res
 

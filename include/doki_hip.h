@@ -186,6 +186,16 @@ int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
  * norm 1.  Replaces join, qops.c:97-122 (registry_join, doki.c:943-1021). */
 int dk_join(const dk_state *most, const dk_state *least, dk_state **out);
 
+/* dk_join and the marginal probabilities of every qubit of the joined state
+ * in one pass over it: *out as dk_join gives it, p[q] = P(qubit q = 1) as
+ * dk_prob_all gives it for *out (p holds most.nq + least.nq doubles).  The
+ * state is written once and not read back.  No counterpart in the reference,
+ * where a caller joins (registry_join, doki.c:943-1021) and then calls
+ * registry_prob (doki.c:1194-1228) once per qubit; for servers and circuit
+ * runners that join registers to read their marginals. */
+int dk_join_prob_all(const dk_state *most, const dk_state *least,
+		     dk_state **out, double *p);
+
 /* Index of the first amplitude whose raw value is nonzero (2^n if none).
  * Device half of get_global_phase, qops.c:46-70. */
 int dk_first_nonzero(const dk_state *s, uint64_t *index);

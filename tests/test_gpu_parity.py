@@ -364,6 +364,30 @@ def test_join_vs_oracle(capi, oracle):
         assert maxabs(d.values(), o.values()) <= ATOL
 
 
+@pytest.mark.parametrize("na,nb", [(3, 4), (4, 5), (1, 12), (12, 1), (9, 10), (14, 9)])
+def test_join_prob_all_equals_join_then_prob_all(capi, oracle, na, nb):
+    """The one-pass join with marginals writes the state dk_join writes and
+    the probabilities dk_prob_all reads from it, bit for bit (the one-pass
+    kernel from 9 joined qubits up, two launches below), in both amplitude
+    types; and matches the oracle's join and per-qubit probability."""
+    a = W.random_normalised_state(70 + na, na) * 0.5
+    b = W.random_normalised_state(90 + nb, nb) * 3.0
+    for dtype in (capi.C128, capi.C64):
+        da, db = capi.from_host(a), capi.from_host(b)
+        if dtype == capi.C64:
+            da, db = capi.convert(da, dtype), capi.convert(db, dtype)
+        j1 = capi.join(da, db)
+        p1 = capi.prob_all(j1)
+        j2, p2 = capi.join_prob_all(da, db)
+        assert j2.nq == na + nb and abs(j2.norm - 1.0) == 0.0
+        assert np.array_equal(j2.raw()[0], j1.raw()[0]), dtype
+        assert np.array_equal(p2, p1), (dtype, np.max(np.abs(p2 - p1)))
+    o = oracle.join(oracle.from_data(na, a), oracle.from_data(nb, b))
+    want = np.array([oracle.prob(o, q) for q in range(na + nb)])
+    _, p = capi.join_prob_all(capi.from_host(a), capi.from_host(b))
+    assert maxabs(p, want) <= ATOL
+
+
 @pytest.mark.parametrize("n,keep", [(5, [0, 1, 2, 3, 4]), (6, [1, 4]),
                                     (14, [0, 3, 5, 8, 11, 13]), (10, []),
                                     (9, [8]), (16, [2, 3, 4, 5, 6, 7, 8, 9, 10, 11])])
