@@ -449,27 +449,34 @@ template <typename T> struct JoinAmps {
 	}
 };
 
+// The grid holds 2^wbits waves and wave w takes chunks c = w + k 2^wbits,
+// k = 0, 1, ...: the chunk bits below wbits are the wave's own and settle
+// at the end; the loop keeps sums only for the bits of k (fewer than KB, as
+// the launch guarantees).
+constexpr int PROB_KB = 10;
+
 template <int LU, typename Src>
 __global__ __launch_bounds__(TPB) void k_prob_all(
-	Src src, u64 len, double *__restrict__ partials,
+	Src src, u64 len, unsigned wbits, double *__restrict__ partials,
 	double *__restrict__ set_norm)
 {
 	constexpr int U = 1 << LU;
+	constexpr int KB = PROB_KB;
 	constexpr int NH = PQ - 6 - LU;
 	__shared__ double red[WAVES][PQ];
 	const auto amp = src.ready();
 	const int lane = threadIdx.x & 63;
 	const u64 wid = (u64)blockIdx.x * WAVES + (threadIdx.x >> 6);
-	const u64 nw = (u64)gridDim.x * WAVES;
 	const u64 nch = len >> (6 + LU);
-	double tot = 0.0, accu[LU], acch[NH];
+	double tot = 0.0, accu[LU], acck[KB];
 #pragma unroll
 	for (int b = 0; b < LU; ++b)
 		accu[b] = 0.0;
 #pragma unroll
-	for (int b = 0; b < NH; ++b)
-		acch[b] = 0.0;
-	for (u64 c = wid; c < nch; c += nw) {
+	for (int b = 0; b < KB; ++b)
+		acck[b] = 0.0;
+	unsigned k = 0;
+	for (u64 c = wid; c < nch; c += 1ull << wbits, ++k) {
 		double x[U];
 		const u64 first = (c << (6 + LU)) + lane;
 #pragma unroll
@@ -492,22 +499,35 @@ __global__ __launch_bounds__(TPB) void k_prob_all(
 		}
 		tot += s;
 #pragma unroll
-		for (int b = 0; b < NH; ++b)
-			acch[b] += ((c >> b) & 1) ? s : 0.0;
+		for (int b = 0; b < KB; ++b)
+			if ((k >> b) & 1) // the same for the whole wave
+				acck[b] += s;
 	}
 	const int w = threadIdx.x >> 6;
+	const double all = wave_sum(tot);
 #pragma unroll
-	for (int q = 0; q < PQ; ++q) {
-		double v;
-		if (q < 6)
-			v = ((lane >> q) & 1) ? tot : 0.0;
-		else if (q < 6 + LU)
-			v = accu[q - 6];
-		else
-			v = acch[q - 6 - LU];
-		v = wave_sum(v);
+	for (int q = 0; q < 6; ++q) {
+		const double v = wave_sum(((lane >> q) & 1) ? tot : 0.0);
 		if (lane == 0)
 			red[w][q] = v;
+	}
+#pragma unroll
+	for (int b = 0; b < LU; ++b) {
+		const double v = wave_sum(accu[b]);
+		if (lane == 0)
+			red[w][6 + b] = v;
+	}
+	// chunk bits: below wbits from the wave index, then k's bits, then 0
+	for (int b = lane; b < NH; b += 64)
+		if (b < (int)wbits)
+			red[w][6 + LU + b] = ((wid >> b) & 1) ? all : 0.0;
+		else if (b >= (int)wbits + KB)
+			red[w][6 + LU + b] = 0.0;
+#pragma unroll
+	for (int j = 0; j < KB; ++j) {
+		const double v = wave_sum(acck[j]);
+		if (lane == 0 && (int)wbits + j < NH)
+			red[w][6 + LU + wbits + j] = v;
 	}
 	__syncthreads();
 	if (threadIdx.x < PQ) {
@@ -1265,19 +1285,26 @@ int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
 }
 
 // Grid of the one-pass marginals kernel over 2^nq amplitudes: four times the
-// usual grid cap, for more chunk loads in flight.  The join-and-marginals
-// launch uses the same grid, so its sums are made in the same order.
-static unsigned prob_all_grid(u64 len, int lu)
+// usual grid cap, for more chunk loads in flight, and at least enough waves
+// that none takes 2^PROB_KB chunks; a power of two, 2^wbits waves.  The
+// join-and-marginals launch uses the same grid, so its sums are made in the
+// same order.
+static unsigned prob_all_grid(u64 len, int lu, unsigned *wbits)
 {
+	static_assert((WAVES & (WAVES - 1)) == 0, "waves per block: a power of two");
 	const u64 nch = len >> (6 + lu);
-	return (unsigned)std::min<u64>(std::max<u64>(1, (nch + WAVES - 1) / WAVES),
-				       4ull * grid_for(~0ull >> 1, 1));
+	u64 nb = std::min<u64>(std::max<u64>(1, nch / WAVES),
+			       4ull * grid_for(~0ull >> 1, 1));
+	nb = std::max<u64>(nb, (nch >> PROB_KB) / WAVES);
+	*wbits = (unsigned)__builtin_ctzll(nb * WAVES);
+	return (unsigned)nb;
 }
 
-static void launch_finish_marginals(unsigned nq, unsigned nb, double *dst)
+static void launch_finish_marginals(const double *parts, unsigned nq,
+				    unsigned nb, double *dst)
 {
 	hipLaunchKernelGGL(k_finish_cols, dim3(nq), dim3(TPB), 0, DK_STREAM,
-			   dk_cur()->partials, (int)nb, PQ, dst);
+			   parts, (int)nb, PQ, dst);
 }
 
 constexpr int PROB_LU = 3;
@@ -1293,15 +1320,18 @@ int dk_launch_prob_all(const dk_state *s, double *dst)
 		}
 		return DK_OK;
 	}
-	const unsigned nb = prob_all_grid(s->len, LU);
+	unsigned wbits;
+	const unsigned nb = prob_all_grid(s->len, LU, &wbits);
+	double *parts;
+	DK_TRY(dk_partials_reserve((u64)nb * PQ, &parts));
 #define L(T)                                                                  \
 	hipLaunchKernelGGL((k_prob_all<LU, ReadAmps<T>>), dim3(nb), dim3(TPB), \
 			   0, DK_STREAM, ReadAmps<T>{(const T *)s->amp, s->scal}, \
-			   s->len, dk_cur()->partials, (double *)nullptr)
+			   s->len, wbits, parts, (double *)nullptr)
 	DK_BY_TYPE(s, L);
 #undef L
 	DK_LAUNCHED();
-	launch_finish_marginals(s->nq, nb, dst);
+	launch_finish_marginals(parts, s->nq, nb, dst);
 	DK_LAUNCHED();
 	return DK_OK;
 }
@@ -1314,18 +1344,21 @@ int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
 		int rc = dk_launch_join(a, b, o);
 		return rc ? rc : dk_launch_prob_all(o, dst);
 	}
-	const unsigned nb = prob_all_grid(o->len, LU);
+	unsigned wbits;
+	const unsigned nb = prob_all_grid(o->len, LU, &wbits);
+	double *parts;
+	DK_TRY(dk_partials_reserve((u64)nb * PQ, &parts));
 #define L(T)                                                                  \
 	hipLaunchKernelGGL((k_prob_all<LU, JoinAmps<T>>), dim3(nb), dim3(TPB), \
 			   0, DK_STREAM,                                       \
 			   JoinAmps<T>{(const T *)a->amp, a->scal,              \
 				       (const T *)b->amp, b->scal, (T *)o->amp, \
 				       b->nq},                                  \
-			   o->len, dk_cur()->partials, o->scal)
+			   o->len, wbits, parts, o->scal)
 	DK_BY_TYPE(o, L);
 #undef L
 	DK_LAUNCHED();
-	launch_finish_marginals(o->nq, nb, dst);
+	launch_finish_marginals(parts, o->nq, nb, dst);
 	DK_LAUNCHED();
 	return DK_OK;
 }
