@@ -79,6 +79,9 @@ __device__ __forceinline__ void dense_tile(
 	const int rt = wv % RT, cw = wv / RT;
 	const int li = lane & 15, lk = lane >> 4;
 	const u64 base = tile_base(tile, ta);
+	// complex128 k = 4 tiles load straight into LDS (measured on gfx950 at
+	// 30 qubits: 6.39 against 6.67 ms; k = 5 and 6 ran 4% and 23% slower)
+	constexpr bool LDS_DIRECT = sizeof(T) == 16 && K == 4;
 
 	// Slot s = threadIdx.x + j * NT: its index bits are this thread's part
 	// (computed once) OR the part of j * NT (the same for the whole block).
@@ -86,10 +89,33 @@ __device__ __forceinline__ void dense_tile(
 	const u64 tpart = base | ta.dep_lo[threadIdx.x & 63] |
 			  ta.dep_hi[threadIdx.x >> 6];
 	// issue the tile's loads first, then set up fragments while they fly
+	if constexpr (LDS_DIRECT) {
+		// Straight into LDS (global_load_lds_dwordx4): a wave fills 64
+		// consecutive positions per step, lane l position p = its slot
+		// index.  Position p holds slot lds_pos(p) (lds_pos is its own
+		// inverse and keeps bits >= 4), whose index differs from slot
+		// p's by the deposit of the 4-bit correction x; the deposit is
+		// linear, so it is XORed in from the deposits of bits 0..3.
+		const u64 d1 = ta.dep_lo[1], d2 = ta.dep_lo[2], d4 = ta.dep_lo[4],
+			  d8 = ta.dep_lo[8];
 #pragma unroll
-	for (int j = 0; j < PER; ++j) {
-		const int s = threadIdx.x + j * NT;
-		lds[lds_pos<T>(s)] = in[tpart | ta.dep_hi[(j * NT) >> 6]];
+		for (int j = 0; j < PER; ++j) {
+			const int p = threadIdx.x + j * NT;
+			const int x = ((p >> 4) ^ (p >> 8)) & 15;
+			const u64 fix = ((x & 1) ? d1 : 0) ^ ((x & 2) ? d2 : 0) ^
+					((x & 4) ? d4 : 0) ^ ((x & 8) ? d8 : 0);
+			const u64 gi = (tpart | ta.dep_hi[(j * NT) >> 6]) ^ fix;
+			__builtin_amdgcn_global_load_lds(
+				(__attribute__((address_space(1))) void *)(in + gi),
+				(__attribute__((address_space(3))) void *)(lds + (p & ~63)),
+				16, 0, 0);
+		}
+	} else {
+#pragma unroll
+		for (int j = 0; j < PER; ++j) {
+			const int s = threadIdx.x + j * NT;
+			lds[lds_pos<T>(s)] = in[tpart | ta.dep_hi[(j * NT) >> 6]];
+		}
 	}
 
 	// A fragments: this wave's 16 rows of M, all k-steps
@@ -110,6 +136,8 @@ __device__ __forceinline__ void dense_tile(
 	for (int ks = 0; ks < KS; ++ks)
 		ks_slot[ks] = ta.colslot[ks * 4 + lk];
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	if constexpr (LDS_DIRECT)
+		__builtin_amdgcn_s_waitcnt(0); // this wave's LDS loads have landed
 	__syncthreads();
 
 	acc_t cr[4], ci[4];
@@ -180,13 +208,16 @@ __global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 }
 
 // Column sets per block for k = 4 (tunable "dense_k4_cw": 1, 2 or 4; 0
-// picks 2 for complex128 and 1 for complex64, the faster on gfx950 at 30
-// qubits: 6.2 against 10.7 ms, 3.3 against 3.4 ms).
+// picks 1, the fastest on gfx950 at 30 qubits averaged over six target
+// positions: complex128 6.43 / 6.62 / 7.36 ms for 1 / 2 / 4 sets, complex64
+// 3.36 / 3.45 / 3.58 ms.  Before complex128 tiles were loaded straight into
+// LDS, two sets were the faster there: 6.2 against 10.7 ms.)
 int k4_cw(int dtype)
 {
+	(void)dtype;
 	if (g_ctx.dense_k4_cw)
 		return g_ctx.dense_k4_cw;
-	return dtype == DK_C64 ? 1 : 2;
+	return 1;
 }
 
 template <typename T, int K, int CW>
