@@ -11,7 +11,14 @@
 // in a device pool and run as cache-blocked passes: a pass is a run of
 // consecutive gates whose targets fit one 2^LB-amplitude tile; the workgroup
 // streams the register through LDS tile by tile, applying the whole run to
-// each tile (controls outside the tile are decided per tile).
+// each tile (controls outside the tile are decided per tile).  Registers of
+// at least "split_qubits" qubits (17 by default) would hold one workgroup
+// for long (a 2^20-amplitude register streams at about one CU's share of
+// HBM), so their passes run before the persistent launch as rounds: round r
+// is one launch in which every workgroup takes tiles of the r-th pass of
+// every such register, the launch boundary ordering the passes; the
+// persistent launch then only reads their final states.  Each tile is
+// computed as in the one-workgroup form, so the results are the same.
 //
 // Gate semantics are registry_apply's (qops.c:176-261); since every gate is
 // unitary, renormalising once after the last gate gives the reference's
@@ -58,7 +65,15 @@ struct BGate {
 	unsigned pad;
 };
 
-enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4, BF_PROBS = 8 };
+enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4, BF_PROBS = 8, BF_SPLIT = 16 };
+
+// One register's pass in a round: its tiles are the round's items
+// first .. first + 2^(nq - LB) - 1.
+struct BRound {
+	u64 first;
+	unsigned reg;
+	unsigned pass;
+};
 
 __device__ __forceinline__ double bsum(double v, double *red)
 {
@@ -82,34 +97,109 @@ __device__ __forceinline__ u64 deposit(u64 x, const unsigned char *pos, int n)
 	return r;
 }
 
-// Applies one 1-qubit gate to the tile (2^tb slots) whose slot bit b is
-// index bit pos[b]; `base` holds the tile's fixed index bits.
+// A gate as one tile sees it: its matrix, the in-tile control masks (slot
+// bits) and whether the out-of-tile controls leave it on for this tile.
+template <typename T> struct TileGate {
+	T m00, m01, m10, m11;
+	u64 cm_in, am_in;
+	int sb;
+	bool on;
+};
+
 template <typename T>
-__device__ void tile_gate(T *tile, int tb, const BGate &g, const BPass &ps,
-			  u64 base)
+__device__ __forceinline__ TileGate<T> tile_gate_of(const BGate &g, const BPass &ps,
+						     u64 base)
 {
 	typedef typename real_of<T>::type R;
-	// controls and anticontrols outside the tile decide the whole tile
-	u64 cm_in = 0, am_in = 0;
-	const u64 cm_out = g.cm & ~ps.pmask, am_out = g.am & ~ps.pmask;
+	TileGate<T> c;
+	c.cm_in = c.am_in = 0;
 	for (u64 m = g.cm & ps.pmask; m; m &= m - 1)
-		cm_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
+		c.cm_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
 	for (u64 m = g.am & ps.pmask; m; m &= m - 1)
-		am_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
-	if ((base & cm_out) != cm_out || (base & am_out) != 0)
+		c.am_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
+	const u64 cm_out = g.cm & ~ps.pmask, am_out = g.am & ~ps.pmask;
+	c.on = (base & cm_out) == cm_out && (base & am_out) == 0;
+	c.sb = ps.slot_of[g.t];
+	c.m00 = mk<T>((R)g.m[0], (R)g.m[1]);
+	c.m01 = mk<T>((R)g.m[2], (R)g.m[3]);
+	c.m10 = mk<T>((R)g.m[4], (R)g.m[5]);
+	c.m11 = mk<T>((R)g.m[6], (R)g.m[7]);
+	return c;
+}
+
+// The gate on the pair (s0, s0 | 2^sb) when s0 meets its in-tile controls.
+template <typename T>
+__device__ __forceinline__ void pair_apply(T *tile, const TileGate<T> &c, int s0)
+{
+	if (((u64)s0 & c.cm_in) != c.cm_in || ((u64)s0 & c.am_in) != 0)
 		return;
-	const int sb = ps.slot_of[g.t];
-	const T m00 = mk<T>((R)g.m[0], (R)g.m[1]), m01 = mk<T>((R)g.m[2], (R)g.m[3]);
-	const T m10 = mk<T>((R)g.m[4], (R)g.m[5]), m11 = mk<T>((R)g.m[6], (R)g.m[7]);
-	const int half = 1 << (tb - 1);
-	for (int p = threadIdx.x; p < half; p += BT) {
-		const int s0 = (int)ins0((u64)p, (unsigned)sb);
-		if (((u64)s0 & cm_in) != cm_in || ((u64)s0 & am_in) != 0)
-			continue;
-		const int s1 = s0 | (1 << sb);
-		const T x = tile[lds_pos<T>(s0)], y = tile[lds_pos<T>(s1)];
-		tile[lds_pos<T>(s0)] = cmac(cmac(czero<T>(), m00, x), m01, y);
-		tile[lds_pos<T>(s1)] = cmac(cmac(czero<T>(), m10, x), m11, y);
+	const int s1 = s0 | (1 << c.sb);
+	const T x = tile[lds_pos<T>(s0)], y = tile[lds_pos<T>(s1)];
+	tile[lds_pos<T>(s0)] = cmac(cmac(czero<T>(), c.m00, x), c.m01, y);
+	tile[lds_pos<T>(s1)] = cmac(cmac(czero<T>(), c.m10, x), c.m11, y);
+}
+
+// Two consecutive gates on distinct slot bits a, b in one sweep: a thread
+// owns the four amplitudes s, s|A, s|B, s|A|B and applies the first gate to
+// its two a-pairs, then the second to its two b-pairs -- each amplitude sees
+// the same operations in the same order as with one sweep per gate.
+template <typename T>
+__device__ void tile_gate_pair(T *tile, int tb, const TileGate<T> &c1,
+			       const TileGate<T> &c2)
+{
+	const int lo = c1.sb < c2.sb ? c1.sb : c2.sb;
+	const int hi = c1.sb < c2.sb ? c2.sb : c1.sb;
+	const int A = 1 << c1.sb, B = 1 << c2.sb;
+	const int quads = 1 << (tb - 2);
+	for (int q = threadIdx.x; q < quads; q += BT) {
+		const int s = (int)ins0(ins0((u64)q, (unsigned)lo), (unsigned)hi);
+		if (c1.on) {
+			pair_apply(tile, c1, s);
+			pair_apply(tile, c1, s | B);
+		}
+		if (c2.on) {
+			pair_apply(tile, c2, s);
+			pair_apply(tile, c2, s | A);
+		}
+	}
+}
+
+// Every gate of pass ps on a tile of 2^tb slots (all threads; ends with a
+// barrier).  Gate records are staged in shared memory GCH at a time, and
+// consecutive gates on distinct slot bits go two per sweep.
+constexpr int GCH = 32;
+
+template <typename T>
+__device__ void tile_pass(T *tile, int tb, const BPass &ps, u64 base,
+			  const BGate *__restrict__ gates, BGate *gsh)
+{
+	// the gate range in registers: after the last barrier below, thread 0
+	// may already be writing the next pass into `ps`
+	const unsigned gb = ps.gate_begin, ge = ps.gate_end;
+	for (unsigned g0 = gb; g0 < ge; g0 += GCH) {
+		const int cnt = (int)min((unsigned)GCH, ge - g0);
+		for (int i = threadIdx.x; i < cnt; i += BT)
+			gsh[i] = gates[g0 + i];
+		__syncthreads();
+		for (int i = 0; i < cnt;) {
+			const TileGate<T> c1 = tile_gate_of<T>(gsh[i], ps, base);
+			if (i + 1 < cnt && tb >= 2) {
+				const TileGate<T> c2 = tile_gate_of<T>(gsh[i + 1], ps, base);
+				if (c2.sb != c1.sb) {
+					tile_gate_pair(tile, tb, c1, c2);
+					__syncthreads();
+					i += 2;
+					continue;
+				}
+			}
+			if (c1.on) {
+				const int half = 1 << (tb - 1);
+				for (int p = threadIdx.x; p < half; p += BT)
+					pair_apply(tile, c1, (int)ins0((u64)p, (unsigned)c1.sb));
+			}
+			__syncthreads();
+			i += 1;
+		}
 	}
 }
 
@@ -136,7 +226,7 @@ __global__ __launch_bounds__(BT) void k_batch(
 	__shared__ double red[BW];
 	__shared__ unsigned item;
 	__shared__ BPass ps;
-	__shared__ BGate gs;
+	__shared__ BGate gsh[GCH];
 
 	for (;;) {
 		if (threadIdx.x == 0)
@@ -164,15 +254,9 @@ __global__ __launch_bounds__(BT) void k_batch(
 				if (threadIdx.x == 0)
 					ps = passes[pi];
 				__syncthreads();
-				for (unsigned gi = ps.gate_begin; gi < ps.gate_end; ++gi) {
-					if (threadIdx.x == 0)
-						gs = gates[gi];
-					__syncthreads();
-					tile_gate(tile, (int)n, gs, ps, 0);
-					__syncthreads();
-				}
+				tile_pass(tile, (int)n, ps, 0, gates, gsh);
 			}
-		} else {
+		} else if (!(R.flags & BF_SPLIT)) {
 			for (u64 i = threadIdx.x; i < len; i += BT) {
 				T v = czero<T>();
 				if (i == 0)
@@ -193,13 +277,7 @@ __global__ __launch_bounds__(BT) void k_batch(
 						tile[lds_pos<T>(s)] =
 							st[base | deposit((u64)s, ps.pos, ps.npos)];
 					__syncthreads();
-					for (unsigned gi = ps.gate_begin; gi < ps.gate_end; ++gi) {
-						if (threadIdx.x == 0)
-							gs = gates[gi];
-						__syncthreads();
-						tile_gate(tile, LB, gs, ps, base);
-						__syncthreads();
-					}
+					tile_pass(tile, LB, ps, base, gates, gsh);
 					for (int s = threadIdx.x; s < LT; s += BT)
 						st[base | deposit((u64)s, ps.pos, ps.npos)] =
 							tile[lds_pos<T>(s)];
@@ -208,12 +286,15 @@ __global__ __launch_bounds__(BT) void k_batch(
 			}
 		}
 
-		// squared norm of the final state
-		double acc = 0.0;
-		for (u64 i = threadIdx.x; i < len; i += BT)
-			acc += cabs2d(amp_at(tile, st, resident, i));
-		const double norm2 = bsum(acc, red);
-		const double inv = 1.0 / sqrt(norm2);
+		// squared norm of the final state (marginals and densities divide
+		// by it; the measurement's P1 is a ratio of its own sums)
+		double norm2 = 1.0;
+		if (R.flags & (BF_PROBS | BF_DENSITY)) {
+			double acc = 0.0;
+			for (u64 i = threadIdx.x; i < len; i += BT)
+				acc += cabs2d(amp_at(tile, st, resident, i));
+			norm2 = bsum(acc, red);
+		}
 
 		if (R.flags & BF_PROBS) {
 			for (unsigned q = 0; q < n; ++q) {
@@ -270,9 +351,24 @@ __global__ __launch_bounds__(BT) void k_batch(
 				const unsigned q = n - 1 - step;
 				const u64 start = prefix << (q + 1), half = 1ull << q;
 				double lo = 0.0, hi = 0.0;
-				for (u64 i = threadIdx.x; i < half; i += BT) {
-					lo += cabs2d(amp_at(tile, st, resident, start + i));
-					hi += cabs2d(amp_at(tile, st, resident, start + half + i));
+				// four index pairs per thread per step: more loads in
+				// flight for pooled registers
+				for (u64 i0 = threadIdx.x; i0 < half; i0 += 4 * BT) {
+					T x[4], y[4];
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						const u64 i = i0 + (u64)u * BT;
+						x[u] = y[u] = czero<T>();
+						if (i < half) {
+							x[u] = amp_at(tile, st, resident, start + i);
+							y[u] = amp_at(tile, st, resident, start + half + i);
+						}
+					}
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						lo += cabs2d(x[u]);
+						hi += cabs2d(y[u]);
+					}
 				}
 				const double slo = bsum(lo, red);
 				const double shi = bsum(hi, red);
@@ -285,7 +381,60 @@ __global__ __launch_bounds__(BT) void k_batch(
 				prefix = (prefix << 1) | (v ? 1u : 0u);
 			}
 		}
-		(void)inv;
+		__syncthreads();
+	}
+}
+
+// One round of the split registers' passes: item `it` is tile it - first of
+// the entry holding it (entries ascend by first); workgroups take items
+// blockIdx.x, blockIdx.x + gridDim.x, ... (measured faster on gfx950 than
+// contiguous runs per workgroup: 4096 registers 21.8 against 27.1 ms).
+// `init`: the first round, whose tiles start from |0...0> instead of the
+// pool.
+template <typename T>
+__global__ __launch_bounds__(BT) void k_batch_round(
+	const BReg *__restrict__ regs, const BPass *__restrict__ passes,
+	const BGate *__restrict__ gates, T *__restrict__ pool,
+	const BRound *__restrict__ ents, unsigned nents, u64 nitems, int init)
+{
+	constexpr int LT = LDS_BYTES / (int)sizeof(T);
+	constexpr int LB = sizeof(T) == 16 ? 12 : 13;
+	__shared__ T tile[LT];
+	__shared__ BPass ps;
+	__shared__ BGate gsh[GCH];
+	__shared__ unsigned ent;
+
+	for (u64 it = blockIdx.x; it < nitems; it += gridDim.x) {
+		if (threadIdx.x == 0) {
+			unsigned lo = 0, hi = nents - 1;
+			while (lo < hi) {
+				const unsigned mid = (lo + hi + 1) / 2;
+				if (ents[mid].first <= it)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			ent = lo;
+			ps = passes[ents[lo].pass];
+		}
+		__syncthreads();
+		const BRound E = ents[ent];
+		T *st = pool + regs[E.reg].off;
+		u64 base = it - E.first;
+		for (int b = 0; b < (int)ps.npos; ++b)
+			base = ins0(base, ps.pos[b]);
+		for (int s = threadIdx.x; s < LT; s += BT) {
+			T v = czero<T>();
+			if (!init)
+				v = st[base | deposit((u64)s, ps.pos, ps.npos)];
+			else if (base == 0 && s == 0)
+				v.x = 1;
+			tile[lds_pos<T>(s)] = v;
+		}
+		__syncthreads();
+		tile_pass(tile, LB, ps, base, gates, gsh);
+		for (int s = threadIdx.x; s < LT; s += BT)
+			st[base | deposit((u64)s, ps.pos, ps.npos)] = tile[lds_pos<T>(s)];
 		__syncthreads();
 	}
 }
@@ -311,6 +460,12 @@ struct dk_batch {
 	int use_graph = 0;
 	int pass_qubits = 0; // 0 = tile size
 	int probs = 0;       // marginal probabilities of every register
+	int split_qubits = 17; // pooled registers this large run their passes
+			       // as grid-wide rounds (0 = never)
+	std::vector<BRound> rounds;       // every round's entries, in order
+	std::vector<size_t> round_off;    // first entry of round r (+ end)
+	std::vector<u64> round_items;     // tiles of round r
+	BRound *d_rounds = nullptr;
 	// device side; buffers are kept (and grown) across uploads
 	bool uploaded = false;
 	char *dbuf = nullptr;
@@ -406,6 +561,8 @@ extern "C" int dk_batch_set(dk_batch *b, const char *key, int64_t value)
 		b->pass_qubits = (int)value;
 	else if (!strcmp(key, "probs"))
 		b->probs = value != 0;
+	else if (!strcmp(key, "split_qubits"))
+		b->split_qubits = (int)std::max<int64_t>(0, value);
 	else {
 		dk_set_error("unknown batch setting %s", key);
 		return DK_E_ARG;
@@ -531,8 +688,11 @@ static int batch_upload(dk_batch *b)
 	// final states are exported
 	u64 pool_amps = 0;
 	for (auto &R : b->regs) {
-		R.flags = (R.flags & ~(BF_EXPORT | BF_PROBS)) |
-			  (b->export_states ? BF_EXPORT : 0) | (b->probs ? BF_PROBS : 0);
+		const bool split = b->split_qubits > 0 && R.nq > (unsigned)lb &&
+				   R.nq >= (unsigned)b->split_qubits;
+		R.flags = (R.flags & ~(BF_EXPORT | BF_PROBS | BF_SPLIT)) |
+			  (b->export_states ? BF_EXPORT : 0) | (b->probs ? BF_PROBS : 0) |
+			  (split ? BF_SPLIT : 0);
 		const u64 len = 1ull << R.nq;
 		if (R.nq > (unsigned)lb || b->export_states) {
 			R.off = pool_amps;
@@ -547,6 +707,27 @@ static int batch_upload(dk_batch *b)
 	std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) {
 		return b->regs[x].nq > b->regs[y].nq;
 	});
+	// rounds of the split registers' passes: round r holds pass r of every
+	// split register with more than r passes
+	b->rounds.clear();
+	b->round_off.clear();
+	b->round_items.clear();
+	for (unsigned r = 0;; ++r) {
+		const size_t first = b->rounds.size();
+		u64 items = 0;
+		for (unsigned i : order) {
+			const BReg &R = b->regs[i];
+			if (!(R.flags & BF_SPLIT) || R.pass_begin + r >= R.pass_end)
+				continue;
+			b->rounds.push_back(BRound{items, i, R.pass_begin + r});
+			items += 1ull << (R.nq - (unsigned)lb);
+		}
+		if (b->rounds.size() == first)
+			break;
+		b->round_off.push_back(first);
+		b->round_items.push_back(items);
+	}
+	b->round_off.push_back(b->rounds.size());
 	auto al = [](u64 x) { return (x + 255) & ~255ull; };
 	const u64 nr = b->regs.size(), nroll = b->rolls.size();
 	u64 o_regs = 0, o_order = al(o_regs + nr * sizeof(BReg));
@@ -558,7 +739,8 @@ static int batch_upload(dk_batch *b)
 	u64 o_probs = al(o_p1 + std::max<u64>(1, nroll) * sizeof(double));
 	u64 o_dens = al(o_probs + std::max<u64>(1, nroll) * sizeof(double));
 	u64 o_cnt = al(o_dens + std::max<u64>(1, b->dens_total) * sizeof(double2));
-	const u64 total = o_cnt + 256;
+	u64 o_rounds = al(o_cnt + 256);
+	const u64 total = o_rounds + std::max<size_t>(1, b->rounds.size()) * sizeof(BRound);
 	if (b->dbuf_bytes < total) {
 		if (b->dbuf)
 			dk_device_free(b->dbuf, b->dbuf_bytes);
@@ -590,6 +772,7 @@ static int batch_upload(dk_batch *b)
 	b->d_probs = (double *)(b->dbuf + o_probs);
 	b->d_dens = (double2 *)(b->dbuf + o_dens);
 	b->d_counter = (unsigned *)(b->dbuf + o_cnt);
+	b->d_rounds = (BRound *)(b->dbuf + o_rounds);
 	hipStream_t st = DK_STREAM;
 	DK_HIP(hipMemcpyAsync(b->d_regs, b->regs.data(), nr * sizeof(BReg),
 			      hipMemcpyHostToDevice, st));
@@ -604,6 +787,10 @@ static int batch_upload(dk_batch *b)
 				      hipMemcpyHostToDevice, st));
 	if (nroll)
 		DK_HIP(hipMemcpyAsync(b->d_rolls, b->rolls.data(), nroll * sizeof(double),
+				      hipMemcpyHostToDevice, st));
+	if (!b->rounds.empty())
+		DK_HIP(hipMemcpyAsync(b->d_rounds, b->rounds.data(),
+				      b->rounds.size() * sizeof(BRound),
 				      hipMemcpyHostToDevice, st));
 	DK_HIP(hipStreamSynchronize(st));
 	// one pinned block: p1 and probs (doubles), density (double2),
@@ -633,8 +820,23 @@ static int batch_enqueue(dk_batch *b)
 {
 	hipStream_t st = DK_STREAM;
 	const unsigned nregs = (unsigned)b->regs.size();
-	hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
 	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
+	for (size_t r = 0; r < b->round_items.size(); ++r) {
+		const BRound *ents = b->d_rounds + b->round_off[r];
+		const unsigned nents = (unsigned)(b->round_off[r + 1] - b->round_off[r]);
+		const u64 items = b->round_items[r];
+		const unsigned rg = (unsigned)std::min<u64>(items, grid);
+		if (b->dtype == DK_C64)
+			hipLaunchKernelGGL(k_batch_round<float2>, dim3(rg), dim3(BT), 0, st,
+					   b->d_regs, b->d_passes, b->d_gates,
+					   (float2 *)b->pool, ents, nents, items, r == 0);
+		else
+			hipLaunchKernelGGL(k_batch_round<double2>, dim3(rg), dim3(BT), 0, st,
+					   b->d_regs, b->d_passes, b->d_gates,
+					   (double2 *)b->pool, ents, nents, items, r == 0);
+		DK_LAUNCHED();
+	}
+	hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
 	if (b->dtype == DK_C64)
 		hipLaunchKernelGGL(k_batch<float2>, dim3(grid), dim3(BT), 0, st,
 				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,

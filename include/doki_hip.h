@@ -313,7 +313,10 @@ void dk_batch_free(dk_batch *b);
 /* Settings: "export_states" (keep final states readable by dk_batch_state),
  * "use_graph" (capture the launch as a HIP graph, replay on later runs),
  * "pass_qubits" (target qubits per cache-blocked pass, 0 = tile width),
- * "probs" (every register reports its marginal qubit probabilities). */
+ * "probs" (every register reports its marginal qubit probabilities),
+ * "split_qubits" (registers of at least this many qubits, and larger than
+ * one LDS tile, run their passes as grid-wide rounds before the persistent
+ * launch; 17 by default, 0 = never; results are the same either way). */
 int dk_batch_set(dk_batch *b, const char *key, int64_t value);
 
 /* Drops every register and keeps the device and pinned buffers for the next

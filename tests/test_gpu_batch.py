@@ -89,3 +89,50 @@ def test_batch_graph_replays_match(capi, oracle):
         s = capi.new(12)
         capi.apply(s, capi.Gate(W.gate_h()), [3], in_place=True)
         capi.synchronize()
+
+
+@pytest.mark.parametrize("dtype,split,pq", [("c128", 13, 0), ("c128", 13, 3), ("c64", 14, 0)])
+def test_batch_split_rounds_equal_single_workgroup(capi, oracle, dtype, split, pq):
+    """Registers at or above split_qubits run their passes as grid-wide
+    rounds (every workgroup takes tiles of one pass of every such register
+    per launch); states, outcomes, P1 and densities equal the one-workgroup
+    form bit for bit, on graph replays too; the mixed batch keeps its small
+    registers in the persistent launch."""
+    dt = capi.C128 if dtype == "c128" else capi.C64
+    rng = W.rng_for(9900 + split + pq)
+    specs = []
+    for i, n in enumerate([13, 14, 15, 16, 17, 18, 9, 5, 14, 19]):
+        steps, rolls = W.random_circuit(9900 + 17 * i + pq, n, int(rng.integers(10, 80)))
+        specs.append((n, steps, rolls))
+    res = {}
+    for mode in (0, split):
+        b = capi.Batch(dt, export_states=1, use_graph=1, split_qubits=mode,
+                       pass_qubits=pq)
+        for n, steps, rolls in specs:
+            b.add(n, steps, rolls, tuple(range(6)) if n == 14 else ())
+        runs = []
+        for rep in range(2):
+            b.run()
+            runs.append([(b.state(i), b.outcomes(i), b.density(i) if specs[i][0] == 14 else None)
+                         for i in range(len(specs))])
+        res[mode] = runs
+    for rep in range(2):
+        for i, (n, _, _) in enumerate(specs):
+            (s0, (o0, p0), d0), (s1, (o1, p1), d1) = res[0][rep][i], res[split][rep][i]
+            assert np.array_equal(s1, s0), (rep, i, n)
+            assert o1 == o0 and np.array_equal(p1, p0), (rep, i, n)
+            if d0 is not None:
+                assert np.array_equal(d1, d0), (rep, i, n)
+    if dt == capi.C128:
+        for i in (4, 9):
+            n, steps, rolls = specs[i]
+            want = oracle_run(oracle, oracle.new(n), steps).values()
+            assert np.max(np.abs(res[split][1][i][0] - want)) <= ATOL, n
+
+
+def test_batch_default_split_vs_oracle(capi, oracle):
+    """The default threshold (17 qubits) with the persistent registers beside
+    it, against the oracle."""
+    specs = [(17, *W.random_circuit(9801, 17, 60)), (18, *W.random_circuit(9802, 18, 45)),
+             (12, *W.random_circuit(9803, 12, 30)), (16, *W.random_circuit(9804, 16, 30))]
+    check_batch(capi, oracle, capi.C128, specs, ATOL)
