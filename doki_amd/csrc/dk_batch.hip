@@ -58,11 +58,14 @@ struct BPass {
 	unsigned char slot_of[64]; // qubit -> tile slot bit (positions only)
 };
 
+// A gate of a pass: the host splits its controls at the pass's tile, so a
+// tile only tests the outside part against its fixed index bits.
 struct BGate {
-	double m[8];  // 2x2 row-major, interleaved
-	u64 cm, am;
-	unsigned t;
-	unsigned pad;
+	double m[8];        // 2x2 row-major, interleaved
+	u64 cm, am;         // controls / anticontrols outside the tile (index bits)
+	unsigned cin, ain;  // controls / anticontrols inside the tile (slot bits)
+	unsigned t;         // target qubit
+	unsigned sb;        // target slot
 };
 
 enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4, BF_PROBS = 8, BF_SPLIT = 16 };
@@ -97,29 +100,44 @@ __device__ __forceinline__ u64 deposit(u64 x, const unsigned char *pos, int n)
 	return r;
 }
 
+// Index of slot s = threadIdx.x + k * BT of a pooled register's tile: the
+// deposit is linear over disjoint slot bits, so a thread deposits its own
+// part once and ORs in the part of k * BT (the same for the whole block).
+template <int PER>
+struct TileIndex {
+	u64 own;
+	u64 part[PER];
+	__device__ __forceinline__ TileIndex(u64 base, const BPass &ps)
+	{
+		own = base | deposit((u64)threadIdx.x, ps.pos, ps.npos);
+#pragma unroll
+		for (int k = 0; k < PER; ++k)
+			part[k] = deposit((u64)k * BT, ps.pos, ps.npos);
+	}
+	__device__ __forceinline__ u64 operator[](int k) const { return own | part[k]; }
+};
+
 // A gate as one tile sees it: its matrix, the in-tile control masks (slot
-// bits) and whether the out-of-tile controls leave it on for this tile.
+// bits; `ctl` when there are any) and whether the out-of-tile controls
+// leave it on for this tile.  Slot arithmetic is 32-bit (a tile holds at
+// most 2^13 slots).
 template <typename T> struct TileGate {
 	T m00, m01, m10, m11;
-	u64 cm_in, am_in;
+	unsigned cm_in, am_in;
 	int sb;
-	bool on;
+	bool on, ctl;
 };
 
 template <typename T>
-__device__ __forceinline__ TileGate<T> tile_gate_of(const BGate &g, const BPass &ps,
-						     u64 base)
+__device__ __forceinline__ TileGate<T> tile_gate_of(const BGate &g, u64 base)
 {
 	typedef typename real_of<T>::type R;
 	TileGate<T> c;
-	c.cm_in = c.am_in = 0;
-	for (u64 m = g.cm & ps.pmask; m; m &= m - 1)
-		c.cm_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
-	for (u64 m = g.am & ps.pmask; m; m &= m - 1)
-		c.am_in |= 1ull << ps.slot_of[__builtin_ctzll(m)];
-	const u64 cm_out = g.cm & ~ps.pmask, am_out = g.am & ~ps.pmask;
-	c.on = (base & cm_out) == cm_out && (base & am_out) == 0;
-	c.sb = ps.slot_of[g.t];
+	c.cm_in = g.cin;
+	c.am_in = g.ain;
+	c.on = (base & g.cm) == g.cm && (base & g.am) == 0;
+	c.ctl = (g.cin | g.ain) != 0;
+	c.sb = (int)g.sb;
 	c.m00 = mk<T>((R)g.m[0], (R)g.m[1]);
 	c.m01 = mk<T>((R)g.m[2], (R)g.m[3]);
 	c.m10 = mk<T>((R)g.m[4], (R)g.m[5]);
@@ -127,16 +145,22 @@ __device__ __forceinline__ TileGate<T> tile_gate_of(const BGate &g, const BPass 
 	return c;
 }
 
+__device__ __forceinline__ int ins0i(int x, int b)
+{
+	return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
+}
+
 // The gate on the pair (s0, s0 | 2^sb) when s0 meets its in-tile controls.
 template <typename T>
 __device__ __forceinline__ void pair_apply(T *tile, const TileGate<T> &c, int s0)
 {
-	if (((u64)s0 & c.cm_in) != c.cm_in || ((u64)s0 & c.am_in) != 0)
+	if (c.ctl && (((unsigned)s0 & c.cm_in) != c.cm_in || ((unsigned)s0 & c.am_in) != 0))
 		return;
 	const int s1 = s0 | (1 << c.sb);
-	const T x = tile[lds_pos<T>(s0)], y = tile[lds_pos<T>(s1)];
-	tile[lds_pos<T>(s0)] = cmac(cmac(czero<T>(), c.m00, x), c.m01, y);
-	tile[lds_pos<T>(s1)] = cmac(cmac(czero<T>(), c.m10, x), c.m11, y);
+	const int p0 = lds_pos<T>(s0), p1 = lds_pos<T>(s1);
+	const T x = tile[p0], y = tile[p1];
+	tile[p0] = cmac(cmac(czero<T>(), c.m00, x), c.m01, y);
+	tile[p1] = cmac(cmac(czero<T>(), c.m10, x), c.m11, y);
 }
 
 // Two consecutive gates on distinct slot bits a, b in one sweep: a thread
@@ -152,7 +176,7 @@ __device__ void tile_gate_pair(T *tile, int tb, const TileGate<T> &c1,
 	const int A = 1 << c1.sb, B = 1 << c2.sb;
 	const int quads = 1 << (tb - 2);
 	for (int q = threadIdx.x; q < quads; q += BT) {
-		const int s = (int)ins0(ins0((u64)q, (unsigned)lo), (unsigned)hi);
+		const int s = ins0i(ins0i(q, lo), hi);
 		if (c1.on) {
 			pair_apply(tile, c1, s);
 			pair_apply(tile, c1, s | B);
@@ -182,9 +206,9 @@ __device__ void tile_pass(T *tile, int tb, const BPass &ps, u64 base,
 			gsh[i] = gates[g0 + i];
 		__syncthreads();
 		for (int i = 0; i < cnt;) {
-			const TileGate<T> c1 = tile_gate_of<T>(gsh[i], ps, base);
+			const TileGate<T> c1 = tile_gate_of<T>(gsh[i], base);
 			if (i + 1 < cnt && tb >= 2) {
-				const TileGate<T> c2 = tile_gate_of<T>(gsh[i + 1], ps, base);
+				const TileGate<T> c2 = tile_gate_of<T>(gsh[i + 1], base);
 				if (c2.sb != c1.sb) {
 					tile_gate_pair(tile, tb, c1, c2);
 					__syncthreads();
@@ -195,7 +219,7 @@ __device__ void tile_pass(T *tile, int tb, const BPass &ps, u64 base,
 			if (c1.on) {
 				const int half = 1 << (tb - 1);
 				for (int p = threadIdx.x; p < half; p += BT)
-					pair_apply(tile, c1, (int)ins0((u64)p, (unsigned)c1.sb));
+					pair_apply(tile, c1, ins0i(p, c1.sb));
 			}
 			__syncthreads();
 			i += 1;
@@ -273,14 +297,15 @@ __global__ __launch_bounds__(BT) void k_batch(
 					u64 base = tl;
 					for (int b = 0; b < (int)ps.npos; ++b)
 						base = ins0(base, ps.pos[b]);
-					for (int s = threadIdx.x; s < LT; s += BT)
-						tile[lds_pos<T>(s)] =
-							st[base | deposit((u64)s, ps.pos, ps.npos)];
+					const TileIndex<LT / BT> ix(base, ps);
+#pragma unroll
+					for (int k = 0; k < LT / BT; ++k)
+						tile[lds_pos<T>((int)threadIdx.x + k * BT)] = st[ix[k]];
 					__syncthreads();
 					tile_pass(tile, LB, ps, base, gates, gsh);
-					for (int s = threadIdx.x; s < LT; s += BT)
-						st[base | deposit((u64)s, ps.pos, ps.npos)] =
-							tile[lds_pos<T>(s)];
+#pragma unroll
+					for (int k = 0; k < LT / BT; ++k)
+						st[ix[k]] = tile[lds_pos<T>((int)threadIdx.x + k * BT)];
 					__syncthreads();
 				}
 			}
@@ -423,18 +448,22 @@ __global__ __launch_bounds__(BT) void k_batch_round(
 		u64 base = it - E.first;
 		for (int b = 0; b < (int)ps.npos; ++b)
 			base = ins0(base, ps.pos[b]);
-		for (int s = threadIdx.x; s < LT; s += BT) {
+		const TileIndex<LT / BT> ix(base, ps);
+#pragma unroll
+		for (int k = 0; k < LT / BT; ++k) {
+			const int s = (int)threadIdx.x + k * BT;
 			T v = czero<T>();
 			if (!init)
-				v = st[base | deposit((u64)s, ps.pos, ps.npos)];
+				v = st[ix[k]];
 			else if (base == 0 && s == 0)
 				v.x = 1;
 			tile[lds_pos<T>(s)] = v;
 		}
 		__syncthreads();
 		tile_pass(tile, LB, ps, base, gates, gsh);
-		for (int s = threadIdx.x; s < LT; s += BT)
-			st[base | deposit((u64)s, ps.pos, ps.npos)] = tile[lds_pos<T>(s)];
+#pragma unroll
+		for (int k = 0; k < LT / BT; ++k)
+			st[ix[k]] = tile[lds_pos<T>((int)threadIdx.x + k * BT)];
 		__syncthreads();
 	}
 }
@@ -617,7 +646,7 @@ extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
 		x.t = targets[g];
 		x.cm = ctrl_masks ? ctrl_masks[g] : 0;
 		x.am = actrl_masks ? actrl_masks[g] : 0;
-		x.pad = 0;
+		x.cin = x.ain = x.sb = 0;
 		if (x.t >= nq || ((x.cm | x.am) & ~valid) || (x.cm & x.am) ||
 		    (((x.cm | x.am) >> x.t) & 1)) {
 			dk_set_error("invalid gate %zu for a %u-qubit register", g, nq);
@@ -669,6 +698,17 @@ extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
 			}
 		P.npos = (unsigned)c;
 		P.pmask = pm;
+		// split each gate's controls at the tile; target slot
+		for (unsigned gi = P.gate_begin; gi < P.gate_end; ++gi) {
+			BGate &x = b->gates[gi];
+			for (u64 m = x.cm & pm; m; m &= m - 1)
+				x.cin |= 1u << P.slot_of[__builtin_ctzll(m)];
+			for (u64 m = x.am & pm; m; m &= m - 1)
+				x.ain |= 1u << P.slot_of[__builtin_ctzll(m)];
+			x.cm &= ~pm;
+			x.am &= ~pm;
+			x.sb = P.slot_of[x.t];
+		}
 		b->passes.push_back(P);
 	} while (g < num_gates);
 	R.pass_end = (unsigned)b->passes.size();
