@@ -1,8 +1,8 @@
 /*
  * doki_oracle.c — CPU restatement of Doki's state-vector core.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product
- * path is compared against; it is never linked into, loaded by, or called
+ * TEST INFRASTRUCTURE ONLY.  This file is what the tests compare the HIP
+ * product path against; it is never linked into, loaded by, or called
  * from the product library or the `doki` extension.  Only the test suite,
  * the smoke check and the CPU-baseline leg of the benchmark load it.
  *

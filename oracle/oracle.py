@@ -1,6 +1,6 @@
 """ctypes front end of the oracle (liboracle.so, built from doki_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+TEST INFRASTRUCTURE ONLY: the CPU restatement the HIP path is compared against.
 Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
 ``bench.py`` may import this module; the product never does.
 
