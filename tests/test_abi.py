@@ -122,3 +122,33 @@ def test_bulk_read_entry_validates_on_host(built):
     assert _err(lambda: d.registry_get_all(5, False, False)) == "NULL pointer to registry"
     assert _err(lambda: d.registry_get_all(5, False)) == (
         "Syntax: registry_get_all(registry, canonical, verbose)")
+
+
+def test_batch_host_validation(built):
+    """The batched engine checks every register on the host, before any
+    device work: gate targets and masks (registry_apply's rules), density
+    size, rolls in [0, 1), settings; without a device its run fails loudly."""
+    from doki_amd import capi
+    from doki_amd import workloads as W
+    b = capi.Batch(capi.C128, split_qubits=13, pass_qubits=3, use_graph=1)
+    steps, rolls = W.random_circuit(3, 6, 20)
+    assert b.add(6, steps, rolls, (0, 1)) == 0
+    assert b.add(18, [(W.gate_h(), [17], {0}, {5})]) == 1
+    h = W.gate_h()
+    bad = [
+        (4, [(h, [4], set(), set())], None, ()),        # target outside
+        (4, [(h, [1], {1}, set())], None, ()),          # control on target
+        (4, [(h, [1], {2}, {2})], None, ()),            # control = anticontrol
+        (4, [(h, [1], {7}, set())], None, ()),          # control outside
+        (8, [], None, tuple(range(7))),                  # density of 7 qubits
+        (3, [], [0.5, 1.0, 0.1], ()),                    # roll not below 1
+        (27, [], None, ()),                              # too many qubits
+    ]
+    for nq, st, r, keep in bad:
+        with pytest.raises(capi.DokiHipError):
+            b.add(nq, st, r, keep)
+    with pytest.raises(capi.DokiHipError):
+        capi.check(capi.lib().dk_batch_set(b.ptr, b"no_such_setting", 1))
+    if capi.lib().dk_init(0) != 0:
+        with pytest.raises(capi.DokiHipError):
+            b.run()
