@@ -83,6 +83,7 @@ def lib():
         "dk_circuit_new": (C.c_int, [C.c_uint, PP]),
         "dk_circuit_free": (None, [P]),
         "dk_circuit_add": (C.c_int, [P, C.c_uint, dp, C.POINTER(C.c_uint), u64, u64]),
+        "dk_circuit_add_many": (C.c_int, [P, C.c_size_t, dp, C.POINTER(C.c_uint), C.POINTER(u64), C.POINTER(u64)]),
         "dk_circuit_fuse": (C.c_int, [P, C.c_uint, C.POINTER(C.c_size_t)]),
         "dk_circuit_num_gates": (C.c_int, [P, C.POINTER(C.c_size_t)]),
         "dk_circuit_block": (C.c_int, [P, C.c_size_t, C.POINTER(C.c_uint),
@@ -344,16 +345,12 @@ class Circuit:
                                    mask(controls), mask(anticontrols)))
 
     def add_packed(self, packed):
-        """1-qubit gates packed as by pack_steps."""
+        """1-qubit gates packed as by pack_steps, in one call."""
         mf, tg, cm, am, ng = packed
-        L = lib()
-        dp = C.POINTER(C.c_double)
-        base = mf.ctypes.data
-        t = (C.c_uint * 1)()
-        for g in range(ng):
-            t[0] = int(tg[g])
-            check(L.dk_circuit_add(self.ptr, 1, C.cast(base + 64 * g, dp), t,
-                                   int(cm[g]), int(am[g])))
+        u = C.POINTER(C.c_uint64)
+        check(lib().dk_circuit_add_many(
+            self.ptr, ng, _dp(mf), tg.ctypes.data_as(C.POINTER(C.c_uint)),
+            cm.ctypes.data_as(u), am.ctypes.data_as(u)))
 
     def fuse(self, k):
         n = C.c_size_t()

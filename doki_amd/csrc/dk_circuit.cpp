@@ -212,6 +212,29 @@ extern "C" int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
 	return DK_OK;
 }
 
+extern "C" int dk_circuit_add_many(dk_circuit *c, size_t count,
+				   const double *matrices, const unsigned *targets,
+				   const uint64_t *ctrl_masks,
+				   const uint64_t *actrl_masks)
+{
+	if (!c || (count && (!matrices || !targets))) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	const size_t before = c->gates.size();
+	for (size_t i = 0; i < count; ++i) {
+		const int rc = dk_circuit_add(c, 1, matrices + 8 * i, targets + i,
+					      ctrl_masks ? ctrl_masks[i] : 0,
+					      actrl_masks ? actrl_masks[i] : 0);
+		if (rc) {
+			// all or nothing: drop the gates of this call
+			c->gates.resize(before);
+			return rc;
+		}
+	}
+	return DK_OK;
+}
+
 extern "C" int dk_circuit_fuse(dk_circuit *c, unsigned max_qubits,
 			       size_t *num_blocks)
 {

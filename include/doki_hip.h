@@ -249,6 +249,16 @@ int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
 		   const unsigned *targets, uint64_t ctrl_mask,
 		   uint64_t actrl_mask);
 
+/* Appends count one-qubit gates in one call: gate i is the 2x2 matrix at
+ * matrices + 8 i (row-major, interleaved), on targets[i], with ctrl_masks[i]
+ * and actrl_masks[i] (either array may be NULL: no controls), checked as
+ * dk_circuit_add checks it.  All or nothing: on an invalid gate none of the
+ * call's gates is kept.  No counterpart in the reference (a circuit layer
+ * above registry_apply, doki.c:722-941, submitting its gate list at once). */
+int dk_circuit_add_many(dk_circuit *c, size_t count, const double *matrices,
+			const unsigned *targets, const uint64_t *ctrl_masks,
+			const uint64_t *actrl_masks);
+
 /* Plans fused blocks of at most max_block_qubits (1..6) qubits; 0 runs the
  * gates one by one.  *num_blocks (may be NULL) receives the block count. */
 int dk_circuit_fuse(dk_circuit *c, unsigned max_block_qubits,

@@ -179,3 +179,30 @@ def test_replanning_after_add(oracle, capi):
     got = run_plan(circ, steps, n, psi0, lone_apply)
     want = oracle_run(oracle, oracle.new(n), steps).values()
     assert np.max(np.abs(got - want)) <= ATOL
+
+
+def test_bulk_gate_submission_equals_one_by_one(capi):
+    """dk_circuit_add_many (one call for a packed gate list) records what
+    dk_circuit_add records gate by gate: the same pass plan and op lists;
+    an invalid gate leaves the circuit as it was."""
+    n = 14
+    steps = mixed_steps(31, n, 120, multi=False)
+    one = capi.Circuit(n)
+    for m, t, c, a in steps:
+        one.add(m, t, c, a)
+    many = capi.Circuit(n)
+    many.add_packed(capi.pack_steps(steps))
+    assert one.passes() == many.passes()
+    assert one.plan() == many.plan()
+    for i in range(one.passes()):
+        assert one.pass_info(i) == many.pass_info(i)
+        assert np.array_equal(one.pass_ops(i), many.pass_ops(i))
+    before = many.num_gates() if hasattr(many, "num_gates") else None
+    bad = capi.pack_steps([steps[0], (W.gate_h(), [n], set(), set())])
+    with pytest.raises(capi.DokiHipError):
+        many.add_packed(bad)
+    cnt = capi.C.c_size_t()
+    capi.check(capi.lib().dk_circuit_num_gates(many.ptr, capi.C.byref(cnt)))
+    assert cnt.value == len(steps)
+    if before is not None:
+        assert before == len(steps)
