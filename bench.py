@@ -556,7 +556,7 @@ def main():
                     help="qubits of the large-register leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=60.0)
-    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-workers", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

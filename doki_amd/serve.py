@@ -136,7 +136,7 @@ def run_operation(spec, states, dtype, one_pass_join=False):
 class Server:
     """Continuous-batching request server on the library's device."""
 
-    def __init__(self, workers=3, small_qubits=SMALL_QUBITS, plan="auto",
+    def __init__(self, workers=6, small_qubits=SMALL_QUBITS, plan="auto",
                  budget=None, max_batch=2048, dtype=capi.C128):
         self.dtype = dtype
         self.small_qubits = small_qubits
