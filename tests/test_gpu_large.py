@@ -1,6 +1,7 @@
-"""Registers beyond 2^31 amplitudes: launches stay below 2^32 threads and the
-kernels loop over their virtual blocks.  32 qubits (68.7 GB complex128), so
-the checks use closed forms instead of the oracle."""
+"""Registers of 2^31 amplitudes and more: one-qubit kernels at their largest
+single launch (31 qubits), and beyond it launches stay below 2^32 threads
+and the kernels loop over their virtual blocks (32 qubits, 68.7 GB
+complex128).  The checks use closed forms instead of the oracle."""
 import numpy as np
 import pytest
 
