@@ -322,33 +322,13 @@ def fused_leg(args):
     return out
 
 
-def batch_specs(seed, count, nmin=4, nmax=20):
-    """Registers with qubit counts log-uniform (p ~ 1/n) on [nmin, nmax],
-    each a mixed random circuit of depth U[10, 100] with rolls for a full
-    measurement; members of 14 qubits also ask for a 6-qubit density."""
-    import numpy as np
-
-    from doki_amd import workloads as W
-    rng = W.rng_for(seed)
-    ns = np.arange(nmin, nmax + 1)
-    p = 1.0 / ns
-    p /= p.sum()
-    specs = []
-    for i in range(count):
-        n = int(rng.choice(ns, p=p))
-        depth = int(rng.integers(10, 101))
-        steps, rolls = W.random_circuit(seed * 100003 + i, n, depth)
-        keep = tuple(range(6)) if n == 14 else ()
-        specs.append((n, steps, rolls, keep))
-    return specs
-
-
 def batch_leg(args):
     """Many small registers: one persistent launch (graph replay) against
     plain per-gate execution of the same registers one at a time."""
     import numpy as np
 
     from doki_amd import capi
+    from doki_amd.workloads import batch_specs
     t0 = time.perf_counter()
     specs = batch_specs(31, args.batch_registers)
     log(f"[bench] batch: {len(specs)} registers generated in "
@@ -357,6 +337,7 @@ def batch_leg(args):
            "gates": int(sum(len(s[1]) for s in specs)),
            "qubits_hist": {int(n): int(c) for n, c in
                            zip(*np.unique([s[0] for s in specs], return_counts=True))}}
+    kept = None
     for dtype, name in ((capi.C128, "c128"), (capi.C64, "c64")):
         b = capi.Batch(dtype, use_graph=1)
         for n, steps, rolls, keep in specs:
@@ -371,12 +352,16 @@ def batch_leg(args):
         out[name] = {"seconds": round(t, 4),
                      "circuits_per_s": round(len(specs) / t, 1)}
         log(f"[bench] batch {name}: {out[name]}")
-        del b
+        if dtype == capi.C128:
+            kept = b                     # its results are checked below
+        else:
+            del b
     # plain: each register through the per-gate C ABI path, one at a time
     rng = np.random.default_rng(5)
     sample = rng.choice(len(specs), size=min(len(specs), args.batch_plain_sample),
                         replace=False)
     gate_cache = {}
+    plain = {}
     t1 = time.perf_counter()
     for i in sample:
         n, steps, rolls, keep = specs[i]
@@ -387,12 +372,29 @@ def batch_leg(args):
             if g is None:
                 g = gate_cache[key] = capi.Gate(m)
             capi.apply(s, g, t, c, a, in_place=True)
-        if keep:
-            capi.reduced_density(s, keep)
-        capi.measure(s, (1 << n) - 1, rolls)
+        rho = capi.reduced_density(s, keep) if keep else None
+        _, outs, p1 = capi.measure(s, (1 << n) - 1, rolls)
+        plain[int(i)] = (outs, p1, rho)
         del s
     capi.synchronize()
     tp = time.perf_counter() - t1
+    # the batched c128 results of the sampled registers against the plain
+    # path: outcomes identical (except where a roll lies within 1e-9 of its
+    # P1), P1 and densities within 1e-12
+    worst = 0.0
+    for i, (outs, p1, rho) in plain.items():
+        bo, bp = kept.outcomes(i)
+        rolls = specs[i][2]
+        if bo != outs and not any(abs(p - r) < 1e-9 for p, r in zip(p1, rolls)):
+            raise AssertionError(f"batched register {i}: outcomes differ from "
+                                 "the plain path")
+        worst = max(worst, float(np.max(np.abs(bp - p1))))
+        if rho is not None:
+            worst = max(worst, float(np.max(np.abs(kept.density(i) - rho))))
+    if worst > 1e-12:
+        raise AssertionError(f"batched vs plain: max deviation {worst:.3e}")
+    out["checked_vs_plain"] = {"registers": len(plain), "max_abs": worst}
+    del kept
     per = tp / len(sample)
     out["plain_c128"] = {"sample": int(len(sample)), "seconds_per_register": round(per, 6),
                          "circuits_per_s": round(1.0 / per, 1)}

@@ -346,7 +346,7 @@ class Circuit:
 
     def add_packed(self, packed):
         """1-qubit gates packed as by pack_steps, in one call."""
-        mf, tg, cm, am, ng = packed
+        mf, tg, cm, am, ng = checked_packed(packed)
         u = C.POINTER(C.c_uint64)
         check(lib().dk_circuit_add_many(
             self.ptr, ng, _dp(mf), tg.ctypes.data_as(C.POINTER(C.c_uint)),
@@ -405,6 +405,31 @@ class Circuit:
         return s if in_place else State(p.value)
 
 
+def checked_packed(packed):
+    """A packed gate list (pack_steps' layout, possibly from a client) as
+    contiguous arrays of the types the C ABI reads: float64 matrices of at
+    least 8 * ng values, and ng uint32 targets and uint64 masks at least.
+    Anything else raises ValueError before a pointer reaches C."""
+    mf, tg, cm, am, ng = packed
+    ng = int(ng)
+    if ng < 0:
+        raise ValueError("negative gate count")
+    mf = np.ascontiguousarray(mf)
+    tg = np.ascontiguousarray(tg)
+    cm = np.ascontiguousarray(cm)
+    am = np.ascontiguousarray(am)
+    for name, a, dt in (("matrices", mf, np.float64), ("targets", tg, np.uint32),
+                        ("control masks", cm, np.uint64),
+                        ("anticontrol masks", am, np.uint64)):
+        if a.dtype != dt:
+            raise ValueError(f"packed {name} must be {np.dtype(dt).name}, "
+                             f"not {a.dtype.name}")
+    rows = max(ng, 1)
+    if mf.size < 8 * rows or tg.size < rows or cm.size < rows or am.size < rows:
+        raise ValueError(f"packed arrays hold fewer than {ng} gates")
+    return mf, tg, cm, am, ng
+
+
 def pack_steps(steps):
     """(2x2 matrix, [target], controls, anticontrols) tuples -> the packed
     arrays dk_batch_add takes."""
@@ -448,7 +473,7 @@ class Batch:
         """packed: (matrices as float64 [g, 8], targets uint32 [g],
         control masks uint64 [g], anticontrol masks uint64 [g], g); the
         arrays hold at least one row."""
-        mf, tg, cm, am, ng = packed
+        mf, tg, cm, am, ng = checked_packed(packed)
         r = None
         if rolls is not None:
             r = np.ascontiguousarray(np.asarray(rolls, dtype=np.float64))

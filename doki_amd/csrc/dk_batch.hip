@@ -629,6 +629,8 @@ extern "C" int dk_batch_set(dk_batch *b, const char *key, int64_t value)
 		return DK_E_ARG;
 	}
 	batch_invalidate(b);
+	// results of an earlier run were made under the old settings
+	b->ran = false;
 	return DK_OK;
 }
 
@@ -667,7 +669,7 @@ extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
 	}
 	if (rolls)
 		for (unsigned i = 0; i < nq; ++i)
-			if (!(rolls[i] >= 0.0 && rolls[i] < 1.0)) {
+			if (rolls[i] < 0.0 || rolls[i] >= 1.0) {  // NaN passes
 				dk_set_error("roll not in interval [0, 1)!");
 				return DK_E_ROLL;
 			}

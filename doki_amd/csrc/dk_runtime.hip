@@ -683,7 +683,7 @@ extern "C" int dk_measure(const dk_state *s, uint64_t mask, const double *rolls,
 			qs.push_back(q);
 	}
 	for (size_t k = 0; k < qs.size(); ++k) {
-		if (k >= num_rolls || !rolls || !(rolls[k] >= 0.0 && rolls[k] < 1.0)) {
+		if (k >= num_rolls || !rolls || rolls[k] < 0.0 || rolls[k] >= 1.0) {
 			dk_set_error("roll not in interval [0, 1)!");
 			return DK_E_ROLL;
 		}

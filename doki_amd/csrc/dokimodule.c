@@ -852,7 +852,7 @@ static PyObject *doki_registry_measure(PyObject *self, PyObject *args)
 		double v = o ? PyFloat_AsDouble(o) : -1.0;
 		if (PyErr_Occurred())
 			PyErr_Clear();
-		if (!(v >= 0 && v < 1)) {
+		if (v < 0 || v >= 1) {  /* NaN passes, as doki.c:1114 */
 			PyMem_Free(rolls);
 			PyErr_SetString(DokiError, "roll not in interval [0, 1)!");
 			return NULL;

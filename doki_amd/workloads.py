@@ -182,3 +182,22 @@ def serving_requests(seed, count, qps, nmax=30, depth=(20, 200)):
             keep = tuple(sorted(int(q) for q in rng.choice(n0, size=a, replace=False)))
         out.append(dict(rid=rid, arrival=t, op=op, regs=regs, rolls=rolls, keep=keep))
     return out
+
+
+def batch_specs(seed, count, nmin=4, nmax=20):
+    """The batched-registers workload: qubit counts log-uniform (p ~ 1/n) on
+    [nmin, nmax], each a mixed random circuit of depth U[10, 100] with rolls
+    for a full measurement; members of 14 qubits also ask for the reduced
+    density of qubits 0..5.  Returns (n, steps, rolls, keep) tuples."""
+    rng = rng_for(seed)
+    ns = np.arange(nmin, nmax + 1)
+    p = 1.0 / ns
+    p /= p.sum()
+    specs = []
+    for i in range(count):
+        n = int(rng.choice(ns, p=p))
+        depth = int(rng.integers(10, 101))
+        steps, rolls = random_circuit(seed * 100003 + i, n, depth)
+        keep = tuple(range(6)) if n == 14 else ()
+        specs.append((n, steps, rolls, keep))
+    return specs

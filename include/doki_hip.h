@@ -172,7 +172,8 @@ int dk_prob(const dk_state *s, unsigned q, double *p);
 int dk_prob_all(const dk_state *s, double *p);
 
 /* Measurement of the qubits in `mask`, most significant first, consuming
- * one roll per measured qubit: outcome = P1 > roll; the kept half is
+ * one roll per measured qubit: outcome = P1 > roll (a NaN roll passes the
+ * [0, 1) check and gives 0, as in the reference); the kept half is
  * renormalised by sqrt(P), P = P1 or 1 - P1.  results[i] describes qubit
  * n-1-i: -1 not measured, 0 or 1.  p1_out (may be NULL) receives P1 per
  * results[] slot.  *out is the remaining state, or NULL when every qubit was

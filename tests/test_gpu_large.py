@@ -74,3 +74,67 @@ def test_join_to_32_qubits(capi):
     pa = [float(np.sum(np.abs(a[(np.arange(1 << 16) >> q) & 1 == 1]) ** 2)) for q in range(16)]
     pb = [float(np.sum(np.abs(b[(np.arange(1 << 16) >> q) & 1 == 1]) ** 2)) for q in range(16)]
     assert np.max(np.abs(capi.prob_all(j) - np.array(pb + pa))) <= 1e-12
+
+
+def _brickwork_closed_form(steps, n):
+    """Amplitudes of one brickwork layer applied to |0...0>: the U3 layer
+    makes the product state prod_q U3_q[b_q, 0]; each CZ on (q, q+1) then
+    flips the sign where both bits are set.  Returns (amp(idx), P(q = 1))."""
+    col = np.zeros((n, 2), dtype=np.complex128)
+    pairs = []
+    for m, t, c, a in steps:
+        if c:
+            pairs.append((min(c), t[0]))
+        else:
+            col[t[0]] = m[:, 0]
+
+    def amp(idx):
+        idx = np.asarray(idx, dtype=np.uint64)
+        v = np.ones(idx.shape, dtype=np.complex128)
+        for q in range(n):
+            v *= col[q][((idx >> np.uint64(q)) & np.uint64(1)).astype(np.int64)]
+        for lo, hi in pairs:
+            both = ((idx >> np.uint64(lo)) & (idx >> np.uint64(hi)) & np.uint64(1)) == 1
+            v = np.where(both, -v, v)
+        return v
+
+    return amp, np.abs(col[:, 1]) ** 2
+
+
+def test_33_qubit_brickwork_layer_closed_form(doki, capi):
+    """The 33-qubit register (137.4 GB complex128) of the brickwork workload:
+    one layer from |0...0>, out of place through the drop-in module (old and
+    new state together: 274.9 GB) and in place through the C ABI; sampled
+    amplitudes and every marginal against the closed form."""
+    n = 33
+    steps = W.brickwork_layer(W.rng_for(3333), n, 0)
+    assert len(steps) == 33 + 16
+    amp, p1 = _brickwork_closed_form(steps, n)
+    rng = W.rng_for(33)
+    idx = np.concatenate([rng.integers(0, 1 << n, size=1500, dtype=np.uint64),
+                          np.array([0, (1 << n) - 1, 1 << (n - 1), 0x155555555],
+                                   dtype=np.uint64)])
+    capi.check(capi.lib().dk_memory_trim())
+    gates = [(doki.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
+    reg = doki.registry_new(n, False)
+    for g, t, c in gates:
+        reg = doki.registry_apply(reg, g, t, c, set(), -1, False)
+    got = np.array([doki.registry_get(reg, int(i), False, False) for i in idx])
+    assert np.max(np.abs(got - amp(idx))) <= 1e-12
+    probs = np.array([doki.registry_prob(reg, q, -1, False) for q in range(n)])
+    assert np.max(np.abs(probs - p1)) <= 1e-12
+    doki.registry_del(reg, False)
+    del reg
+    capi.synchronize()
+
+    s = capi.new(n)
+    for m, t, c, a in steps:
+        capi.apply(s, capi.Gate(m), t, c, a, in_place=True)
+    assert np.max(np.abs(capi.prob_all(s) - p1)) <= 1e-12
+    assert abs(s.norm - 1.0) <= 1e-12
+    for first in (0, (1 << 20) + 7, (1 << 32) - 4096, (1 << n) - 65536):
+        raw, nrm = s.raw(first, 65536)
+        want = amp(np.arange(first, first + 65536, dtype=np.uint64))
+        assert np.max(np.abs(raw / nrm - want)) <= 1e-12, first
+    del s
+    capi.synchronize()

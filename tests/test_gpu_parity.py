@@ -488,3 +488,29 @@ def test_state_compare(capi, n):
     a32 = a.astype(np.complex64).astype(np.complex128)
     assert abs(inner - np.vdot(a, a32)) <= 1e-6
     assert abs(mx - np.max(np.abs(a - a32))) <= 1e-7
+
+
+def test_measure_nan_roll_vs_oracle(doki, capi, oracle):
+    """A NaN roll is accepted, as the reference's range check lets it through
+    (doki.c:1114), and decides outcome 0 (P1 > NaN is false); the drop-in
+    module, the C ABI and the batched engine agree with the oracle."""
+    n = 6
+    steps, rolls = W.random_circuit(516, n, 50)
+    rolls = list(rolls)
+    rolls[0] = float("nan")          # the most significant qubit
+    rolls[3] = float("nan")
+    reg = doki_run(doki, doki.registry_new(n, False), steps)
+    st = oracle_run(oracle, oracle.new(n), steps)
+    left, lst = doki.registry_measure(reg, (1 << n) - 1, rolls, 1, False)
+    rc, oleft, olst = oracle.measure(st, (1 << n) - 1, rolls)
+    assert lst == olst and lst[0] is False and lst[3] is False
+    left, lst = doki.registry_measure(reg, 0b110000, rolls[:2], 1, False)
+    rc, oleft, olst = oracle.measure(st, 0b110000, rolls[:2])
+    assert lst == olst
+    assert maxabs(doki_values(doki, left, n - 2), oleft.values()) <= ATOL
+    b = capi.Batch(capi.C128)
+    b.add(n, steps, rolls)
+    b.run()
+    outs, p1 = b.outcomes(0)
+    _, full = doki.registry_measure(reg, (1 << n) - 1, rolls, 1, False)
+    assert outs == full

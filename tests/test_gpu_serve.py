@@ -153,3 +153,97 @@ def test_serve_rejects_over_budget(capi):
         ok = dict(rid=1, arrival=0.0, op="prob", regs=[(12, steps)], rolls=None,
                   keep=())
         assert srv.submit(ok).wait(60)["probs"].shape == (12,)
+
+
+def _product_expect(spec):
+    """Closed forms for a register run by uncontrolled one-qubit gates only
+    (a product state): P(q = 1) = |U_q[1, 0]|^2 with U_q the product of the
+    gates on q; measuring one qubit leaves the others' marginals unchanged,
+    so outcome(q) = P1_q > its roll; the reduced density is the kron of the
+    kept qubits' 2x2 densities, highest kept qubit leftmost."""
+    n, steps = spec["regs"][0]
+    U = [np.eye(2, dtype=np.complex128) for _ in range(n)]
+    for m, t, c, a in steps:
+        assert not c and not a
+        U[t[0]] = np.asarray(m) @ U[t[0]]
+    col = [u[:, 0] for u in U]
+    p1 = np.array([abs(c[1]) ** 2 for c in col])
+    if spec["op"] == "prob":
+        return {"probs": p1}
+    if spec["op"] == "measure":
+        # list slot i is qubit n-1-i and takes roll i
+        return {"outcomes": [bool(p1[n - 1 - i] > r) for i, r in enumerate(spec["rolls"])],
+                "p1": np.array([p1[n - 1 - i] for i in range(n)])}
+    rho = np.ones((1, 1), dtype=np.complex128)
+    for q in spec["keep"]:
+        rho = np.kron(np.outer(col[q], col[q].conj()), rho)
+    return {"rho": rho}
+
+
+def test_serve_large_band_and_big_joins(capi):
+    """The served mix's 27-30-qubit band (worker lanes, cache-blocked passes)
+    and joins to 31 and 32 qubits, which no oracle run reaches in seconds:
+    product-state requests against closed forms, mixed-circuit requests
+    against plain per-gate execution, join marginals against the marginals
+    of their two operands (a kron join's marginals are its operands'), and
+    every result the same alone and in the concurrent mix."""
+    from doki_amd import serve
+    rng = W.rng_for(2730)
+    specs, kinds = [], {}
+
+    def add(op, regs, kind):
+        n0 = regs[0][0]
+        rolls = rng.random(n0).tolist() if op == "measure" else None
+        keep = tuple(sorted(int(q) for q in rng.choice(n0, size=3, replace=False))) \
+            if op == "density" else ()
+        specs.append(dict(rid=len(specs), arrival=0.0, op=op, regs=regs,
+                          rolls=rolls, keep=keep))
+        kinds[len(specs) - 1] = kind
+
+    for n, op in ((27, "measure"), (28, "prob"), (29, "density"), (30, "prob")):
+        add(op, [(n, [W.random_gate_step(rng, n, 0.0, 0.0) for _ in range(60)])],
+            "product")
+    for n, op in ((27, "prob"), (30, "density"), (28, "measure")):
+        add(op, [(n, [W.random_gate_step(rng, n) for _ in range(40)])], "mixed")
+    for na, nb in ((16, 15), (16, 16)):
+        add("join", [(na, [W.random_gate_step(rng, na) for _ in range(50)]),
+                     (nb, [W.random_gate_step(rng, nb) for _ in range(50)])], "join")
+
+    with serve.Server(workers=2) as srv:
+        alone = {}
+        for s in specs:
+            alone[s["rid"]] = srv.submit(s).wait(300)
+            print(f"[serve-large] request {s['rid']} ({s['op']}, "
+                  f"{[r[0] for r in s['regs']]} qubits) alone done", flush=True)
+    with serve.Server(workers=3) as srv:
+        hs, _ = serve.serve_trace(srv, specs, time_scale=0.0)
+        mix = {h.spec["rid"]: h.wait(0) for h in hs}
+    for s in specs:
+        assert_same(alone[s["rid"]], mix[s["rid"]])
+    print("[serve-large] alone == mix", flush=True)
+
+    mixed = [s for s in specs if kinds[s["rid"]] == "mixed"]
+    plain, _ = serve.run_plain(mixed, time_scale=0.0)
+    for s, (res, _lat) in zip(mixed, plain):
+        assert not isinstance(res, Exception), res
+        assert_close(mix[s["rid"]], res, s)
+    for s in specs:
+        got = mix[s["rid"]]
+        if kinds[s["rid"]] == "product":
+            want = _product_expect(s)
+            if "outcomes" in want:
+                assert np.max(np.abs(got["p1"] - want["p1"])) <= ATOL
+            assert_close(got, want, s)
+        elif kinds[s["rid"]] == "join":
+            (na, sa), (nb, sb) = s["regs"]
+            pa = capi.prob_all(serve_plain_state(capi, na, sa))
+            pb = capi.prob_all(serve_plain_state(capi, nb, sb))
+            assert got["probs"].shape == (na + nb,)
+            assert np.max(np.abs(got["probs"] - np.concatenate([pb, pa]))) <= ATOL
+
+
+def serve_plain_state(capi, n, steps):
+    st = capi.new(n)
+    for m, t, c, a in steps:
+        st = capi.apply(st, capi.Gate(m), t, c, a)
+    return st

@@ -214,6 +214,33 @@ def test_measure_roll_order_and_rule(oracle):
     assert lst == [True]
 
 
+def test_measure_nan_roll(oracle):
+    """A NaN roll passes the reference's range check (roll < 0 || roll >= 1,
+    doki.c:1114) and P1 > NaN is false: outcome 0, the kept half renormalised
+    by sqrt(1 - P1).  Checked against the compiled reference where it is
+    built."""
+    n = 4
+    steps, _ = W.random_circuit(515, n, 30)
+    st = oracle_run(oracle, oracle.new(n), steps)
+    p3 = oracle.prob(st, 3)
+    rc, one, lst1 = oracle.measure(st, 0b1000, [float("nan")])
+    assert rc == 0 and lst1[0] is False
+    assert abs(one.norm - math.sqrt(1.0 - p3)) <= 1e-15
+    rc, left, lst = oracle.measure(st, 0b1001, [float("nan"), 0.25])
+    assert rc == 0 and lst[0] is False
+    ref = oracle.load_reference()
+    if ref is None:
+        pytest.skip("compiled reference not built")
+    reg = ref.registry_new(n, False)
+    for m, t, c, a in steps:
+        g = ref.gate_new(1, np.asarray(m).tolist(), False)
+        reg = ref.registry_apply(reg, g, list(t), set(c), set(a), 1, False)
+    rleft, rlst = ref.registry_measure(reg, 0b1001, [float("nan"), 0.25], 1, False)
+    assert rlst == lst
+    got = np.array([ref.registry_get(rleft, i, False, False) for i in range(1 << (n - 2))])
+    assert np.array_equal(got, left.values())
+
+
 def test_join_is_kron(oracle):
     a = W.random_normalised_state(1, 3)
     b = W.random_normalised_state(2, 2)
