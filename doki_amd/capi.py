@@ -58,6 +58,7 @@ def lib():
         "dk_gate_new": (C.c_int, [C.c_uint, dp, PP]),
         "dk_gate_free": (None, [P]),
         "dk_apply": (C.c_int, [P, P, C.POINTER(C.c_uint), u64, u64, C.c_int, PP]),
+        "dk_apply_noalloc": (C.c_int, [P, P, C.POINTER(C.c_uint), u64, u64, P]),
         "dk_prob": (C.c_int, [P, C.c_uint, dp]),
         "dk_prob_all": (C.c_int, [P, dp]),
         "dk_measure": (C.c_int, [P, u64, dp, C.c_size_t,
@@ -218,6 +219,16 @@ def apply(s, gate, targets, controls=(), anticontrols=(), in_place=False):
     if in_place:
         return s
     return State(p.value)
+
+
+def apply_noalloc(s, gate, targets, controls=(), anticontrols=(), out=None):
+    """dk_apply_noalloc: the gate applied to s, written into `out` (a state
+    of s's size and type; s itself when None), with no device allocation."""
+    out = s if out is None else out
+    t = (C.c_uint * len(targets))(*targets)
+    check(lib().dk_apply_noalloc(s.ptr, gate.ptr, t, mask(controls),
+                                 mask(anticontrols), out.ptr))
+    return out
 
 
 def prob(s, q):

@@ -275,7 +275,8 @@ unsigned dk_dense_tile_bits(unsigned k, int dtype)
 }
 
 int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
-			  const unsigned *targets, double **parts, u64 *nparts)
+			  const unsigned *targets, double **parts, u64 *nparts,
+			  double *scratch, u64 scratch_cap)
 {
 	// one block per tile of 64 * CW groups: targets plus the 6 + log2(CW)
 	// lowest other bits
@@ -287,7 +288,8 @@ int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 	TileArgs ta;
 	dk_tile_args(in->nq, g->k, targets, ta, (int)tb);
 	const u64 nblocks = in->len >> ta.npos;
-	int rc = dk_partials_reserve(nblocks, parts);
+	int rc = scratch ? dk_partials_in(scratch, scratch_cap, nblocks, parts)
+			 : dk_partials_reserve(nblocks, parts);
 	if (rc)
 		return rc;
 	*nparts = nblocks;

@@ -115,6 +115,10 @@ struct dk_ctx {
 	int dense_k4_cw = 0;    // column sets per dense k = 4 block (0 = auto)
 	u64 arena_max_chunks = 0;     // cap on 2 GiB state-arena chunks
 	u64 arena_default_chunks = 0; // the cap at init: free memory less 6 GiB
+	// device address of the file-scope scratch array of dk_runtime.hip
+	// (block partials of dk_apply_noalloc), taken once at init with the
+	// device current; the library binds one device per process
+	double *noalloc_scratch = nullptr;
 };
 
 enum { DK_PINNED = 1 << 16 };
@@ -167,19 +171,27 @@ static inline unsigned dk_grid_cap(u64 nvb, unsigned threads)
 }
 
 // Gate application; block partial sums of |out|^2 finish into out's norm on
-// the device.
+// the device.  The partials go to the calling lane's growing buffer, or, when
+// `scratch` is given, into those `scratch_cap` doubles (nothing is allocated;
+// DK_E_ARG when the launch needs more).
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
-		    const unsigned *targets, u64 cm, u64 acm);
+		    const unsigned *targets, u64 cm, u64 acm,
+		    double *scratch = nullptr, u64 scratch_cap = 0);
 
 // Dense k-qubit gate (4 <= k <= 6, no controls) on MFMA, for states of at
 // least dk_dense_tile_bits(k) qubits (the index bits one block covers).
 unsigned dk_dense_tile_bits(unsigned k, int dtype);
 int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
-			  const unsigned *targets, double **parts, u64 *nparts);
+			  const unsigned *targets, double **parts, u64 *nparts,
+			  double *scratch = nullptr, u64 scratch_cap = 0);
 
 // Shared partials buffer with room for `count` block partials plus the
 // chunked finishing pass; deterministic finishing sum of such partials.
 int dk_partials_reserve(u64 count, double **out);
+// Doubles a partials buffer needs for `count` partials and their finishing.
+u64 dk_partials_need(u64 count);
+// `count` partials in caller-given scratch of `cap` doubles (no allocation).
+int dk_partials_in(double *scratch, u64 cap, u64 count, double **out);
 int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt);
 
 // sum_{i: bit q of i = 1} |raw_i / norm|^2 -> *dst (device scalar).

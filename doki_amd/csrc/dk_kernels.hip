@@ -1178,7 +1178,8 @@ void dk_tile_args(unsigned n, unsigned k, const unsigned *targets, TileArgs &ta,
 }
 
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
-		    const unsigned *targets, u64 cm, u64 acm)
+		    const unsigned *targets, u64 cm, u64 acm, double *scratch,
+		    u64 scratch_cap)
 {
 	double *parts = nullptr;
 	u64 nblocks = 0;
@@ -1192,9 +1193,13 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	DK_TRY(dk_time_mark(0));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
 	    in->nq >= dk_dense_tile_bits(g->k, in->dtype)) {
-		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks);
+		rc = dk_launch_apply_dense(in, out, g, targets, &parts, &nblocks,
+					   scratch, scratch_cap);
 	} else {
-		rc = dk_partials_reserve(std::max<u64>(in->len / 64, 1), &parts);
+		// every non-dense form leaves at most one partial per 64 amplitudes
+		const u64 bound = std::max<u64>(in->len / 64, 1);
+		rc = scratch ? dk_partials_in(scratch, scratch_cap, bound, &parts)
+			     : dk_partials_reserve(bound, &parts);
 		if (rc)
 			return rc;
 		rc = in->dtype == DK_C64
@@ -1229,13 +1234,32 @@ int dk_finish_partials(double *parts, u64 count, double *dst, int take_sqrt)
 	return dk_finish_partials(second, nb, dst, take_sqrt);
 }
 
-// Ensures the shared partials buffer holds `count` values plus the room the
-// chunked finishing pass needs after them.  Growth is stream-ordered.
-int dk_partials_reserve(u64 count, double **out)
+u64 dk_partials_need(u64 count)
 {
 	u64 need = count + 16;
 	for (u64 c = count; c > 4 * RCHUNK; c = (c + RCHUNK - 1) / RCHUNK)
 		need += (c + RCHUNK - 1) / RCHUNK;
+	return need;
+}
+
+int dk_partials_in(double *scratch, u64 cap, u64 count, double **out)
+{
+	const u64 need = dk_partials_need(count);
+	if (need > cap) {
+		dk_set_error("the launch needs %llu scratch doubles; the fixed "
+			     "scratch holds %llu",
+			     (unsigned long long)need, (unsigned long long)cap);
+		return DK_E_ARG;
+	}
+	*out = scratch;
+	return DK_OK;
+}
+
+// Ensures the shared partials buffer holds `count` values plus the room the
+// chunked finishing pass needs after them.  Growth is stream-ordered.
+int dk_partials_reserve(u64 count, double **out)
+{
+	const u64 need = dk_partials_need(count);
 	if (need > dk_cur()->partials_cap) {
 		double *p = nullptr;
 		DK_TRY(dk_device_alloc(need * sizeof(double), (void **)&p));

@@ -11,6 +11,13 @@
 
 dk_ctx g_ctx;
 static thread_local char t_err[512];
+
+// Block partials of dk_apply_noalloc: a fixed device array, so that entry
+// point allocates nothing.  Host code never passes `scratch` itself to a
+// kernel (on the host the name is only a stand-in); it uses the device
+// address taken at init (g_ctx.noalloc_scratch).
+enum : u64 { DK_NOALLOC_DOUBLES = DK_NOALLOC_SCRATCH_BYTES / sizeof(double) };
+static __device__ __align__(256) double scratch[DK_NOALLOC_DOUBLES];
 static bool g_closed = false;
 
 void dk_set_error(const char *fmt, ...)
@@ -86,6 +93,10 @@ extern "C" int dk_init(int device)
 	}
 	g_ctx.device = device;
 	g_ctx.num_cus = prop.multiProcessorCount;
+	// the no-alloc scratch's address on this device (the device is current)
+	double *sp = nullptr;
+	DK_HIP(hipGetSymbolAddress((void **)&sp, HIP_SYMBOL(scratch)));
+	g_ctx.noalloc_scratch = sp;
 	int rc = dk_lane_setup(&g_ctx.lane0);
 	if (rc)
 		return rc;
@@ -542,15 +553,11 @@ extern "C" void dk_gate_free(dk_gate *g)
 
 // ---- operations ----------------------------------------------------------------
 
-extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
-			const unsigned *targets, uint64_t ctrl_mask,
-			uint64_t actrl_mask, int in_place, dk_state **out)
+// Argument checks shared by dk_apply and dk_apply_noalloc.
+static int check_apply_args(const dk_state *in, const dk_gate *gate,
+			    const unsigned *targets, u64 ctrl_mask,
+			    u64 actrl_mask)
 {
-	DK_INIT();
-	if (!in || !gate || !targets || !out) {
-		dk_set_error("NULL argument");
-		return DK_E_ARG;
-	}
 	const unsigned n = in->nq;
 	const u64 valid = n >= 64 ? ~0ull : ((1ull << n) - 1);
 	u64 tmask = 0;
@@ -577,6 +584,20 @@ extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
 		dk_set_error("A target cannot also be a control or an anticontrol");
 		return DK_E_ARG;
 	}
+	return DK_OK;
+}
+
+extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
+			const unsigned *targets, uint64_t ctrl_mask,
+			uint64_t actrl_mask, int in_place, dk_state **out)
+{
+	DK_INIT();
+	if (!in || !gate || !targets || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	DK_TRY(check_apply_args(in, gate, targets, ctrl_mask, actrl_mask));
+	const unsigned n = in->nq;
 	DK_SETTLE(in);
 	dk_state *o = in;
 	if (!in_place) {
@@ -592,6 +613,43 @@ extern "C" int dk_apply(dk_state *in, const dk_gate *gate,
 	}
 	o->lane = dk_cur();
 	*out = o;
+	return DK_OK;
+}
+
+// dk_apply into a state the caller owns, with the block partials of the
+// norm in the fixed scratch array: no device memory is allocated or freed.
+// Calls on one device share the array, so calls that overlap on the device
+// (from different lanes) race; every partial is written before it is read,
+// so nothing in the array needs clearing between calls.
+extern "C" int dk_apply_noalloc(dk_state *in, const dk_gate *gate,
+				const unsigned *targets, uint64_t ctrl_mask,
+				uint64_t actrl_mask, dk_state *out)
+{
+	if (!in || !gate || !targets || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	if (!g_ctx.ready || !g_ctx.noalloc_scratch) {
+		dk_set_error("the library is not initialised (dk_init)");
+		return DK_E_DEVICE;
+	}
+	if (in->nq > DK_NOALLOC_MAX_QUBITS) {
+		dk_set_error("dk_apply_noalloc: a state of %u qubits exceeds the "
+			     "fixed scratch (%u qubits at most)", in->nq,
+			     (unsigned)DK_NOALLOC_MAX_QUBITS);
+		return DK_E_ARG;
+	}
+	if (out != in && (out->nq != in->nq || out->dtype != in->dtype)) {
+		dk_set_error("dk_apply_noalloc: the output state must have the "
+			     "input's qubit count and amplitude type");
+		return DK_E_ARG;
+	}
+	DK_TRY(check_apply_args(in, gate, targets, ctrl_mask, actrl_mask));
+	DK_SETTLE(in);
+	DK_SETTLE(out);
+	DK_TRY(dk_launch_apply(in, out, gate, targets, ctrl_mask, actrl_mask,
+			       g_ctx.noalloc_scratch, DK_NOALLOC_DOUBLES));
+	out->lane = dk_cur();
 	return DK_OK;
 }
 

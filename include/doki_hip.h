@@ -162,6 +162,22 @@ int dk_apply(dk_state *in, const dk_gate *gate, const unsigned *targets,
 	     uint64_t ctrl_mask, uint64_t actrl_mask, int in_place,
 	     dk_state **out);
 
+/* dk_apply writing into `out`, a live state the caller owns with in's qubit
+ * count and amplitude type (out == in applies in place), that allocates no
+ * device memory: the block partials of the norm live in a fixed device array
+ * of DK_NOALLOC_SCRATCH_BYTES, so states of at most DK_NOALLOC_MAX_QUBITS
+ * qubits are accepted (DK_E_ARG beyond).  Values and norm equal dk_apply's.
+ * Every call on the device shares that array: calls that overlap on the
+ * device (issued from different lanes) race and must be ordered by the
+ * caller.  The library must be initialised (dk_init, or any earlier call).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid it). */
+#define DK_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define DK_NOALLOC_MAX_QUBITS 26
+int dk_apply_noalloc(dk_state *in, const dk_gate *gate,
+		     const unsigned *targets, uint64_t ctrl_mask,
+		     uint64_t actrl_mask, dk_state *out);
+
 /* P(qubit q = 1) = sum over indices with bit q set of |raw/norm|^2.
  * Replaces probability, qops.c:72-95 (registry_prob, doki.c:1194-1228). */
 int dk_prob(const dk_state *s, unsigned q, double *p);
