@@ -4,8 +4,11 @@
 Workload (the plain per-gate GPU path): a 30-qubit complex128 register, the
 brickwork circuit — each layer is U3 (angles ~ U[0, 2pi)) on every qubit then
 CZ on alternating neighbour pairs — applied one `registry_apply` at a time
-through the drop-in `doki` module (out-of-place, norm fused, as the
-reference's apply_gate).  One step = one layer (30 U3 + 15 or 14 CZ).
+through the drop-in `doki` module in its per-gate execution mode: every gate
+is one pass over the state with the norm fused (as the reference's
+apply_gate), run in place once the registry it was applied to is no longer
+referenced (the caller's loop drops it).  One step = one layer (30 U3 + 15
+or 14 CZ).
 
 value = whole-job HBM GB/s per gate = gates * 32 * 2^n bytes / timed seconds
 (every unfused gate reads and writes each amplitude once: 16 B + 16 B).
@@ -55,6 +58,7 @@ def gpu_leg(args):
     import doki_amd
     from doki_amd import capi, workloads as W
     d = doki_amd.doki
+    d.execution_mode("gates", False)
     n = args.qubits
     bpg = 32 * (1 << n)
     layers = make_layers(d, W, n, args.warmup + args.steps + 1)
@@ -63,7 +67,7 @@ def gpu_leg(args):
     reg = d.registry_new(n, False)
     for li in range(args.warmup):
         reg = run_layer(d, reg, layers[li])
-    capi.synchronize()
+    d.registry_sync(reg, False)
     log("[bench] warmup done")
     # every gate kernel of the timed region records a start and an end HIP
     # event on the library's stream (no synchronisation in between)
@@ -74,7 +78,7 @@ def gpu_leg(args):
     for li in range(args.warmup, args.warmup + args.steps):
         reg = run_layer(d, reg, layers[li])
         ngates += len(layers[li])
-    capi.synchronize()
+    d.registry_sync(reg, False)      # the last gate runs, then the device drains
     t1 = time.perf_counter()
     elapsed = t1 - t0
     kms = capi.time_ring_read()
@@ -97,6 +101,7 @@ def gpu_leg(args):
     for t in sorted({0, 3, 6, 12, n // 2, n - 10, n - 5, n - 1}):
         for _ in range(5):
             reg = d.registry_apply(reg, u3, [t], set(), None, -1, False)
+        d.registry_sync(reg, False)
         ms = capi.time_ring_read()
         per_target[str(t)] = {"kernel_ms": min(ms), "avg_ms": sum(ms) / len(ms),
                               "GBps": bpg / (min(ms) * 1e-3) / 1e9}
@@ -123,6 +128,7 @@ def gpu_leg(args):
     tp1 = time.perf_counter()
     assert all(0.0 <= p <= 1.0 + 1e-12 for p in probs)
 
+    d.execution_mode("fuse", False)
     return {
         "elapsed": elapsed, "ngates": ngates, "bpg": bpg, "kavg_ms": kavg,
         "kernel_GBps_timed_by_target": by_target, "per_target": per_target,
@@ -178,19 +184,22 @@ def big_leg(args):
            "gates": len(steps), "bytes_per_gate": bpg}
     gates = [(d.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
     # (loops inline: a caller's reference would keep a third state alive)
-    reg = d.registry_new(n, False)
-    g0, t_, c_ = gates[0]
-    reg = d.registry_apply(reg, g0, t_, c_, None, -1, False)  # second state
-    capi.synchronize()
-    t0 = time.perf_counter()
-    for g, t_, c_ in gates:
-        reg = d.registry_apply(reg, g, t_, c_, None, -1, False)
-    capi.synchronize()
-    el = time.perf_counter() - t0
-    out["out_of_place_GBps"] = round(len(gates) * bpg / el / 1e9, 1)
-    out["out_of_place_ms_per_gate"] = round(el / len(gates) * 1e3, 3)
-    del reg
-    capi.synchronize()
+    for mode, key in (("eager", "out_of_place"), ("gates", "drop_in_in_place")):
+        prev = d.execution_mode(mode, False)
+        reg = d.registry_new(n, False)
+        g0, t_, c_ = gates[0]
+        reg = d.registry_apply(reg, g0, t_, c_, None, -1, False)  # second state
+        d.registry_sync(reg, False)
+        t0 = time.perf_counter()
+        for g, t_, c_ in gates:
+            reg = d.registry_apply(reg, g, t_, c_, None, -1, False)
+        d.registry_sync(reg, False)
+        el = time.perf_counter() - t0
+        d.execution_mode(prev, False)
+        out[f"{key}_GBps"] = round(len(gates) * bpg / el / 1e9, 1)
+        out[f"{key}_ms_per_gate"] = round(el / len(gates) * 1e3, 3)
+        del reg
+        capi.synchronize()
     s = capi.new(n)
     cg = [(capi.Gate(m), t, c) for m, t, c, a in steps]
     capi.apply(s, cg[0][0], cg[0][1], cg[0][2], in_place=True)
@@ -212,6 +221,52 @@ def big_leg(args):
     del s
     capi.synchronize()
     log(f"[bench] {n} qubits: {out}")
+    return out
+
+
+def dropin_leg(args):
+    """The brickwork circuit (the headline register, depth args.fused_depth)
+    and the marginals of every qubit, through plain `doki` calls only, under
+    each execution mode of the drop-in module: "eager" (each registry_apply
+    launches at once, out of place, as the reference does), "gates" (one
+    pass per gate, in place once the input registry is unreferenced) and
+    "fuse" (the default: the gates run as cache-blocked passes when the
+    registry is read).  Circuits/s, and each mode's marginals against
+    eager's."""
+    import numpy as np
+
+    import doki_amd
+    from doki_amd import capi, workloads as W
+    d = doki_amd.doki
+    n = args.qubits
+    steps = W.brickwork_circuit(77, n, args.fused_depth)
+    gates = [(d.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
+    out = {"qubits": n, "depth": args.fused_depth, "gates": len(steps)}
+    probs = {}
+    for mode in ("eager", "gates", "fuse"):
+        prev = d.execution_mode(mode, False)
+        runs = []
+        for rep in range(2):            # the first run warms the gate uploads
+            reg = d.registry_new(n, False)
+            d.registry_sync(reg, False)
+            t0 = time.perf_counter()
+            for g, t, c in gates:
+                reg = d.registry_apply(reg, g, t, c, None, -1, False)
+            p = [d.registry_prob(reg, q, -1, False) for q in range(n)]
+            runs.append(time.perf_counter() - t0)
+            del reg
+            capi.synchronize()
+        d.execution_mode(prev, False)
+        probs[mode] = np.array(p)
+        out[mode] = {"seconds": round(runs[-1], 4),
+                     "circuits_per_s": round(1.0 / runs[-1], 3)}
+        if mode != "eager":
+            out[mode]["max_abs_prob_vs_eager"] = float(
+                np.max(np.abs(probs[mode] - probs["eager"])))
+        log(f"[bench] drop-in {mode}: {out[mode]}")
+    for mode in ("gates", "fuse"):
+        out[f"{mode}_speedup_vs_eager"] = round(
+            out["eager"]["seconds"] / out[mode]["seconds"], 2)
     return out
 
 
@@ -477,23 +532,41 @@ def serve_leg(args):
     return out
 
 
-def cpu_leg(args, gb_per_gate_unit=True):
-    """The reference's own CPU path (oracle/_ref, compiled from its sources),
-    OpenMP on the host cores, over a bounded sample of the same workload:
-    brickwork-layer U3 gates at a smaller register, until ~cpu_seconds."""
+def cpu_model():
+    """The host CPU's model name (as lscpu prints it), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(args):
+    """The reference's own CPU path (oracle/_ref: its extension compiled from
+    its sources) on the host cores, over bounded samples of the workloads:
+
+    * the headline workload itself: brickwork gates (U3 on a low and a high
+      target, a CZ) on a 30-qubit register with every host thread the run
+      may use: seconds per gate and GB/s (the value reported);
+    * one brickwork layer at 24 qubits, with those threads and with one;
+    * the 12-qubit, 200-gate parity circuit in full (marginals and a full
+      measurement), with those threads and with one;
+    * a subset of the batched-registers workload, registers one at a time.
+
+    The thread count goes to the reference through registry_apply's
+    num_threads argument (its omp_set_num_threads)."""
     from oracle import oracle as O
     from doki_amd import workloads as W
     ref = O.load_reference()
-    kind = "reference"
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    n = args.cpu_qubits
-    rng = W.rng_for(7)
-    steps = []
-    for layer in range(40):
-        steps += W.brickwork_layer(rng, n, layer)
+    info = {"cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     if ref is None:
-        # fall back to the single-threaded oracle port
-        kind, threads = "port", 1
+        # the single-threaded oracle port, 24 qubits
+        n = args.cpu_qubits
+        steps = W.brickwork_layer(W.rng_for(7), n, 0)
         st = O.new(n)
         t0 = time.perf_counter()
         g = 0
@@ -502,41 +575,106 @@ def cpu_leg(args, gb_per_gate_unit=True):
             st = O.apply(st, m, t, sorted(c), sorted(a))
             g += 1
         el = time.perf_counter() - t0
-    else:
-        gates = [(ref.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
-        reg = ref.registry_new(n, False)
-        # one untimed gate (first touch of the allocator)
-        reg = ref.registry_apply(reg, gates[0][0], gates[0][1], gates[0][2], None,
-                                 threads, False)
+        return {"value": round(g * 32 * (1 << n) / el / 1e9, 3), "unit": "GB/s",
+                "cores": 1, "kind": "port",
+                "sample": f"{g} brickwork gates at n={n} through the single-threaded "
+                          "oracle port", **info}
+
+    def gate(m):
+        return ref.gate_new(1, m.tolist(), False)
+
+    def run_gates(reg, gl, nthr, budget=None):
         t0 = time.perf_counter()
-        g = 0
-        while time.perf_counter() - t0 < args.cpu_seconds and g < len(gates):
-            gg, t, c = gates[g]
-            reg = ref.registry_apply(reg, gg, t, c, None, threads, False)
-            g += 1
-        el = time.perf_counter() - t0
-    val = g * 32 * (1 << n) / el / 1e9
-    return {"value": round(val, 3), "unit": "GB/s", "cores": threads, "kind": kind,
-            "sample": f"{g} brickwork gates (U3/CZ) at n={n} complex128, "
-                      f"{el:.1f} s, registry_apply of the reference's OpenMP path"
-                      if kind == "reference" else
-                      f"{g} gates at n={n} through the single-threaded oracle port"}
+        done = 0
+        for gg, t, c in gl:
+            reg = ref.registry_apply(reg, gg, t, c, set(), nthr, False)
+            done += 1
+            if budget is not None and time.perf_counter() - t0 > budget:
+                break
+        return reg, done, time.perf_counter() - t0
+
+    # headline register: 30 qubits, a low-target U3, a high-target U3, a CZ
+    n = args.qubits
+    lay = W.brickwork_layer(W.rng_for(7), n, 0)
+    pick = [lay[0], lay[n - 1], lay[n]]            # U3 t=0, U3 t=n-1, CZ (0,1)
+    gl = [(gate(m), t, set(c)) for m, t, c, a in pick]
+    reg = ref.registry_new(n, False)
+    reg, _, _ = run_gates(reg, gl[:1], threads)    # first touch of both states
+    per = {}
+    for (gg, t, c), name in zip(gl, ("u3_t0", f"u3_t{n - 1}", "cz_0_1")):
+        reg, _, el = run_gates(reg, [(gg, t, c)], threads)
+        per[name] = round(el, 3)
+    del reg
+    sec = sum(per.values()) / len(per)
+    value = 32 * (1 << n) / sec / 1e9
+
+    # one brickwork layer at 24 qubits, all threads and one thread
+    m24 = args.cpu_qubits
+    lay24 = [(gate(m), t, set(c)) for m, t, c, a in W.brickwork_layer(W.rng_for(8), m24, 0)]
+    layer24 = {}
+    for nthr in (threads, 1):
+        reg = ref.registry_new(m24, False)
+        reg, done, el = run_gates(reg, lay24, nthr, budget=args.cpu_seconds / 3)
+        layer24[f"threads_{nthr}"] = {"gates": done, "seconds_per_gate": round(el / done, 4),
+                                      "GBps": round(done * 32 * (1 << m24) / el / 1e9, 2)}
+        del reg
+
+    # the 12-qubit parity circuit in full
+    steps, rolls = W.random_circuit(0, 12, 200)
+    c0g = [(ref.gate_new(1, m.tolist(), False), t, set(c), set(a)) for m, t, c, a in steps]
+    c0 = {}
+    for nthr in (threads, 1):
+        t0 = time.perf_counter()
+        reg = ref.registry_new(12, False)
+        for gg, t, c, a in c0g:
+            reg = ref.registry_apply(reg, gg, t, c, a, nthr, False)
+        _ = [ref.registry_prob(reg, q, nthr, False) for q in range(12)]
+        ref.registry_measure(reg, (1 << 12) - 1, rolls, nthr, False)
+        c0[f"threads_{nthr}_seconds"] = round(time.perf_counter() - t0, 4)
+
+    # batched-registers subset, one register at a time (all threads)
+    specs = W.batch_specs(31, args.cpu_batch_registers)
+    t0 = time.perf_counter()
+    for nq, st_, rl, keep in specs:
+        reg = ref.registry_new(nq, False)
+        for m, t, c, a in st_:
+            reg = ref.registry_apply(reg, ref.gate_new(1, m.tolist(), False), t, c, a,
+                                     threads, False)
+        if keep:
+            rho = ref.registry_density(reg, False)
+            for e in range(nq - 1, 5, -1):
+                rho = ref.funmatrix_partialtrace(rho, e, False)
+            ref.funmatrix_get(rho, 0, 0, False)
+        ref.registry_measure(reg, (1 << nq) - 1, rl, threads, False)
+    el = time.perf_counter() - t0
+    batch = {"registers": len(specs), "seconds": round(el, 3),
+             "circuits_per_s": round(len(specs) / el, 1)}
+    return {"value": round(value, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
+            "sample": f"U3 t=0, U3 t={n - 1} and CZ(0,1) on a {n}-qubit complex128 "
+                      f"register, registry_apply of the reference's OpenMP path "
+                      f"with {threads} threads, {sec:.2f} s per gate",
+            **info, "headline_register": {"qubits": n, "seconds_per_gate": per},
+            "brickwork_layer": {"qubits": m24, **layer24},
+            "c0_12q_200_gates": c0, "batch_subset": batch}
 
 
 def measured_traffic(n):
     """HBM bytes per launch of the 1-qubit gate kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (copy-kernel
-    calibrated); None when no profile covers this register size."""
-    path = os.path.join(ROOT, "profiles", f"apply1_{n}q_c128_hbm_traffic.txt")
+    calibrated), and where they come from (file and the commit measured);
+    (None, None) when no profile covers this register size."""
+    name = f"apply1_{n}q_c128_hbm_traffic.txt"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
     vals = {}
     with open(path) as f:
         for line in f:
             if ":" in line and not line.startswith("#"):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
-    return int(float(vals["traffic_bytes_per_launch"]))
+    src = f"profiles/{name}, measured at commit {vals.get('measured_at_commit', 'unknown')}"
+    return int(float(vals["traffic_bytes_per_launch"])), src
 
 
 def main():
@@ -546,7 +684,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--cpu-qubits", type=int, default=24)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--cpu-batch-registers", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--fused-depth", type=int, default=10)
@@ -571,6 +710,7 @@ def main():
     g = gpu_leg(args)
     big = big_leg(args) if args.big_qubits else None
     joins = join_leg(args.qubits + 2)
+    dropin = None if args.no_fused else dropin_leg(args)
     fused = None if args.no_fused else fused_leg(args)
     batched = None if args.no_batch else batch_leg(args)
     served = None if args.no_serve else serve_leg(args)
@@ -578,6 +718,7 @@ def main():
     if cpu:
         log(f"[bench] cpu baseline {cpu}")
     value = g["ngates"] * g["bpg"] / g["elapsed"] / 1e9
+    traffic, traffic_src = measured_traffic(args.qubits)
     achieved = g["bpg"] / (g["kavg_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -594,7 +735,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"brickwork circuit, {args.qubits}-qubit complex128 register, "
-                        "plain per-gate registry_apply (out-of-place, fused norm); "
+                        "plain per-gate registry_apply through the drop-in module "
+                        "(execution_mode 'gates': one state pass per gate, fused norm, "
+                        "in place once the input registry is unreferenced); "
                         "one step = one layer of U3 on every qubit + CZ pairs",
             "qubits": args.qubits,
             "gates_timed": g["ngates"],
@@ -606,7 +749,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.qubits),
+            "traffic": traffic,
+            "traffic_source": traffic_src,
         },
         "cpu_baseline": cpu,
         "extras": {
@@ -617,6 +761,7 @@ def main():
             "registry_prob_all_qubits_ms": round(g["prob_all_ms"], 3),
             "large_register": big,
             "join_prob_all": joins,
+            "drop_in_modes": dropin,
             "fused": fused,
             "batched": batched,
             "served": served,

@@ -66,15 +66,74 @@ static int check_threads(int num_threads)
 	return 0;
 }
 
+/* A gate capsule: its matrix on the host, and on the device once used. */
+typedef struct {
+	unsigned k;
+	unsigned dim;
+	double *mat;     /* dim*dim interleaved, row-major */
+	dk_gate *dev;    /* created on first use */
+} gate_t;
+
+static dk_gate *gate_device(gate_t *g)
+{
+	if (!g->dev) {
+		int rc = dk_gate_new(g->k, g->mat, &g->dev);
+		if (rc) {
+			dk_fail(rc, "Failed to apply gate");
+			return NULL;
+		}
+	}
+	return g->dev;
+}
+
 /* ------------------------------------------------------------------------ */
 /* registries                                                               */
 /* ------------------------------------------------------------------------ */
 
 #define CHUNK 4096 /* amplitudes per read-back chunk of registry_get */
 
+/*
+ * Registries are values (every registry_apply returns a new one, doki.c:939,
+ * qops.c:192), so the device work behind them may run later than the call
+ * that asked for it, as long as every read sees the reference's value.  A
+ * registry is either materialised (st holds its state) or pending: it is
+ * its parent registry with one more gate.  registry_apply makes pending
+ * registries; anything that reads a registry materialises it first, running
+ * the chain of pending gates from the nearest materialised ancestor:
+ *
+ *  - EXEC_FUSE (the default): the whole chain at once, as cache-blocked
+ *    passes (dk_circuit_passes: one state pass for many gates);
+ *  - EXEC_GATES: one gate per registry_apply, deferred by one call (the call
+ *    that applies a gate to a pending registry first runs that registry's
+ *    own gate), so each gate runs when it is known whether its input is
+ *    still referenced;
+ *  - EXEC_EAGER: each registry_apply launches its gate at once, out of place,
+ *    as the reference does.
+ *
+ * A chain runs in place on its ancestor's state when nothing else can read
+ * that ancestor: its capsule is referenced only by the chain (Python
+ * reference count 1, or the registry was deleted) and no other pending
+ * registry derives from it.  Otherwise the first gate writes a new state.
+ * Pending registries that something else references (a caller's variable,
+ * another pending registry, a density) are materialised on the way, so a
+ * branch taken from an intermediate registry reads that registry's value.
+ */
+enum { EXEC_EAGER = 0, EXEC_GATES = 1, EXEC_FUSE = 2 };
+static int g_exec = EXEC_FUSE;
+/* pending gates after which registry_apply materialises a chain anyway */
+#define FUSE_MAX_PENDING 2048
+
 typedef struct {
-	dk_state *st;       /* NULL once deleted */
+	dk_state *st;       /* the state; NULL while pending or once deleted */
 	unsigned nq;
+	int deleted;        /* registry_del was called */
+	/* pending: this registry is `parent` with `gate` applied */
+	PyObject *parent;   /* registry capsule (a reference is held) */
+	PyObject *gate;     /* gate capsule (a reference is held) */
+	unsigned targets[DK_MAX_GATE_QUBITS];
+	uint64_t cm, am;
+	unsigned depth;     /* pending gates from the materialised ancestor */
+	unsigned children;  /* pending registries whose parent this is */
 	int norm_known;
 	double norm;
 	int phase_known;
@@ -85,10 +144,37 @@ typedef struct {
 	double *probs;       /* all single-qubit probabilities, or NULL */
 } reg_t;
 
+static reg_t *reg_of(PyObject *cap)
+{
+	return (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+}
+
+/* Detaches a pending registry from its parent (the caller releases the
+ * returned reference).  A deleted parent that no pending registry needs any
+ * more gives its state back at once. */
+static PyObject *reg_detach(reg_t *r)
+{
+	PyObject *p = r->parent;
+	r->parent = NULL;
+	Py_CLEAR(r->gate);
+	if (p) {
+		reg_t *pr = reg_of(p);
+		if (pr) {
+			pr->children--;
+			if (pr->deleted && pr->children == 0 && pr->st) {
+				dk_state_free(pr->st);
+				pr->st = NULL;
+			}
+		}
+	}
+	return p;
+}
+
 static void reg_free(reg_t *r)
 {
 	if (!r)
 		return;
+	Py_XDECREF(reg_detach(r));
 	if (r->st)
 		dk_state_free(r->st);
 	PyMem_Free(r->cache);
@@ -98,39 +184,171 @@ static void reg_free(reg_t *r)
 
 static void reg_capsule_destroy(PyObject *cap)
 {
-	reg_t *r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	reg_t *r = reg_of(cap);
 	if (r)
 		reg_free(r);
+}
+
+static PyObject *reg_capsule(reg_t *r)
+{
+	PyObject *cap = PyCapsule_New(r, CAP_STATE, reg_capsule_destroy);
+	if (!cap)
+		reg_free(r);
+	return cap;
 }
 
 static PyObject *reg_wrap(dk_state *st)
 {
 	reg_t *r = (reg_t *)PyMem_Calloc(1, sizeof(reg_t));
-	PyObject *cap;
 	if (!r) {
 		dk_state_free(st);
 		return PyErr_NoMemory();
 	}
 	r->st = st;
 	r->nq = dk_state_num_qubits(st);
-	cap = PyCapsule_New(r, CAP_STATE, reg_capsule_destroy);
-	if (!cap)
-		reg_free(r);
-	return cap;
+	return reg_capsule(r);
 }
 
-/* The registry of a capsule, or NULL with doki.error set. */
-static reg_t *reg_get(PyObject *cap, const char *nullmsg)
+/* The registry of a capsule, materialised or not, or NULL with doki.error
+ * set. */
+static reg_t *reg_peek(PyObject *cap, const char *nullmsg)
 {
-	reg_t *r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	reg_t *r = reg_of(cap);
 	if (!r) {
 		PyErr_SetString(DokiError, nullmsg);
 		return NULL;
 	}
-	if (!r->st) {
+	if (r->deleted) {
 		PyErr_SetString(DokiError, "registry has been deleted");
 		return NULL;
 	}
+	return r;
+}
+
+/* Runs `count` pending gates (nodes[0] first) on `in`, in place or into a
+ * new state (*out). */
+static int run_gates(dk_state *in, int in_place, reg_t **nodes, unsigned count,
+		     dk_state **out)
+{
+	dk_circuit *c = NULL;
+	int rc;
+	*out = NULL;
+	if (count > 1 && g_exec == EXEC_FUSE) {
+		rc = dk_circuit_new(dk_state_num_qubits(in), &c);
+		for (unsigned i = 0; !rc && i < count; i++) {
+			gate_t *g = (gate_t *)PyCapsule_GetPointer(nodes[i]->gate, CAP_GATE);
+			rc = dk_circuit_add(c, g->k, g->mat, nodes[i]->targets,
+					    nodes[i]->cm, nodes[i]->am);
+		}
+		if (!rc)
+			rc = dk_circuit_passes(c, NULL);
+		if (!rc)
+			rc = dk_circuit_run(c, in, in_place, out);
+		dk_circuit_free(c);
+		return rc;
+	}
+	for (unsigned i = 0; i < count; i++) {
+		gate_t *g = (gate_t *)PyCapsule_GetPointer(nodes[i]->gate, CAP_GATE);
+		dk_gate *dg = gate_device(g);
+		dk_state *o = NULL;
+		if (!dg)
+			return DK_E_ALLOC;
+		rc = dk_apply(i == 0 ? in : *out, dg, nodes[i]->targets, nodes[i]->cm,
+			      nodes[i]->am, i > 0 || in_place, &o);
+		if (rc) {
+			if (i > 0 && !in_place)
+				dk_state_free(*out);
+			*out = NULL;
+			return rc;
+		}
+		*out = o;
+	}
+	return DK_OK;
+}
+
+/* Materialises a pending registry (see EXEC_FUSE above); 0, or -1 with an
+ * exception set. */
+static int reg_materialise(PyObject *cap)
+{
+	reg_t *n = reg_of(cap);
+	PyObject **path;
+	reg_t **nodes;
+	unsigned len = 0, cap_len, i, root, seg;
+	int rc = 0;
+	if (n->st)
+		return 0;
+	cap_len = n->depth + 1;
+	path = (PyObject **)PyMem_Malloc(cap_len * sizeof(PyObject *));
+	nodes = (reg_t **)PyMem_Malloc(cap_len * sizeof(reg_t *));
+	if (!path || !nodes) {
+		PyMem_Free(path);
+		PyMem_Free(nodes);
+		PyErr_NoMemory();
+		return -1;
+	}
+	/* path[0] = this registry, ..., path[root] = materialised ancestor */
+	path[len++] = cap;
+	while (!reg_of(path[len - 1])->st) {
+		PyObject *p = reg_of(path[len - 1])->parent;
+		if (!p || len == cap_len) {
+			PyMem_Free(path);
+			PyMem_Free(nodes);
+			PyErr_SetString(DokiError, "registry has lost its source state");
+			return -1;
+		}
+		path[len++] = p;
+	}
+	root = len - 1;
+	/* segments from the root down: each ends at this registry or at a
+	 * pending registry something else references, which keeps its value */
+	seg = root;
+	for (i = root; i-- > 0;) {
+		reg_t *x = reg_of(path[i]);
+		if (i > 0 && Py_REFCNT(path[i]) == 1)
+			continue;
+		{
+			reg_t *src = reg_of(path[seg]);
+			unsigned cnt = seg - i, j;
+			int in_place = src->children == 1 &&
+				       (Py_REFCNT(path[seg]) == 1 || src->deleted);
+			dk_state *in = src->st, *out = NULL;
+			for (j = 0; j < cnt; j++)
+				nodes[j] = reg_of(path[seg - 1 - j]);
+			if (in_place)
+				src->st = NULL;     /* the chain takes the state */
+			rc = run_gates(in, in_place, nodes, cnt, &out);
+			if (rc) {
+				if (in_place)
+					src->st = in;
+				dk_fail(rc, "Failed to apply gate");
+				break;
+			}
+			x->st = out;
+			seg = i;
+		}
+	}
+	if (!rc) {
+		/* every registry on the path below the root now holds or no
+		 * longer needs its source: release the links (no cascade) */
+		for (i = 0; i < root; i++)
+			path[i] = reg_detach(reg_of(path[i]));
+		for (i = 0; i < root; i++)
+			Py_XDECREF(path[i]);
+	}
+	PyMem_Free(path);
+	PyMem_Free(nodes);
+	return rc ? -1 : 0;
+}
+
+/* The registry of a capsule with its state materialised, or NULL with
+ * doki.error set. */
+static reg_t *reg_get(PyObject *cap, const char *nullmsg)
+{
+	reg_t *r = reg_peek(cap, nullmsg);
+	if (!r)
+		return NULL;
+	if (!r->st && reg_materialise(cap))
+		return NULL;
 	return r;
 }
 
@@ -267,11 +485,20 @@ static PyObject *doki_registry_del(PyObject *self, PyObject *args)
 		PyErr_SetString(DokiError, "Syntax: registry_del(registry, verbose)");
 		return NULL;
 	}
-	r = (reg_t *)PyCapsule_GetPointer(cap, CAP_STATE);
+	r = reg_of(cap);
 	if (r) {
-		if (r->st)
-			dk_state_free(r->st);
-		r->st = NULL;
+		/* freed at once unless pending registries still derive from it
+		 * (then when the last of them no longer needs it) */
+		r->deleted = 1;
+		if (r->children == 0) {
+			if (r->st)
+				dk_state_free(r->st);
+			r->st = NULL;
+			Py_XDECREF(reg_detach(r));
+		}
+		PyMem_Free(r->cache);
+		r->cache = NULL;
+		r->cache_count = 0;
 	} else {
 		PyErr_Clear();
 	}
@@ -464,22 +691,17 @@ static PyObject *doki_registry_mem(PyObject *self, PyObject *args)
 		PyErr_SetString(DokiError, "Syntax: registry_mem(state, verbose)");
 		return NULL;
 	}
-	r = reg_get(cap, "NULL pointer to registry");
+	r = reg_peek(cap, "NULL pointer to registry");
 	if (!r)
 		return NULL;
-	return PyLong_FromUnsignedLongLong(dk_state_mem_size(r->st));
+	/* state_mem_size on LP64 (qstate.c:123-135), known before the state is
+	 * materialised */
+	return PyLong_FromUnsignedLongLong((uint64_t)64 + ((uint64_t)16 << r->nq));
 }
 
 /* ------------------------------------------------------------------------ */
 /* gates                                                                    */
 /* ------------------------------------------------------------------------ */
-
-typedef struct {
-	unsigned k;
-	unsigned dim;
-	double *mat;     /* dim*dim interleaved, row-major */
-	dk_gate *dev;    /* created on first use */
-} gate_t;
 
 static void gate_capsule_destroy(PyObject *cap)
 {
@@ -595,17 +817,6 @@ static PyObject *doki_gate_get(PyObject *self, PyObject *args)
 	return res;
 }
 
-static dk_gate *gate_device(gate_t *g)
-{
-	if (!g->dev) {
-		int rc = dk_gate_new(g->k, g->mat, &g->dev);
-		if (rc) {
-			dk_fail(rc, "Failed to apply gate");
-			return NULL;
-		}
-	}
-	return g->dev;
-}
 
 /* ------------------------------------------------------------------------ */
 /* registry_apply: argument checks in the reference's order and wording     */
@@ -657,7 +868,7 @@ static PyObject *doki_registry_apply(PyObject *self, PyObject *args)
 	}
 	if (check_threads(nthreads))
 		return NULL;
-	r = reg_get(scap, "NULL pointer to registry");
+	r = reg_peek(scap, "NULL pointer to registry");
 	if (!r)
 		return NULL;
 	g = (gate_t *)PyCapsule_GetPointer(gcap, CAP_GATE);
@@ -770,13 +981,38 @@ static PyObject *doki_registry_apply(PyObject *self, PyObject *args)
 	PyMem_Free(ctrl);
 	PyMem_Free(actrl);
 	ctrl = actrl = NULL;
-	dg = gate_device(g);
-	if (!dg)
+	if (g_exec == EXEC_EAGER) {
+		if (!r->st && reg_materialise(scap))
+			return NULL;
+		dg = gate_device(g);
+		if (!dg)
+			return NULL;
+		rc = dk_apply(r->st, dg, targets, cm, am, 0, &out);
+		if (rc)
+			return dk_fail(rc, "Failed to apply gate");
+		return reg_wrap(out);
+	}
+	/* a pending registry: the input's own gate runs first in EXEC_GATES,
+	 * and long chains are cut in EXEC_FUSE */
+	if (!r->st && (g_exec == EXEC_GATES || r->depth >= FUSE_MAX_PENDING) &&
+	    reg_materialise(scap))
 		return NULL;
-	rc = dk_apply(r->st, dg, targets, cm, am, 0, &out);
-	if (rc)
-		return dk_fail(rc, "Failed to apply gate");
-	return reg_wrap(out);
+	{
+		reg_t *x = (reg_t *)PyMem_Calloc(1, sizeof(reg_t));
+		if (!x)
+			return PyErr_NoMemory();
+		x->nq = r->nq;
+		x->parent = scap;
+		Py_INCREF(scap);
+		x->gate = gcap;
+		Py_INCREF(gcap);
+		memcpy(x->targets, targets, nt * sizeof(unsigned));
+		x->cm = cm;
+		x->am = am;
+		x->depth = r->st ? 1 : r->depth + 1;
+		r->children++;
+		return reg_capsule(x);
+	}
 fail:
 	PyMem_Free(ctrl);
 	PyMem_Free(actrl);
@@ -1085,13 +1321,13 @@ static PyObject *doki_registry_density(PyObject *self, PyObject *args)
 		PyErr_SetString(DokiError, "Syntax: registry_density(state, verbose)");
 		return NULL;
 	}
-	r = reg_get(cap, "NULL pointer to registry");
+	r = reg_peek(cap, "NULL pointer to registry");
 	if (!r)
 		return NULL;
 	f = fm_new(FM_DENSITY, (int64_t)1 << r->nq, (int64_t)1 << r->nq);
 	if (!f)
 		return NULL;
-	f->mem = REF_FMATRIX_BYTES + (size_t)dk_state_mem_size(r->st);
+	f->mem = REF_FMATRIX_BYTES + (size_t)64 + ((size_t)16 << r->nq);
 	f->state_cap = cap;
 	Py_INCREF(cap);
 	f->keep_count = r->nq;
@@ -2145,6 +2381,61 @@ fail:
 
 /* ------------------------------------------------------------------------ */
 
+/* execution_mode(mode, verbose): how registry_apply runs its gates (see
+ * EXEC_FUSE): "fuse" (chains run as cache-blocked passes when read, the
+ * default), "gates" (one gate per call, in place when its input is no
+ * longer referenced) or "eager" (each call launches its gate at once, out of
+ * place, as the reference does).  None only queries.  Returns the mode in
+ * force before the call.  Registries made under one mode read the same
+ * values under any other.  No reference counterpart. */
+static PyObject *doki_execution_mode(PyObject *self, PyObject *args)
+{
+	static const char *names[] = {"eager", "gates", "fuse"};
+	PyObject *m;
+	int verbose = 0, prev = g_exec;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "O|p", &m, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: execution_mode(mode, verbose)");
+		return NULL;
+	}
+	if (m != Py_None) {
+		const char *v = PyUnicode_Check(m) ? PyUnicode_AsUTF8(m) : NULL;
+		int k;
+		for (k = 0; v && k < 3; k++)
+			if (!strcmp(v, names[k]))
+				break;
+		if (!v || k == 3) {
+			PyErr_Clear();
+			PyErr_SetString(DokiError,
+					"mode must be \"fuse\", \"gates\", \"eager\" or None");
+			return NULL;
+		}
+		g_exec = k;
+	}
+	return PyUnicode_FromString(names[prev]);
+}
+
+/* registry_sync(registry, verbose): runs every pending gate of the registry
+ * and waits until the device has finished the work queued so far (for
+ * timing; reads do not need it).  No reference counterpart: the reference
+ * computes in the call. */
+static PyObject *doki_registry_sync(PyObject *self, PyObject *args)
+{
+	PyObject *cap;
+	int verbose, rc;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_sync(registry, verbose)");
+		return NULL;
+	}
+	if (!reg_get(cap, "NULL pointer to registry"))
+		return NULL;
+	rc = dk_synchronize();
+	if (rc)
+		return dk_fail(rc, "Failed to synchronise");
+	Py_RETURN_NONE;
+}
+
 static PyMethodDef DokiMethods[] = {
 	{"gate_new", doki_gate_new, METH_VARARGS, "Create new gate"},
 	{"gate_get", doki_gate_get, METH_VARARGS, "Get matrix associated to gate"},
@@ -2166,6 +2457,10 @@ static PyMethodDef DokiMethods[] = {
 	 "Get the density matrix"},
 	{"registry_mem", doki_registry_mem, METH_VARARGS,
 	 "Get the memory allocated by this registry in bytes"},
+	{"registry_sync", doki_registry_sync, METH_VARARGS,
+	 "Run the registry's pending gates and wait for the device"},
+	{"execution_mode", doki_execution_mode, METH_VARARGS,
+	 "Get or set how registry_apply runs gates: fuse, gates or eager"},
 	{"funmatrix_create", doki_funmatrix_create, METH_VARARGS,
 	 "Create a functional matrix from a matrix"},
 	{"funmatrix_identity", doki_funmatrix_identity, METH_VARARGS,

@@ -101,11 +101,13 @@ def _brickwork_closed_form(steps, n):
     return amp, np.abs(col[:, 1]) ** 2
 
 
-def test_33_qubit_brickwork_layer_closed_form(doki, capi):
+@pytest.mark.parametrize("mode", ["eager", "fuse"])
+def test_33_qubit_brickwork_layer_closed_form(doki, capi, mode):
     """The 33-qubit register (137.4 GB complex128) of the brickwork workload:
-    one layer from |0...0>, out of place through the drop-in module (old and
-    new state together: 274.9 GB) and in place through the C ABI; sampled
-    amplitudes and every marginal against the closed form."""
+    one layer from |0...0> through the drop-in module, gate by gate out of
+    place ("eager": old and new state together, 274.9 GB) or as
+    cache-blocked passes in place ("fuse"), then in place through the C ABI;
+    sampled amplitudes and every marginal against the closed form."""
     n = 33
     steps = W.brickwork_layer(W.rng_for(3333), n, 0)
     assert len(steps) == 33 + 16
@@ -116,9 +118,14 @@ def test_33_qubit_brickwork_layer_closed_form(doki, capi):
                                    dtype=np.uint64)])
     capi.check(capi.lib().dk_memory_trim())
     gates = [(doki.gate_new(1, m.tolist(), False), t, set(c)) for m, t, c, a in steps]
-    reg = doki.registry_new(n, False)
-    for g, t, c in gates:
-        reg = doki.registry_apply(reg, g, t, c, set(), -1, False)
+    prev = doki.execution_mode(mode, False)
+    try:
+        reg = doki.registry_new(n, False)
+        for g, t, c in gates:
+            reg = doki.registry_apply(reg, g, t, c, set(), -1, False)
+        doki.registry_sync(reg, False)
+    finally:
+        doki.execution_mode(prev, False)
     got = np.array([doki.registry_get(reg, int(i), False, False) for i in idx])
     assert np.max(np.abs(got - amp(idx))) <= 1e-12
     probs = np.array([doki.registry_prob(reg, q, -1, False) for q in range(n)])
