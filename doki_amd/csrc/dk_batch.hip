@@ -14,11 +14,15 @@
 // each tile (controls outside the tile are decided per tile).  Registers of
 // at least "split_qubits" qubits (17 by default) would hold one workgroup
 // for long (a 2^20-amplitude register streams at about one CU's share of
-// HBM), so their passes run before the persistent launch as rounds: round r
-// is one launch in which every workgroup takes tiles of the r-th pass of
+// HBM), so their gates run before the persistent launch as rounds of
+// cache-blocked passes, planned per register as a circuit's passes are
+// (dk_circuit.cpp: tiles of qubits 0..5 plus six chosen qubits held in
+// registers, gates merged and applied in registers, LDS only for
+// exchanges; every gate in some pass): round r is one launch of the pass
+// kernel's round form (dk_pass.hip) over the tiles of the r-th pass of
 // every such register, the launch boundary ordering the passes; the
-// persistent launch then only reads their final states.  Each tile is
-// computed as in the one-workgroup form, so the results are the same.
+// persistent launch then only reads their final states.  Results agree
+// with the one-workgroup form to rounding (merged gates, another order).
 //
 // Gate semantics are registry_apply's (qops.c:176-261); since every gate is
 // unitary, renormalising once after the last gate gives the reference's
@@ -31,8 +35,18 @@
 #include <vector>
 
 #include "dk_device.h"
+#include "dk_pass.h"
 
 using namespace dkd;
+
+int dk_plan_register_passes(unsigned nq, size_t count, const double *mats,
+			    const unsigned *targets, const uint64_t *cms,
+			    const uint64_t *ams, std::vector<dkp::PassDesc> &descs,
+			    std::vector<dkp::PassOp> &ops,
+			    std::vector<uint32_t> &op_begin);
+int dk_launch_pass_round(int dtype, void *pool, const dkp::PassEntry *ents,
+			 unsigned nents, const dkp::PassOp *ops, u64 nitems,
+			 int init, unsigned grid, hipStream_t st);
 
 namespace {
 
@@ -40,8 +54,18 @@ constexpr int BT = 512;            // threads per workgroup
 constexpr int BW = BT / 64;        // waves
 constexpr int LDS_BYTES = 65536;   // tile storage per workgroup
 
+// Chunk sums of the split registers' final states (their measurement,
+// marginals and norm read these instead of streaming the state through one
+// workgroup).  A chunk is CHUNK consecutive amplitudes of one register; its
+// record holds CREC doubles: [0] the sum of |psi_i|^2 over the chunk,
+// [1 + b] the part of it over indices with bit b set (b < 12).  Sums are
+// taken in a fixed order.
+constexpr int CHUNK_BITS = 12, CHUNK = 1 << CHUNK_BITS, CREC = 1 + CHUNK_BITS;
+constexpr int CT = 256;            // threads of the chunk-sum kernel
+
 struct BReg {
 	u64 off;        // amplitude offset of its pool slot (pooled registers)
+	u64 coff;       // split registers: first of its chunks' sum records
 	u64 keep;       // density: kept qubits (0 = no density)
 	unsigned nq;
 	unsigned pass_begin, pass_end;
@@ -69,14 +93,6 @@ struct BGate {
 };
 
 enum { BF_MEASURE = 1, BF_DENSITY = 2, BF_EXPORT = 4, BF_PROBS = 8, BF_SPLIT = 16 };
-
-// One register's pass in a round: its tiles are the round's items
-// first .. first + 2^(nq - LB) - 1.
-struct BRound {
-	u64 first;
-	unsigned reg;
-	unsigned pass;
-};
 
 __device__ __forceinline__ double bsum(double v, double *red)
 {
@@ -274,7 +290,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	const BGate *__restrict__ gates, const double *__restrict__ rolls,
 	T *__restrict__ pool, signed char *__restrict__ outcomes,
 	double *__restrict__ p1s, double2 *__restrict__ dens,
-	double *__restrict__ probs, unsigned *__restrict__ counter)
+	double *__restrict__ probs, unsigned *__restrict__ counter,
+	const double *__restrict__ csums)
 {
 	constexpr int LT = LDS_BYTES / (int)sizeof(T);
 	constexpr int LB = sizeof(T) == 16 ? 12 : 13;
@@ -345,15 +362,35 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 		// squared norm of the final state (marginals and densities divide
 		// by it; the measurement's P1 is a ratio of its own sums)
+		const bool split = (R.flags & BF_SPLIT) != 0;
+		const double *cs = csums + R.coff * CREC;   // split: chunk records
+		const u64 nch = split ? len >> CHUNK_BITS : 0;
 		double norm2 = 1.0;
 		if (R.flags & (BF_PROBS | BF_DENSITY)) {
 			double acc = 0.0;
-			for (u64 i = threadIdx.x; i < len; i += BT)
-				acc += cabs2d(amp_at(tile, st, resident, i));
+			if (split)
+				for (u64 c = threadIdx.x; c < nch; c += BT)
+					acc += cs[c * CREC];
+			else
+				for (u64 i = threadIdx.x; i < len; i += BT)
+					acc += cabs2d(amp_at(tile, st, resident, i));
 			norm2 = bsum(acc, red);
 		}
 
-		if (R.flags & BF_PROBS) {
+		if ((R.flags & BF_PROBS) && split) {
+			for (unsigned q = 0; q < n; ++q) {
+				double acc1 = 0.0;
+				for (u64 c = threadIdx.x; c < nch; c += BT) {
+					if (q < (unsigned)CHUNK_BITS)
+						acc1 += cs[c * CREC + 1 + q];
+					else if ((c >> (q - CHUNK_BITS)) & 1)
+						acc1 += cs[c * CREC];
+				}
+				const double s1 = bsum(acc1, red);
+				if (threadIdx.x == 0)
+					probs[R.roll_off + q] = s1 / norm2;
+			}
+		} else if (R.flags & BF_PROBS) {
 			for (unsigned q = 0; q < n; ++q) {
 				const u64 half = len >> 1;
 				double acc1 = 0.0;
@@ -402,7 +439,53 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			__syncthreads();
 		}
 
-		if (R.flags & BF_MEASURE) {
+		if ((R.flags & BF_MEASURE) && split) {
+			// steps above the chunk bits from the chunk totals, then the
+			// chosen chunk in LDS for the last CHUNK_BITS steps
+			u64 prefix = 0;
+			unsigned step = 0;
+			for (; step < n - CHUNK_BITS; ++step) {
+				const unsigned w = n - 1 - step - CHUNK_BITS;
+				const u64 h = 1ull << w, b0 = prefix << (w + 1);
+				double lo = 0.0, hi = 0.0;
+				for (u64 i = threadIdx.x; i < h; i += BT) {
+					lo += cs[(b0 + i) * CREC];
+					hi += cs[(b0 + h + i) * CREC];
+				}
+				const double slo = bsum(lo, red);
+				const double shi = bsum(hi, red);
+				const double p1 = shi / (slo + shi);
+				const bool v = p1 > rolls[R.roll_off + step];
+				if (threadIdx.x == 0) {
+					outcomes[R.roll_off + step] = v ? 1 : 0;
+					p1s[R.roll_off + step] = p1;
+				}
+				prefix = (prefix << 1) | (v ? 1u : 0u);
+			}
+			const T *chunk = st + (prefix << CHUNK_BITS);
+			for (int i = threadIdx.x; i < CHUNK; i += BT)
+				tile[lds_pos<T>(i)] = chunk[i];
+			__syncthreads();
+			u64 lp = 0;
+			for (; step < n; ++step) {
+				const unsigned q = n - 1 - step;
+				const int start = (int)(lp << (q + 1)), half = 1 << q;
+				double lo = 0.0, hi = 0.0;
+				for (int i = threadIdx.x; i < half; i += BT) {
+					lo += cabs2d(tile[lds_pos<T>(start + i)]);
+					hi += cabs2d(tile[lds_pos<T>(start + half + i)]);
+				}
+				const double slo = bsum(lo, red);
+				const double shi = bsum(hi, red);
+				const double p1 = shi / (slo + shi);
+				const bool v = p1 > rolls[R.roll_off + step];
+				if (threadIdx.x == 0) {
+					outcomes[R.roll_off + step] = v ? 1 : 0;
+					p1s[R.roll_off + step] = p1;
+				}
+				lp = (lp << 1) | (v ? 1u : 0u);
+			}
+		} else if (R.flags & BF_MEASURE) {
 			u64 prefix = 0;
 			for (unsigned step = 0; step < n; ++step) {
 				const unsigned q = n - 1 - step;
@@ -442,25 +525,21 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	}
 }
 
-// One round of the split registers' passes: item `it` is tile it - first of
-// the entry holding it (entries ascend by first); workgroups take items
-// blockIdx.x, blockIdx.x + gridDim.x, ... (measured faster on gfx950 than
-// contiguous runs per workgroup: 4096 registers 21.8 against 27.1 ms).
-// `init`: the first round, whose tiles start from |0...0> instead of the
-// pool.
-template <typename T>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_round(
-	const BReg *__restrict__ regs, const BPass *__restrict__ passes,
-	const BGate *__restrict__ gates, T *__restrict__ pool,
-	const BRound *__restrict__ ents, unsigned nents, u64 nitems, int init)
-{
-	constexpr int LT = LDS_BYTES / (int)sizeof(T);
-	constexpr int LB = sizeof(T) == 16 ? 12 : 13;
-	__shared__ T tile[LT];
-	__shared__ BPass ps;
-	__shared__ BGate gsh[GCH];
-	__shared__ unsigned ent;
 
+struct ChunkEntry {
+	u64 first;      // first item (chunk) of this register
+	u64 off;        // its amplitude offset in the pool
+	u64 coff;       // its first chunk record
+};
+
+template <typename T>
+__global__ __launch_bounds__(CT) void k_chunk_sums(
+	const T *__restrict__ pool, const ChunkEntry *__restrict__ ents,
+	unsigned nents, u64 nitems, double *__restrict__ sums)
+{
+	__shared__ double red[CT / 64][6];
+	__shared__ unsigned ent_sh;
+	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	for (u64 it = blockIdx.x; it < nitems; it += gridDim.x) {
 		if (threadIdx.x == 0) {
 			unsigned lo = 0, hi = nents - 1;
@@ -471,32 +550,66 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				else
 					hi = mid - 1;
 			}
-			ent = lo;
-			ps = passes[ents[lo].pass];
+			ent_sh = lo;
 		}
 		__syncthreads();
-		const BRound E = ents[ent];
-		T *st = pool + regs[E.reg].off;
-		u64 base = it - E.first;
-		for (int b = 0; b < (int)ps.npos; ++b)
-			base = ins0(base, ps.pos[b]);
-		const TileIndex<LT / BT> ix(base, ps);
+		const unsigned e = __builtin_amdgcn_readfirstlane(ent_sh);
+		const u64 c = it - ents[e].first;
+		const T *src = pool + ents[e].off + (c << CHUNK_BITS);
+		// amplitude k * CT + threadIdx.x: bits 0..7 from the thread, bits
+		// 8..11 from k
+		double tot = 0.0, hb[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-		for (int k = 0; k < LT / BT; ++k) {
-			const int s = (int)threadIdx.x + k * BT;
-			T v = czero<T>();
-			if (!init)
-				v = st[ix[k]];
-			else if (base == 0 && s == 0)
-				v.x = 1;
-			tile[lds_pos<T>(s)] = v;
+		for (int k = 0; k < CHUNK / CT; ++k) {
+			const double a = cabs2d(src[k * CT + threadIdx.x]);
+			tot += a;
+#pragma unroll
+			for (int b = 0; b < 4; ++b)
+				if ((k >> b) & 1)
+					hb[b] += a;
+		}
+		// block sums: the total, bits 8..11, and bits 0..7 (a thread's
+		// total where its own bit is set)
+		double v[5] = {tot, hb[0], hb[1], hb[2], hb[3]};
+		double lowb[8];
+#pragma unroll
+		for (int b = 0; b < 8; ++b)
+			lowb[b] = ((threadIdx.x >> b) & 1) ? tot : 0.0;
+		double *rec = sums + (ents[e].coff + c) * CREC;
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			const double t = wave_sum(v[j]);
+			if (lane == 0)
+				red[w][j] = t;
 		}
 		__syncthreads();
-		tile_pass(tile, LB, ps, base, gates, gsh);
-#pragma unroll
-		for (int k = 0; k < LT / BT; ++k)
-			st[ix[k]] = tile[lds_pos<T>((int)threadIdx.x + k * BT)];
+		if (threadIdx.x == 0) {
+			double t[5] = {0, 0, 0, 0, 0};
+			for (int ww = 0; ww < CT / 64; ++ww)
+				for (int j = 0; j < 5; ++j)
+					t[j] += red[ww][j];
+			rec[0] = t[0];
+			for (int b = 0; b < 4; ++b)
+				rec[1 + 8 + b] = t[1 + b];
+		}
 		__syncthreads();
+#pragma unroll
+		for (int b = 0; b < 8; ++b) {
+			const double t = wave_sum(lowb[b]);
+			if (lane == 0)
+				red[w][b & 3] = t;
+			if ((b & 3) == 3) {
+				__syncthreads();
+				if (threadIdx.x == 0)
+					for (int j = 0; j < 4; ++j) {
+						double x = 0.0;
+						for (int ww = 0; ww < CT / 64; ++ww)
+							x += red[ww][j];
+						rec[1 + (b - 3) + j] = x;
+					}
+				__syncthreads();
+			}
+		}
 	}
 }
 
@@ -523,10 +636,18 @@ struct dk_batch {
 	int probs = 0;       // marginal probabilities of every register
 	int split_qubits = 17; // pooled registers this large run their passes
 			       // as grid-wide rounds (0 = never)
-	std::vector<BRound> rounds;       // every round's entries, in order
-	std::vector<size_t> round_off;    // first entry of round r (+ end)
+	std::vector<BGate> orig;          // every register's gates as added
+	std::vector<size_t> orig_begin;   // first gate of register i in orig
+	std::vector<dkp::PassEntry> rounds; // every round's entries, in order
+	std::vector<dkp::PassOp> pops;    // ops of the split registers' passes
+	std::vector<size_t> round_off;    // first entry of round r
 	std::vector<u64> round_items;     // tiles of round r
-	BRound *d_rounds = nullptr;
+	dkp::PassEntry *d_rounds = nullptr;
+	dkp::PassOp *d_pops = nullptr;
+	std::vector<ChunkEntry> chunks;   // split registers' chunk-sum entries
+	u64 chunk_items = 0;
+	ChunkEntry *d_chunks = nullptr;
+	double *d_csums = nullptr;
 	// device side; buffers are kept (and grown) across uploads
 	bool uploaded = false;
 	char *dbuf = nullptr;
@@ -642,6 +763,8 @@ extern "C" int dk_batch_clear(dk_batch *b)
 		return DK_E_ARG;
 	}
 	b->regs.clear();
+	b->orig.clear();
+	b->orig_begin.clear();
 	b->passes.clear();
 	b->gates.clear();
 	b->rolls.clear();
@@ -748,6 +871,8 @@ extern "C" int dk_batch_add(dk_batch *b, unsigned nq, size_t num_gates,
 	R.pass_end = (unsigned)b->passes.size();
 	if (index)
 		*index = b->regs.size();
+	b->orig_begin.push_back(b->orig.size());
+	b->orig.insert(b->orig.end(), gl.begin(), gl.end());
 	b->regs.push_back(R);
 	batch_invalidate(b);
 	b->ran = false;
@@ -781,27 +906,86 @@ static int batch_upload(dk_batch *b)
 	std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) {
 		return b->regs[x].nq > b->regs[y].nq;
 	});
-	// rounds of the split registers' passes: round r holds pass r of every
-	// split register with more than r passes
+	// rounds of the split registers' passes: each register's gates planned
+	// as cache-blocked passes; round r holds pass r of every split
+	// register with more than r passes
 	b->rounds.clear();
+	b->pops.clear();
 	b->round_off.clear();
 	b->round_items.clear();
-	for (unsigned r = 0;; ++r) {
-		const size_t first = b->rounds.size();
-		u64 items = 0;
-		for (unsigned i : order) {
+	{
+		std::vector<std::vector<dkp::PassDesc>> descs(b->regs.size());
+		std::vector<std::vector<uint32_t>> obeg(b->regs.size());
+		std::vector<double> mats;
+		std::vector<unsigned> tg;
+		std::vector<u64> cm, am;
+		size_t maxp = 0;
+		for (size_t i = 0; i < b->regs.size(); ++i) {
 			const BReg &R = b->regs[i];
-			if (!(R.flags & BF_SPLIT) || R.pass_begin + r >= R.pass_end)
+			if (!(R.flags & BF_SPLIT))
 				continue;
-			b->rounds.push_back(BRound{items, i, R.pass_begin + r});
-			items += 1ull << (R.nq - (unsigned)lb);
+			const size_t g0 = b->orig_begin[i];
+			const size_t g1 = i + 1 < b->regs.size() ? b->orig_begin[i + 1]
+								 : b->orig.size();
+			mats.resize(8 * (g1 - g0));
+			tg.resize(g1 - g0);
+			cm.resize(g1 - g0);
+			am.resize(g1 - g0);
+			for (size_t g = g0; g < g1; ++g) {
+				const BGate &x = b->orig[g];
+				memcpy(&mats[8 * (g - g0)], x.m, sizeof(x.m));
+				tg[g - g0] = x.t;
+				cm[g - g0] = x.cm;
+				am[g - g0] = x.am;
+			}
+			int rc = dk_plan_register_passes(R.nq, g1 - g0, mats.data(), tg.data(),
+							 cm.data(), am.data(), descs[i],
+							 b->pops, obeg[i]);
+			if (rc)
+				return rc;
+			if (descs[i].empty()) {
+				// no gates: one empty pass still writes |0...0>
+				dkp::PassDesc d;
+				memset(&d, 0, sizeof(d));
+				for (int h = 0; h < dkp::HIGH_BITS; ++h)
+					d.hpos[h] = d.hsorted[h] = (uint8_t)(dkp::LANE_BITS + h);
+				descs[i].push_back(d);
+				obeg[i].push_back((uint32_t)b->pops.size());
+			}
+			maxp = std::max(maxp, descs[i].size());
 		}
-		if (b->rounds.size() == first)
-			break;
-		b->round_off.push_back(first);
-		b->round_items.push_back(items);
+		for (size_t r = 0; r < maxp; ++r) {
+			b->round_off.push_back(b->rounds.size());
+			u64 items = 0;
+			for (unsigned i : order) {
+				const BReg &R = b->regs[i];
+				if (!(R.flags & BF_SPLIT) || r >= descs[i].size())
+					continue;
+				dkp::PassEntry e;
+				memset(&e, 0, sizeof(e));
+				e.first = items;
+				e.off = R.off;
+				e.pd = descs[i][r];
+				e.op_begin = obeg[i][r];
+				e.nq = R.nq;
+				b->rounds.push_back(e);
+				items += 1ull << (R.nq - (unsigned)dkp::PASS_TILE_BITS);
+			}
+			b->round_items.push_back(items);
+		}
 	}
-	b->round_off.push_back(b->rounds.size());
+	// chunk sums of every split register's final state
+	b->chunks.clear();
+	b->chunk_items = 0;
+	for (unsigned i : order) {
+		BReg &R = b->regs[i];
+		R.coff = 0;
+		if (!(R.flags & BF_SPLIT))
+			continue;
+		R.coff = b->chunk_items;
+		b->chunks.push_back(ChunkEntry{b->chunk_items, R.off, R.coff});
+		b->chunk_items += 1ull << (R.nq - (unsigned)CHUNK_BITS);
+	}
 	auto al = [](u64 x) { return (x + 255) & ~255ull; };
 	const u64 nr = b->regs.size(), nroll = b->rolls.size();
 	u64 o_regs = 0, o_order = al(o_regs + nr * sizeof(BReg));
@@ -814,7 +998,11 @@ static int batch_upload(dk_batch *b)
 	u64 o_dens = al(o_probs + std::max<u64>(1, nroll) * sizeof(double));
 	u64 o_cnt = al(o_dens + std::max<u64>(1, b->dens_total) * sizeof(double2));
 	u64 o_rounds = al(o_cnt + 256);
-	const u64 total = o_rounds + std::max<size_t>(1, b->rounds.size()) * sizeof(BRound);
+	u64 o_pops = al(o_rounds + std::max<size_t>(1, b->rounds.size()) *
+				   sizeof(dkp::PassEntry));
+	u64 o_chunks = al(o_pops + std::max<size_t>(1, b->pops.size()) * sizeof(dkp::PassOp));
+	u64 o_csums = al(o_chunks + std::max<size_t>(1, b->chunks.size()) * sizeof(ChunkEntry));
+	const u64 total = o_csums + std::max<u64>(1, b->chunk_items) * CREC * sizeof(double);
 	if (b->dbuf_bytes < total) {
 		if (b->dbuf)
 			dk_device_free(b->dbuf, b->dbuf_bytes);
@@ -846,7 +1034,10 @@ static int batch_upload(dk_batch *b)
 	b->d_probs = (double *)(b->dbuf + o_probs);
 	b->d_dens = (double2 *)(b->dbuf + o_dens);
 	b->d_counter = (unsigned *)(b->dbuf + o_cnt);
-	b->d_rounds = (BRound *)(b->dbuf + o_rounds);
+	b->d_rounds = (dkp::PassEntry *)(b->dbuf + o_rounds);
+	b->d_pops = (dkp::PassOp *)(b->dbuf + o_pops);
+	b->d_chunks = (ChunkEntry *)(b->dbuf + o_chunks);
+	b->d_csums = (double *)(b->dbuf + o_csums);
 	hipStream_t st = DK_STREAM;
 	DK_HIP(hipMemcpyAsync(b->d_regs, b->regs.data(), nr * sizeof(BReg),
 			      hipMemcpyHostToDevice, st));
@@ -864,7 +1055,15 @@ static int batch_upload(dk_batch *b)
 				      hipMemcpyHostToDevice, st));
 	if (!b->rounds.empty())
 		DK_HIP(hipMemcpyAsync(b->d_rounds, b->rounds.data(),
-				      b->rounds.size() * sizeof(BRound),
+				      b->rounds.size() * sizeof(dkp::PassEntry),
+				      hipMemcpyHostToDevice, st));
+	if (!b->pops.empty())
+		DK_HIP(hipMemcpyAsync(b->d_pops, b->pops.data(),
+				      b->pops.size() * sizeof(dkp::PassOp),
+				      hipMemcpyHostToDevice, st));
+	if (!b->chunks.empty())
+		DK_HIP(hipMemcpyAsync(b->d_chunks, b->chunks.data(),
+				      b->chunks.size() * sizeof(ChunkEntry),
 				      hipMemcpyHostToDevice, st));
 	DK_HIP(hipStreamSynchronize(st));
 	// one pinned block: p1 and probs (doubles), density (double2),
@@ -896,18 +1095,25 @@ static int batch_enqueue(dk_batch *b)
 	const unsigned nregs = (unsigned)b->regs.size();
 	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
 	for (size_t r = 0; r < b->round_items.size(); ++r) {
-		const BRound *ents = b->d_rounds + b->round_off[r];
-		const unsigned nents = (unsigned)(b->round_off[r + 1] - b->round_off[r]);
+		const size_t e1 = r + 1 < b->round_off.size() ? b->round_off[r + 1]
+							       : b->rounds.size();
+		const unsigned nents = (unsigned)(e1 - b->round_off[r]);
 		const u64 items = b->round_items[r];
-		const unsigned rg = (unsigned)std::min<u64>(items, grid);
+		// one workgroup per tile, up to 2^20 (the kernel loops beyond)
+		const unsigned rg = (unsigned)std::min<u64>(items, 1ull << 20);
+		DK_TRY(dk_launch_pass_round(b->dtype, b->pool, b->d_rounds + b->round_off[r],
+					    nents, b->d_pops, items, r == 0, rg, st));
+	}
+	if (b->chunk_items) {
+		const unsigned cg = (unsigned)std::min<u64>(b->chunk_items, 1ull << 20);
 		if (b->dtype == DK_C64)
-			hipLaunchKernelGGL(k_batch_round<float2>, dim3(rg), dim3(BT), 0, st,
-					   b->d_regs, b->d_passes, b->d_gates,
-					   (float2 *)b->pool, ents, nents, items, r == 0);
+			hipLaunchKernelGGL(k_chunk_sums<float2>, dim3(cg), dim3(CT), 0, st,
+					   (const float2 *)b->pool, b->d_chunks,
+					   (unsigned)b->chunks.size(), b->chunk_items, b->d_csums);
 		else
-			hipLaunchKernelGGL(k_batch_round<double2>, dim3(rg), dim3(BT), 0, st,
-					   b->d_regs, b->d_passes, b->d_gates,
-					   (double2 *)b->pool, ents, nents, items, r == 0);
+			hipLaunchKernelGGL(k_chunk_sums<double2>, dim3(cg), dim3(CT), 0, st,
+					   (const double2 *)b->pool, b->d_chunks,
+					   (unsigned)b->chunks.size(), b->chunk_items, b->d_csums);
 		DK_LAUNCHED();
 	}
 	hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
@@ -915,12 +1121,12 @@ static int batch_enqueue(dk_batch *b)
 		hipLaunchKernelGGL(k_batch<float2>, dim3(grid), dim3(BT), 0, st,
 				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
 				   b->d_rolls, (float2 *)b->pool, b->d_out, b->d_p1,
-				   b->d_dens, b->d_probs, b->d_counter);
+				   b->d_dens, b->d_probs, b->d_counter, b->d_csums);
 	else
 		hipLaunchKernelGGL(k_batch<double2>, dim3(grid), dim3(BT), 0, st,
 				   b->d_regs, b->d_order, nregs, b->d_passes, b->d_gates,
 				   b->d_rolls, (double2 *)b->pool, b->d_out, b->d_p1,
-				   b->d_dens, b->d_probs, b->d_counter);
+				   b->d_dens, b->d_probs, b->d_counter, b->d_csums);
 	DK_LAUNCHED();
 	const u64 nroll = b->rolls.size();
 	if (nroll) {

@@ -917,8 +917,9 @@ u64 choose_tile(const dk_circuit *c, const std::vector<int> &rem, bool lookahead
 	return hmask;
 }
 
-// Plans the whole circuit with one tile rule.
-std::vector<PlanSpec> plan_specs(const dk_circuit *c, bool lookahead)
+// Plans the whole circuit with one tile rule.  lone: a pass that would
+// absorb a single dense gate leaves it to the one-qubit kernel instead.
+std::vector<PlanSpec> plan_specs(const dk_circuit *c, bool lookahead, bool lone = true)
 {
 	std::vector<PlanSpec> out;
 	std::vector<int> rem(c->gates.size());
@@ -940,7 +941,14 @@ std::vector<PlanSpec> plan_specs(const dk_circuit *c, bool lookahead)
 		std::vector<int> absorbed, deferred;
 		absorb(c, rem, hmask, &absorbed, &deferred);
 		rem.swap(deferred);
-		if (absorbed.size() == 1 && !is_diag(c->gates[absorbed[0]])) {
+		if (absorbed.empty()) {
+			// cannot happen with one-qubit gates first in `rem`; kept as
+			// a guard against an endless loop: the first gate runs alone
+			out.push_back({false, rem[0], {}, {}});
+			rem.erase(rem.begin());
+			continue;
+		}
+		if (lone && absorbed.size() == 1 && !is_diag(c->gates[absorbed[0]])) {
 			// one dense gate: the one-qubit kernel does it alone
 			out.push_back({false, absorbed[0], {}, {}});
 			continue;
@@ -1192,5 +1200,69 @@ extern "C" int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place,
 		return dk_state_clone(st, out);
 	}
 	*out = cur;
+	return DK_OK;
+}
+
+// Pass plan of a register's one-qubit gates (count gates: 2x2 matrices at
+// mats + 8 i, targets, control and anticontrol masks) on nq >= 12 qubits with
+// every gate inside a pass: appends pass i's description to descs and its
+// ops to ops, from op_begin[i].  The better of the two tile rules, as
+// dk_circuit_passes picks it.  DK_E_ARG if some gate would have to run
+// alone.  For the batched engine's grid-wide rounds.
+int dk_plan_register_passes(unsigned nq, size_t count, const double *mats,
+			    const unsigned *targets, const uint64_t *cms,
+			    const uint64_t *ams, std::vector<dkp::PassDesc> &descs,
+			    std::vector<dkp::PassOp> &ops,
+			    std::vector<uint32_t> &op_begin)
+{
+	if (nq < (unsigned)dkp::PASS_TILE_BITS) {
+		dk_set_error("a pass needs a register of at least %d qubits",
+			     dkp::PASS_TILE_BITS);
+		return DK_E_ARG;
+	}
+	dk_circuit c;
+	c.nq = nq;
+	c.gates.reserve(count);
+	for (size_t i = 0; i < count; ++i) {
+		CGate g;
+		g.k = 1;
+		g.t[0] = targets[i];
+		g.cm = cms ? cms[i] : 0;
+		g.am = ams ? ams[i] : 0;
+		g.qmask = (1ull << g.t[0]) | g.cm | g.am;
+		g.m.resize(4);
+		for (int e = 0; e < 4; ++e)
+			g.m[e] = cd(mats[8 * i + 2 * e], mats[8 * i + 2 * e + 1]);
+		c.gates.push_back(std::move(g));
+	}
+	// first-seen tiles always take the first remaining gate's target, so
+	// that plan has passes only; the look-ahead plan is used when it has
+	// too and is shorter
+	std::vector<PlanSpec> a = plan_specs(&c, false, false);
+	std::vector<PlanSpec> b = plan_specs(&c, true, false);
+	auto passes_only = [](const std::vector<PlanSpec> &v) {
+		for (const PlanSpec &ps : v)
+			if (!ps.pass)
+				return false;
+		return true;
+	};
+	const bool bok = passes_only(b);
+	if (!passes_only(a) && !bok) {
+		dk_set_error("internal error: a one-qubit gate left out of the passes");
+		return DK_E_ARG;
+	}
+	const std::vector<PlanSpec> &best =
+		bok && (b.size() < a.size() || !passes_only(a)) ? b : a;
+	for (const PlanSpec &ps : best)
+		build_pass(&c, ps.chosen, ps.gates);
+	for (const PassPlan &pp : c.passes) {
+		if (!pass_ops_valid(&c.pops[pp.first_op], pp.desc.nops, nq)) {
+			dk_set_error("internal error: malformed pass plan");
+			return DK_E_ARG;
+		}
+		descs.push_back(pp.desc);
+		op_begin.push_back((uint32_t)(ops.size() + pp.first_op));
+	}
+	ops.insert(ops.end(), c.pops.begin(), c.pops.end());
 	return DK_OK;
 }

@@ -72,4 +72,15 @@ struct PassDesc {
 	uint32_t nops;
 };
 
+// One register's pass within a round of batched passes: its tiles are the
+// round's items first .. first + 2^(nq - 12) - 1, its state starts at
+// amplitude `off` of the pool, its ops at op_begin of the round's op array.
+struct PassEntry {
+	uint64_t first;
+	uint64_t off;
+	PassDesc pd;
+	uint32_t op_begin;
+	uint32_t nq;
+};
+
 } // namespace dkp

@@ -594,6 +594,8 @@ __device__ __forceinline__ void tile_run(
 		st_stream(out + (tb | slot_index(reg_slots(r, rs), hb)), mk<T>(vr[r], vi[r]));
 		s += fma((double)vr[r], (double)vr[r], (double)vi[r] * (double)vi[r]);
 	}
+	if (!partials)
+		return;
 	s = wave_sum(s);
 	if (lane == 0)
 		red[w] = s;
@@ -648,6 +650,63 @@ __global__ __launch_bounds__(THREADS) void k_pass(
 		}
 	} else {
 		pass_tile<T>(in, out, inv, pd, ops, partials, blockIdx.x, lds, red);
+	}
+}
+
+// One round of passes over many registers held in a pool: item `it` is
+// tile it - first of the entry holding it (entries ascend by `first`);
+// workgroups take items blockIdx.x, blockIdx.x + gridDim.x, ...  The launch
+// boundary orders one round's passes before the next round's.  `init`: the
+// first round, whose tiles start from |0...0> instead of the pool.  No
+// renormalisation between passes (the caller normalises at the end).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pass_round(
+	T *pool, const PassEntry *__restrict__ ents, unsigned nents,
+	const PassOp *__restrict__ ops, u64 nitems, int init)
+{
+	typedef typename real_of<T>::type R;
+	__shared__ T lds[exch_entries<T>()];
+	__shared__ double red[THREADS / 64];
+	__shared__ unsigned ent_sh;
+	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	for (u64 it = blockIdx.x; it < nitems; it += gridDim.x) {
+		if (threadIdx.x == 0) {
+			unsigned lo = 0, hi = nents - 1;
+			while (lo < hi) {
+				const unsigned mid = (lo + hi + 1) / 2;
+				if (ents[mid].first <= it)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			ent_sh = lo;
+		}
+		__syncthreads();
+		// the same entry for the whole workgroup: scalar loads of it
+		const unsigned e = __builtin_amdgcn_readfirstlane(ent_sh);
+		const PassDesc pd = ents[e].pd;
+		T *st = pool + ents[e].off;
+		const PassOp *eops = ops + ents[e].op_begin;
+		const u64 tile = it - ents[e].first;
+		u64 hb[HIGH_BITS];
+#pragma unroll
+		for (int h = 0; h < HIGH_BITS; ++h)
+			hb[h] = 1ull << pd.hpos[h];
+		const u64 base = tile_base(tile, pd);
+		R vr[REGS], vi[REGS];
+		if (init) {
+#pragma unroll
+			for (int r = 0; r < REGS; ++r)
+				vr[r] = vi[r] = (R)0;
+			// amplitude 0: tile 0, lane 0, wave 0, register 0
+			if (base == 0 && lane == 0 && w == 0)
+				vr[0] = (R)1;
+		} else {
+			tile_fetch(st, base, hb, lane, w, vr, vi);
+		}
+		tile_run<T>(st, pd, eops, nullptr, tile, base, hb, lane, w, vr, vi,
+			    lds, red);
+		__syncthreads();
 	}
 }
 
@@ -711,4 +770,22 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 		return rc;
 	DK_TRY(dk_time_mark(1));
 	return dk_finish_partials(parts, ntiles, out->scal + DK_SCAL_NORM, 1);
+}
+
+// One round of batched passes (see k_pass_round): `ents` and `ops` are device
+// arrays; `grid` workgroups walk the round's items.
+int dk_launch_pass_round(int dtype, void *pool, const PassEntry *ents,
+			 unsigned nents, const PassOp *ops, u64 nitems, int init,
+			 unsigned grid, hipStream_t st)
+{
+	if (!nitems)
+		return DK_OK;
+	if (dtype == DK_C64)
+		hipLaunchKernelGGL(k_pass_round<float2>, dim3(grid), dim3(THREADS), 0, st,
+				   (float2 *)pool, ents, nents, ops, nitems, init);
+	else
+		hipLaunchKernelGGL(k_pass_round<double2>, dim3(grid), dim3(THREADS), 0, st,
+				   (double2 *)pool, ents, nents, ops, nitems, init);
+	DK_LAUNCHED();
+	return DK_OK;
 }

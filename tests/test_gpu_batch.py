@@ -93,11 +93,13 @@ def test_batch_graph_replays_match(capi, oracle):
 
 @pytest.mark.parametrize("dtype,split,pq", [("c128", 13, 0), ("c128", 13, 3), ("c64", 14, 0)])
 def test_batch_split_rounds_equal_single_workgroup(capi, oracle, dtype, split, pq):
-    """Registers at or above split_qubits run their passes as grid-wide
-    rounds (every workgroup takes tiles of one pass of every such register
-    per launch); states, outcomes, P1 and densities equal the one-workgroup
-    form bit for bit, on graph replays too; the mixed batch keeps its small
-    registers in the persistent launch."""
+    """Registers at or above split_qubits run their gates as grid-wide
+    rounds of cache-blocked passes (planned per register, gates merged);
+    states, P1 and densities agree with the one-workgroup form to rounding
+    (1e-12 complex128, 5e-6 complex64), outcomes are identical up to the
+    first roll within 1e-5 of its P1, graph replays repeat each form bit for
+    bit, and the mixed batch keeps its small registers in the persistent
+    launch."""
     dt = capi.C128 if dtype == "c128" else capi.C64
     rng = W.rng_for(9900 + split + pq)
     specs = []
@@ -116,13 +118,21 @@ def test_batch_split_rounds_equal_single_workgroup(capi, oracle, dtype, split, p
             runs.append([(b.state(i), b.outcomes(i), b.density(i) if specs[i][0] == 14 else None)
                          for i in range(len(specs))])
         res[mode] = runs
-    for rep in range(2):
-        for i, (n, _, _) in enumerate(specs):
-            (s0, (o0, p0), d0), (s1, (o1, p1), d1) = res[0][rep][i], res[split][rep][i]
-            assert np.array_equal(s1, s0), (rep, i, n)
-            assert o1 == o0 and np.array_equal(p1, p0), (rep, i, n)
-            if d0 is not None:
-                assert np.array_equal(d1, d0), (rep, i, n)
+    tol = ATOL if dt == capi.C128 else 5e-6
+    for i, (n, _, rolls) in enumerate(specs):
+        for mode in (0, split):
+            (sa, (oa, pa), da), (sb, (ob, pb), db) = res[mode][0][i], res[mode][1][i]
+            assert np.array_equal(sa, sb) and oa == ob and np.array_equal(pa, pb)
+            if da is not None:
+                assert np.array_equal(da, db)
+        (s0, (o0, p0), d0), (s1, (o1, p1), d1) = res[0][0][i], res[split][0][i]
+        assert np.max(np.abs(s1 - s0)) <= tol, (i, n)
+        assert np.max(np.abs(p1 - p0)) <= tol, (i, n)
+        close = [abs(p0[k] - rolls[k]) < 1e-5 for k in range(n)]
+        upto = close.index(True) if any(close) else n
+        assert o1[:upto] == o0[:upto], (i, n)
+        if d0 is not None:
+            assert np.max(np.abs(d1 - d0)) <= tol, (i, n)
     if dt == capi.C128:
         for i in (4, 9):
             n, steps, rolls = specs[i]

@@ -3,15 +3,15 @@ log-uniform on [4, 20], depth U[10, 100], full measurement, 6-qubit densities
 on the 14-qubit members) exactly as it is timed, about 500 of its registers
 of 17-20 qubits running their passes as grid-wide rounds.
 
-* The split run (rounds, the default) and the single-workgroup run
-  (split_qubits = 0) must agree bit for bit on every register: final state,
-  outcomes, the P1 each outcome was decided on, marginals and densities.
+* The split run (rounds of cache-blocked passes, the default) and the
+  single-workgroup run (split_qubits = 0) must agree on every register:
+  final state, the P1 each outcome was decided on, marginals and densities
+  to rounding (1e-12 complex128, 5e-6 complex64), outcomes identical up to
+  the first roll within 1e-5 of its P1.
 * A sample holding every qubit count 4..20, with at least three 20-qubit
   registers, is checked against the oracle running the same gates and rolls
   (states, marginals and densities within 1e-12, outcomes identical).
 """
-import sys
-
 import numpy as np
 import pytest
 
@@ -58,15 +58,22 @@ def test_full_batch_split_equals_single_workgroup(capi, dtype):
     split = run_batch(capi, dt, 17)
     single = run_batch(capi, dt, 0)
     print(f"[batch-full] {dtype}: both runs done", flush=True)
+    tol = 1e-12 if dtype == "c128" else 5e-6
+    cut = 0
     for i, (n, steps, rolls, keep) in enumerate(sp):
-        assert np.array_equal(split.state(i), single.state(i)), (i, n)
+        assert np.max(np.abs(split.state(i) - single.state(i))) <= tol, (i, n)
         o1, p1 = split.outcomes(i)
         o0, p0 = single.outcomes(i)
-        assert o1 == o0 and np.array_equal(p1, p0), (i, n)
-        assert np.array_equal(split.probs(i), single.probs(i)), (i, n)
+        assert np.max(np.abs(p1 - p0)) <= tol, (i, n)
+        close = [abs(p0[k] - rolls[k]) < 1e-5 for k in range(n)]
+        upto = close.index(True) if any(close) else n
+        cut += upto < n
+        assert o1[:upto] == o0[:upto], (i, n)
+        assert np.max(np.abs(split.probs(i) - single.probs(i))) <= tol, (i, n)
         if keep:
-            assert np.array_equal(split.density(i), single.density(i)), (i, n)
-    sys.stdout.flush()
+            assert np.max(np.abs(split.density(i) - single.density(i))) <= tol, (i, n)
+    print(f"[batch-full] {dtype}: {len(sp)} registers agree ({cut} cut at a "
+          "knife-edge roll)", flush=True)
 
 
 def test_full_batch_sample_vs_oracle(capi, oracle):
