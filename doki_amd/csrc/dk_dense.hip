@@ -20,6 +20,7 @@
 // D[out_row(l/16, r)][l%16] (see mfma16 below).
 // Input scaling by 1/norm is applied to the outputs (the product is linear).
 #include <algorithm>
+#include <cstring>
 
 #include "dk_device.h"
 
@@ -207,6 +208,208 @@ __global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Streaming form.  Persistent waves; each block first stages the gate in LDS
+// in fragment order (entry (rt * KS + ks) * 64 + lane holds A[rt * 16 +
+// lane % 16][ks * 4 + lane / 16], so fragment reads are conflict free).  The
+// groups (2^k amplitudes differing in the target bits) are numbered by
+// their other index bits; a column block is 16 consecutive groups.  A wave
+// takes column blocks gw, gw + W, ... (W waves in the grid); it reads its
+// block straight from HBM into registers in the B layout of the 16x16x4
+// MFMA (lane l: group l % 16, gate column ks * 4 + l / 16), reads the next
+// block while the matrix cores work on this one, and stores the D rows of
+// results straight from the accumulators.  No tile barriers; a wave never
+// reads what another writes, so in == out is safe.  One |out|^2 partial per
+// wave.
+// ---------------------------------------------------------------------------
+struct StreamArgs {
+	u64 rowoff[64];            // gate row / column j -> its target index bits
+	unsigned char tsorted[8];  // target bit positions, ascending
+};
+
+constexpr int SW = 4;              // waves per streaming block
+
+// One amplitude as one vector load / store, non-temporal.
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef float fv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ldv(const double2 *p)
+{
+	const dv2 v = __builtin_nontemporal_load((const dv2 *)p);
+	return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ldv(const float2 *p)
+{
+	const fv2 v = __builtin_nontemporal_load((const fv2 *)p);
+	return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stv(double2 *p, double2 v)
+{
+	dv2 x;
+	x.x = v.x;
+	x.y = v.y;
+	__builtin_nontemporal_store(x, (dv2 *)p);
+}
+__device__ __forceinline__ void stv(float2 *p, float2 v)
+{
+	fv2 x;
+	x.x = v.x;
+	x.y = v.y;
+	__builtin_nontemporal_store(x, (fv2 *)p);
+}
+
+template <typename T, int K, int CT>
+__global__ __launch_bounds__(SW * 64) void k_dense_stream(
+	const T *in, T *out, const double *__restrict__ nrm,
+	const T *__restrict__ M, StreamArgs sa, double *__restrict__ partials,
+	u64 ncg)
+{
+	typedef typename real_of<T>::type R;
+	typedef mfma16<R> MF;
+	typedef typename MF::acc acc_t;
+	constexpr int D = 1 << K, KS = D / 4, RT = D / 16;
+	__shared__ T afr[D * D];
+	__shared__ u64 roff[D];
+	const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+	for (int e = threadIdx.x; e < D * D; e += SW * 64) {
+		const int l = e & 63, f = e >> 6;
+		const int rt = f / KS, ks = f % KS;
+		afr[e] = M[(size_t)(rt * 16 + (l & 15)) * D + ks * 4 + (l >> 4)];
+	}
+	for (int e = threadIdx.x; e < D; e += SW * 64)
+		roff[e] = sa.rowoff[e];
+	__syncthreads();
+	// group li + 16 c of a column block (c < CT), deposited around the
+	// targets: the deposit is linear over disjoint bits
+	u64 col[CT];
+#pragma unroll
+	for (int c = 0; c < CT; ++c) {
+		u64 x = (u64)(li + 16 * c);
+#pragma unroll
+		for (int b = 0; b < K; ++b)
+			x = ins0(x, sa.tsorted[b]);
+		col[c] = x;
+	}
+	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
+	const u64 W = (u64)gridDim.x * SW;
+	const u64 gw = (u64)blockIdx.x * SW + (threadIdx.x >> 6);
+	auto block_base = [&](u64 cg) {
+		u64 x = cg * (16 * CT);
+#pragma unroll
+		for (int b = 0; b < K; ++b)
+			x = ins0(x, sa.tsorted[b]);
+		return x;
+	};
+	double nacc = 0.0;
+	T bn[CT][KS];
+	if (gw < ncg) {
+		const u64 nb = block_base(gw);
+#pragma unroll
+		for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+			for (int c = 0; c < CT; ++c)
+				bn[c][ks] = ldv(in + (nb | col[c] | roff[ks * 4 + lk]));
+	}
+	for (u64 cg = gw; cg < ncg; cg += W) {
+		// the LDS tables are read inside the loop: hoisted out of it, the
+		// gate's fragments alone would take 256 registers (k = 6)
+		int fl = lane;
+		asm volatile("" : "+v"(fl));
+		const int flk = fl >> 4;
+		T bc[CT][KS];
+#pragma unroll
+		for (int c = 0; c < CT; ++c)
+#pragma unroll
+			for (int ks = 0; ks < KS; ++ks)
+				bc[c][ks] = bn[c][ks];
+		const u64 base = block_base(cg);
+		if (cg + W < ncg) {
+			const u64 nb = block_base(cg + W);
+#pragma unroll
+			for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+				for (int c = 0; c < CT; ++c)
+					bn[c][ks] = ldv(in + (nb | col[c] | roff[ks * 4 + flk]));
+		}
+		acc_t cr[CT][RT], ci[CT][RT];
+#pragma unroll
+		for (int c = 0; c < CT; ++c)
+#pragma unroll
+			for (int rt = 0; rt < RT; ++rt) {
+				cr[c][rt] = acc_t{0, 0, 0, 0};
+				ci[c][rt] = acc_t{0, 0, 0, 0};
+			}
+		// per k-step, every accumulator's real-part product first and its
+		// imaginary-part product second: an accumulator's two MFMAs are
+		// 2 * RT * CT instructions apart (back to back they stall the
+		// matrix pipe on the dependency); each element still sums its
+		// terms in the same order
+#pragma unroll
+		for (int ks = 0; ks < KS; ++ks) {
+			T a[RT];
+#pragma unroll
+			for (int rt = 0; rt < RT; ++rt)
+				a[rt] = afr[(rt * KS + ks) * 64 + fl];
+#pragma unroll
+			for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+				for (int c = 0; c < CT; ++c) {
+					cr[c][rt] = MF::op(a[rt].x, bc[c][ks].x, cr[c][rt]);
+					ci[c][rt] = MF::op(a[rt].x, bc[c][ks].y, ci[c][rt]);
+				}
+#pragma unroll
+			for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+				for (int c = 0; c < CT; ++c) {
+					cr[c][rt] = MF::op(a[rt].y, -bc[c][ks].y, cr[c][rt]);
+					ci[c][rt] = MF::op(a[rt].y, bc[c][ks].x, ci[c][rt]);
+				}
+		}
+#pragma unroll
+		for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const u64 ro = roff[rt * 16 + MF::out_row(flk, r)];
+#pragma unroll
+				for (int c = 0; c < CT; ++c) {
+					const T o = mk<T>(cr[c][rt][r] * inv, ci[c][rt][r] * inv);
+					stv(out + (base | col[c] | ro), o);
+					nacc += cabs2d(o);
+				}
+			}
+	}
+	nacc = wave_sum(nacc);
+	if (lane == 0)
+		partials[gw] = nacc;
+}
+
+// Column tiles (16 groups each) a wave takes per step: enough bytes in
+// flight for the small gates, few enough registers for the large ones.
+template <typename T, int K> constexpr int stream_ct()
+{
+	return sizeof(T) == 16 ? (K == 4 ? 2 : 1) : (K == 4 ? 4 : K == 5 ? 2 : 1);
+}
+
+// Grid of the streaming form: as many blocks as are resident at once (the
+// blocks are persistent and take equal shares), fewer for small states.
+template <typename T, int K> unsigned stream_grid(u64 ncg)
+{
+	static int bpc = 0;
+	if (!bpc && hipOccupancyMaxActiveBlocksPerMultiprocessor(
+			    &bpc, k_dense_stream<T, K, stream_ct<T, K>()>, SW * 64, 0) !=
+			    hipSuccess)
+		bpc = 1;
+	const u64 want = (u64)std::max(1, g_ctx.num_cus) * (u64)std::max(1, bpc);
+	return (unsigned)std::max<u64>(1, std::min<u64>(want, (ncg + SW - 1) / SW));
+}
+
+template <typename T, int K>
+void launch_stream(const T *src, T *dst, const double *scal, const T *M,
+		   const StreamArgs &sa, double *parts, u64 ncg, unsigned grid)
+{
+	hipLaunchKernelGGL((k_dense_stream<T, K, stream_ct<T, K>()>), dim3(grid),
+			   dim3(SW * 64), 0, DK_STREAM, src, dst, scal, M, sa, parts, ncg);
+}
+
 // Column sets per block for k = 4 (tunable "dense_k4_cw": 1, 2 or 4; 0
 // picks 1, the fastest on gfx950 at 30 qubits averaged over six target
 // positions: complex128 6.43 / 6.62 / 7.36 ms for 1 / 2 / 4 sets, complex64
@@ -274,10 +477,90 @@ unsigned dk_dense_tile_bits(unsigned k, int dtype)
 	return k + 6 + cwb;
 }
 
+// Streaming dense launch: grid, partials (one per wave) and the gate's
+// index tables.
+static int launch_dense_stream(const dk_state *in, dk_state *out, const dk_gate *g,
+			       const unsigned *targets, double **parts, u64 *nparts,
+			       double *scratch, u64 scratch_cap)
+{
+	const unsigned k = g->k, D = 1u << k;
+	StreamArgs sa;
+	memset(&sa, 0, sizeof(sa));
+	for (unsigned j = 0; j < D; ++j) {
+		u64 o = 0;
+		for (unsigned b = 0; b < k; ++b)
+			if ((j >> b) & 1)
+				o |= 1ull << targets[b];
+		sa.rowoff[j] = o;
+	}
+	unsigned ts[8] = {0};
+	for (unsigned b = 0; b < k; ++b)
+		ts[b] = targets[b];
+	std::sort(ts, ts + k);
+	for (unsigned b = 0; b < k; ++b)
+		sa.tsorted[b] = (unsigned char)ts[b];
+	const bool c128 = in->dtype != DK_C64;
+	int ct = 1;
+	if (k == 4)
+		ct = c128 ? stream_ct<double2, 4>() : stream_ct<float2, 4>();
+	else if (k == 5)
+		ct = c128 ? stream_ct<double2, 5>() : stream_ct<float2, 5>();
+	else
+		ct = c128 ? stream_ct<double2, 6>() : stream_ct<float2, 6>();
+	const u64 ncg = (in->len >> (k + 4)) / (u64)ct;  // blocks of 16 ct groups
+	unsigned grid = 1;
+	if (k == 4)
+		grid = c128 ? stream_grid<double2, 4>(ncg) : stream_grid<float2, 4>(ncg);
+	else if (k == 5)
+		grid = c128 ? stream_grid<double2, 5>(ncg) : stream_grid<float2, 5>(ncg);
+	else
+		grid = c128 ? stream_grid<double2, 6>(ncg) : stream_grid<float2, 6>(ncg);
+	const u64 nw = (u64)grid * SW;
+	int rc = scratch ? dk_partials_in(scratch, scratch_cap, nw, parts)
+			 : dk_partials_reserve(nw, parts);
+	if (rc)
+		return rc;
+	*nparts = nw;
+	const double2 *M128 = g->dmat;
+	const float2 *M64 = g->dmat32;
+#define DK_STREAM_K(KK)                                                        \
+	case KK:                                                               \
+		if (c128)                                                      \
+			launch_stream<double2, KK>((const double2 *)in->amp,   \
+						   (double2 *)out->amp, in->scal, M128, \
+						   sa, *parts, ncg, grid);      \
+		else                                                           \
+			launch_stream<float2, KK>((const float2 *)in->amp,     \
+						  (float2 *)out->amp, in->scal, M64, \
+						  sa, *parts, ncg, grid);       \
+		break;
+	switch (k) {
+		DK_STREAM_K(4)
+		DK_STREAM_K(5)
+		DK_STREAM_K(6)
+	default:
+		dk_set_error("dense gates act on 4..6 qubits");
+		return DK_E_ARG;
+	}
+#undef DK_STREAM_K
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
 int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 			  const unsigned *targets, double **parts, u64 *nparts,
 			  double *scratch, u64 scratch_cap)
 {
+	// the streaming form where it measured faster (30 qubits, gfx950):
+	// complex128 k = 5 6.79 against 8.04 ms, k = 6 8.16 against 11.28 ms
+	// (67 TF/s); the tiles stay ahead for complex128 k = 4 (6.23 against
+	// 6.72 ms) and for complex64 (k = 4 / 5 / 6: 3.3 / 4.2 / 6.4 against
+	// 8.0 / 6.9 / 7.2 ms)
+	const bool stream = g_ctx.dense_stream == 2 ||
+			    (g_ctx.dense_stream == 1 && in->dtype == DK_C128 && g->k >= 5);
+	if (stream)
+		return launch_dense_stream(in, out, g, targets, parts, nparts, scratch,
+					   scratch_cap);
 	// one block per tile of 64 * CW groups: targets plus the 6 + log2(CW)
 	// lowest other bits
 	const unsigned tb = dk_dense_tile_bits(g->k, in->dtype);

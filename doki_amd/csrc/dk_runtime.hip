@@ -1145,6 +1145,14 @@ extern "C" int dk_tune(const char *key, int64_t value)
 		g_ctx.apply1_group = (int)value;
 		return DK_OK;
 	}
+	if (!strcmp(key, "dense_stream")) {
+		if (value < 0 || value > 2) {
+			dk_set_error("dense_stream is 0 (tiles), 1 (auto) or 2 (streaming)");
+			return DK_E_ARG;
+		}
+		g_ctx.dense_stream = (int)value;
+		return DK_OK;
+	}
 	if (!strcmp(key, "dense_k4_cw")) {
 		if (value != 0 && value != 1 && value != 2 && value != 4) {
 			dk_set_error("dense_k4_cw is 0 (auto), 1, 2 or 4");

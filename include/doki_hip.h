@@ -402,7 +402,9 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
 /* Tunables: "apply_blocks" (grid cap of grid-stride kernels, 0 = auto),
  * "time_apply" (events 62/63 bracket every gate kernel), "dense_mfma"
  * (dense k >= 4 gates on MFMA, else on the vector ALUs), "apply1_group"
- * (1, 2 or 4 pair groups per block of the 1-qubit gate kernels), "dense_k4_cw"
+ * (1, 2 or 4 pair groups per block of the 1-qubit gate kernels),
+ * "dense_stream" (dense gates on MFMA: 0 LDS tiles, 1 the faster form per
+ * amplitude type and k (default), 2 the streaming form), "dense_k4_cw"
  * (1, 2 or 4 sets of 64 amplitude groups per dense 4-qubit block, 0 =
  * by amplitude type), "arena_max_chunks"
  * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the

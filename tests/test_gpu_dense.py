@@ -22,6 +22,37 @@ def maxabs(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
 
 
+@pytest.mark.parametrize("form", [0, 2])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_dense_forms_vs_oracle(capi, oracle, form, dtype, k):
+    """Both dense forms (0: LDS tiles, 2: streaming from HBM into registers)
+    in both precisions, whatever the default dispatch picks, against the
+    oracle: in place and out of place, targets anywhere (lowest qubits in
+    reverse order, spread out, the highest ones)."""
+    n = 15
+    dt = capi.C128 if dtype == "c128" else capi.C64
+    tol = ATOL if dt == capi.C128 else C64_TOL
+    rng = W.rng_for(9300 + 7 * k + form)
+    psi = W.random_normalised_state(93 + k, n)
+    o = oracle.from_data(n, psi)
+    d = capi.convert(capi.from_host(psi), dt)
+    capi.tune("dense_stream", form)
+    try:
+        for targets in (list(range(k))[::-1],
+                        [int(x) for x in rng.choice(n, size=k, replace=False)],
+                        list(range(n - k, n))):
+            u = W.haar_unitary(rng, k)
+            g = capi.Gate(u)
+            d = capi.apply(d, g, targets)
+            d = capi.apply(d, g, targets, in_place=True)
+            o = oracle.apply(oracle.apply(o, u, targets), u, targets)
+            assert maxabs(d.values(), o.values()) <= tol, targets
+            assert abs(d.norm - 1.0) <= tol
+    finally:
+        capi.tune("dense_stream", 1)
+
+
 @pytest.mark.parametrize("k", [4, 5, 6])
 @pytest.mark.parametrize("n", [12, 13, 16])
 def test_dense_mfma_vs_oracle(capi, oracle, k, n):
