@@ -475,7 +475,10 @@ def serve_leg(args):
     warm = W.serving_requests(42, 40, 1000.0, nmax=20, depth=(20, 40))
     with serve.Server(workers=args.serve_workers) as srv:
         for h in [srv.submit(s) for s in warm]:
-            h._ev.wait()
+            try:
+                h.wait()
+            except serve.Rejected:
+                pass                 # a join too large for the device
     out = {"requests": len(specs), "qps_offered": args.serve_qps,
            "ops": {op: sum(1 for s in specs if s["op"] == op) for op in W.SERVE_OPS},
            "qubits_max": int(max(s["regs"][0][0] for s in specs))}
@@ -513,6 +516,20 @@ def serve_leg(args):
     log(f"[bench] serve burst: {out['served_burst']}")
     _, out["plain_burst"] = plain_run(0.0)
     log(f"[bench] serve plain burst: {out['plain_burst']}")
+    # load curve: the same trace offered at other rates (arrival times
+    # scaled), up to saturation; plain beyond its capacity only queues
+    curve = []
+    for qps in args.serve_curve:
+        scale = args.serve_qps / qps
+        _, sv = served(scale)
+        pt = {"qps_offered": qps, "served": {k: sv[k] for k in (
+            "throughput_rps", "p50_ms", "p99_ms", "completed")}}
+        if qps <= 2 * args.serve_qps:
+            _, pl = plain_run(scale)
+            pt["plain"] = {k: pl[k] for k in ("throughput_rps", "p50_ms", "p99_ms")}
+        curve.append(pt)
+        log(f"[bench] serve curve: {pt}")
+    out["curve"] = curve
     out["speedup_p99"] = round(out["plain"]["p99_ms"] / out["served"]["p99_ms"], 2)
     out["speedup_capacity"] = round(out["served_burst"]["throughput_rps"] /
                                     out["plain_burst"]["throughput_rps"], 2)
@@ -698,6 +715,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=60.0)
     ap.add_argument("--serve-workers", type=int, default=6)
+    ap.add_argument("--serve-curve", type=float, nargs="*",
+                    default=[30.0, 120.0, 240.0, 480.0],
+                    help="further offered rates (requests/s) of the load curve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

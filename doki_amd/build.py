@@ -21,7 +21,8 @@ ARCH = "gfx950"
 EXT = sysconfig.get_config_var("EXT_SUFFIX")
 
 HIP_SOURCES = ["dk_runtime.hip", "dk_kernels.hip", "dk_dense.hip",
-               "dk_batch.hip", "dk_pass.hip", "dk_circuit.cpp", "dk_memory.cpp"]
+               "dk_batch.hip", "dk_pass.hip", "dk_circuit.cpp", "dk_memory.cpp",
+               "dk_serve.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of the device code

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "doki_hip.h"
 
@@ -75,7 +77,13 @@ struct dk_gate {
 	unsigned dim;       // 2^k
 	double2 *dmat;      // device FP64 copy, row-major (k > 1)
 	float2 *dmat32;     // device FP32 copy, row-major (k > 1)
-	mutable dk_lane *lane; // lane that last queued work reading dmat
+	dk_lane *lane;      // lane that uploaded dmat
+	hipEvent_t uploaded = nullptr;  // recorded after the upload (k > 1)
+	// per lane, an event recorded after its last launch reading dmat (the
+	// gate's release waits for them); launches on several lanes may read
+	// the gate concurrently
+	mutable std::mutex uses_mu;
+	mutable std::vector<std::pair<dk_lane *, hipEvent_t>> uses;
 	double2 hmat[64 * 64];
 	float2 hmat32[64 * 64]; // source of the FP32 upload
 };
@@ -178,6 +186,9 @@ static inline unsigned dk_grid_cap(u64 nvb, unsigned threads)
 int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 		    const unsigned *targets, u64 cm, u64 acm,
 		    double *scratch = nullptr, u64 scratch_cap = 0);
+
+// Records that the calling lane's queued work reads gate g's device copy.
+int dk_gate_mark_use(const dk_gate *g);
 
 // Dense k-qubit gate (4 <= k <= 6, no controls) on MFMA, for states of at
 // least dk_dense_tile_bits(k) qubits (the index bits one block covers).

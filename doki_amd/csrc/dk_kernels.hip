@@ -1184,12 +1184,10 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	double *parts = nullptr;
 	u64 nblocks = 0;
 	int rc;
-	if (g->lane != dk_cur()) {
-		// the gate's upload (or last use) was queued on another lane
-		if (g->dmat)
-			DK_HIP(hipStreamSynchronize(g->lane->stream));
-		g->lane = dk_cur();
-	}
+	// a gate uploaded on another lane: this lane's stream waits for the
+	// upload on the device (the host does not block)
+	if (g->dmat && g->lane != dk_cur())
+		DK_HIP(hipStreamWaitEvent(DK_STREAM, g->uploaded, 0));
 	DK_TRY(dk_time_mark(0));
 	if (g_ctx.dense_mfma && g->k >= 4 && cm == 0 && acm == 0 &&
 	    in->nq >= dk_dense_tile_bits(g->k, in->dtype)) {
@@ -1211,7 +1209,27 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	if (rc)
 		return rc;
 	DK_TRY(dk_time_mark(1));
+	if (g->dmat)
+		DK_TRY(dk_gate_mark_use(g));
 	return dk_finish_partials(parts, nblocks, out->scal + DK_SCAL_NORM, 1);
+}
+
+// Records, on the calling lane, that its queued work has read gate g (the
+// gate's release waits for the last such record of every lane).
+int dk_gate_mark_use(const dk_gate *g)
+{
+	dk_lane *l = dk_cur();
+	std::lock_guard<std::mutex> lk(g->uses_mu);
+	hipEvent_t ev = nullptr;
+	for (auto &u : g->uses)
+		if (u.first == l)
+			ev = u.second;
+	if (!ev) {
+		DK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		g->uses.emplace_back(l, ev);
+	}
+	DK_HIP(hipEventRecord(ev, l->stream));
+	return DK_OK;
 }
 
 // Deterministic sum of `count` device partials into *dst (sqrt when asked):

@@ -532,6 +532,15 @@ extern "C" int dk_gate_new(unsigned num_qubits, const double *matrix,
 		}
 		g->dmat = (double2 *)buf;
 		g->dmat32 = (float2 *)(buf + b64);
+		if (hipEventCreateWithFlags(&g->uploaded, hipEventDisableTiming) !=
+			    hipSuccess ||
+		    hipEventRecord(g->uploaded, st) != hipSuccess) {
+			dk_set_error("gate upload event failed");
+			(void)hipStreamSynchronize(st);
+			dk_device_free(buf, b64 + b32);
+			delete g;
+			return DK_E_DEVICE;
+		}
 	}
 	*out = g;
 	return DK_OK;
@@ -542,11 +551,19 @@ extern "C" void dk_gate_free(dk_gate *g)
 	if (!g)
 		return;
 	if (g->dmat && !g_closed) {
-		// queued uploads and kernels reading the gate finish first; the
-		// release is then ordered on the calling lane
-		(void)hipStreamSynchronize(g->lane->stream);
+		// the upload and every lane's last launch reading the gate finish
+		// first; the release is then ordered on the calling lane
+		(void)hipEventSynchronize(g->uploaded);
+		for (auto &u : g->uses)
+			(void)hipEventSynchronize(u.second);
 		const size_t n = (size_t)g->dim * g->dim;
 		dk_device_free(g->dmat, n * (sizeof(double2) + sizeof(float2)));
+	}
+	if (!g_closed) {
+		if (g->uploaded)
+			(void)hipEventDestroy(g->uploaded);
+		for (auto &u : g->uses)
+			(void)hipEventDestroy(u.second);
 	}
 	delete g;
 }

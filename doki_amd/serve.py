@@ -1,40 +1,38 @@
-"""Request serving over the state-vector core on one GPU.
+"""Request serving over the state-vector core on one GPU: a Python client of
+the library's native server (dk_server_*, dk_submit_circuit, dk_poll in
+include/doki_hip.h).
 
-Requests (dicts as made by workloads.serving_requests: op, regs, rolls, keep)
-are admitted by the device memory their states need, against a byte budget:
+The server itself is C++ in libdoki_hip.so: it admits requests by the device
+memory their states need, runs small registers together in one batched
+launch per round (continuous batching: arrivals during a launch join the
+next one) and large registers and joins on worker threads with execution
+lanes of their own, as cache-blocked passes in place.  Session registers
+stay resident on the device between requests, so later requests can apply
+more gates to them, read them or join two of them.  No Python runs between
+a request's submission and its result, and ctypes releases the GIL while
+the client waits.
 
-* registers of at most ``small_qubits`` qubits whose operation the batched
-  engine covers (marginal probabilities, full measurement, reduced density of
-  at most 6 qubits) go to a batcher thread.  It runs everything queued since
-  its previous launch as one persistent dk_batch launch, so arrivals during a
-  launch join the next one (continuous batching);
-* larger registers and joins go to worker threads, each bound to its own
-  execution lane (a HIP stream with its own scratch), so that several
-  requests run on the device at once.  Their circuits run in place as
-  cache-blocked passes (or fused dense blocks, as the tuned plan says), and
-  their states stay resident on the device from creation to the final
-  operation.  A join and the marginals read from it are made in one pass
-  (the joined state is written once and not read back).
-
-ctypes calls release the GIL, so host threads overlap while the device works.
-Every request's result depends only on the request: the batched engine runs
-each register in one workgroup in a fixed order, and the lane path launches
-the same kernels with the same grids whatever else is running.
+Requests are dicts as workloads.serving_requests makes them (rid, arrival,
+op, regs [(nq, steps)], rolls, keep), with optional "packed" gate lists per
+register (capi.pack_steps), "keep_result" (the request's register -- the
+joined one for a join -- becomes a session; op "keep" only keeps it) and
+registers given as ("session", id, steps) to use a live session register.
 
 ``run_plain`` is the baseline: one request at a time in arrival order, each
 gate applied out of place as registry_apply does (doki.c:722-941), on the
 library's default lane.
 """
-import collections
+import ctypes as C
 import threading
 import time
 
 import numpy as np
 
 from . import capi
-from . import tune
 
 SMALL_QUBITS = 14
+OPS = {"prob": 0, "measure": 1, "density": 2, "join": 3, "keep": 4}
+E_BUDGET, E_PENDING = 9, 10
 
 
 def state_bytes(nq, dtype=capi.C128):
@@ -42,78 +40,261 @@ def state_bytes(nq, dtype=capi.C128):
 
 
 def request_need(spec, dtype=capi.C128):
-    """Peak device bytes of a request's states: the register, plus the
-    collapse chain of a measurement (at most one more register's worth), or
-    both registers and their join."""
-    n0 = spec["regs"][0][0]
+    """Peak device bytes of a request's new states: the register, plus the
+    collapse chain of a measurement, or both registers and their join."""
+    n = [r[0] for r in spec["regs"]]
     if spec["op"] == "join":
-        n1 = spec["regs"][1][0]
-        return (state_bytes(n0, dtype) + state_bytes(n1, dtype) +
-                state_bytes(n0 + n1, dtype))
+        return (state_bytes(n[0], dtype) + state_bytes(n[1], dtype) +
+                state_bytes(n[0] + n[1], dtype))
     if spec["op"] == "measure":
-        return 2 * state_bytes(n0, dtype)
-    return state_bytes(n0, dtype)
+        return 2 * state_bytes(n[0], dtype)
+    return state_bytes(n[0], dtype)
 
 
 class Rejected(Exception):
     pass
 
 
-class Handle:
-    """One submitted request: wait() returns its result dict."""
+class _RegSpec(C.Structure):
+    _fields_ = [("num_qubits", C.c_uint), ("session", C.c_uint64),
+                ("num_gates", C.c_size_t), ("matrices", C.POINTER(C.c_double)),
+                ("targets", C.POINTER(C.c_uint)),
+                ("ctrl_masks", C.POINTER(C.c_uint64)),
+                ("actrl_masks", C.POINTER(C.c_uint64))]
 
-    def __init__(self, spec, dtype):
+
+class _Request(C.Structure):
+    _fields_ = [("op", C.c_int), ("num_regs", C.c_uint), ("regs", _RegSpec * 2),
+                ("rolls", C.POINTER(C.c_double)), ("keep_mask", C.c_uint64),
+                ("keep_result", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("probs", C.POINTER(C.c_double)), ("outcomes", C.POINTER(C.c_int8)),
+                ("p1", C.POINTER(C.c_double)), ("rho", C.POINTER(C.c_double)),
+                ("session", C.c_uint64), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double),
+                ("error", C.c_char * 256)]
+
+
+def _bind():
+    L = capi.lib()
+    if getattr(L, "_serve_bound", False):
+        return L
+    P, u64, PP = C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)
+    for name, res, args in (
+            ("dk_server_new", C.c_int, [C.c_int, C.c_uint, C.c_int, u64, PP]),
+            ("dk_server_set", C.c_int, [P, C.c_char_p, C.c_int64]),
+            ("dk_submit_circuit", C.c_int, [P, C.POINTER(_Request), C.POINTER(u64)]),
+            ("dk_poll", C.c_int, [P, u64, C.c_int, C.POINTER(_Result)]),
+            ("dk_session_free", C.c_int, [P, u64]),
+            ("dk_server_stats", C.c_int, [P] + [C.POINTER(u64)] * 4),
+            ("dk_server_free", None, [P])):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L._serve_bound = True
+    return L
+
+
+class Handle:
+    """One submitted request: wait() returns its result dict (probs, or
+    outcomes and p1, or rho; "session" when a register was kept)."""
+
+    def __init__(self, server, spec, ticket, sizes):
+        self.server = server
         self.spec = spec
-        self.need = request_need(spec, dtype)
-        # a client may send its gates packed already (capi.pack_steps)
-        self.packed = spec.get("packed") or [capi.pack_steps(steps)
-                                             for _, steps in spec["regs"]]
+        self.ticket = ticket
+        self.sizes = sizes
         self.result = None
         self.error = None
-        self.t_submit = None
-        self.t_start = None   # execution began (batch launch or worker)
-        self.t_done = None
-        self._ev = threading.Event()
+        self.t_submit = self.t_start = self.t_done = None
+        self._done = False
+        self._lock = threading.Lock()
+
+    def _fetch(self, timeout):
+        with self._lock:
+            if self._done:
+                return True
+            op, n, a = self.sizes
+            probs = np.empty(n) if op in ("prob", "join") else None
+            outs = np.zeros(n, dtype=np.int8) if op == "measure" else None
+            p1 = np.empty(n) if op == "measure" else None
+            rho = np.empty(2 << (2 * a)) if op == "density" else None
+            res = _Result()
+            if probs is not None:
+                res.probs = probs.ctypes.data_as(C.POINTER(C.c_double))
+            if outs is not None:
+                res.outcomes = outs.ctypes.data_as(C.POINTER(C.c_int8))
+                res.p1 = p1.ctypes.data_as(C.POINTER(C.c_double))
+            if rho is not None:
+                res.rho = rho.ctypes.data_as(C.POINTER(C.c_double))
+            ms = -1 if timeout is None else max(0, int(timeout * 1000))
+            rc = _bind().dk_poll(self.server.ptr, self.ticket, ms, C.byref(res))
+            if rc == E_PENDING:
+                return False
+            self.t_submit, self.t_start, self.t_done = res.t_submit, res.t_start, res.t_done
+            if rc == E_BUDGET:
+                self.error = Rejected("request %s: %s" % (self.spec.get("rid"),
+                                                          res.error.decode()))
+            elif rc:
+                self.error = capi.DokiHipError(rc, res.error.decode())
+            else:
+                r = {}
+                if probs is not None:
+                    r["probs"] = probs
+                if outs is not None:
+                    r["outcomes"] = [bool(x) for x in outs]
+                    r["p1"] = p1
+                if rho is not None:
+                    r["rho"] = rho.view(np.complex128).reshape(1 << a, 1 << a)
+                if res.session:
+                    r["session"] = int(res.session)
+                self.result = r
+            self._done = True
+            return True
 
     def wait(self, timeout=None):
-        if not self._ev.wait(timeout):
-            raise TimeoutError("request %d still running" % self.spec["rid"])
+        if not self._fetch(timeout):
+            raise TimeoutError("request %s still running" % self.spec.get("rid"))
         if self.error is not None:
             raise self.error
         return self.result
 
     @property
     def done(self):
-        return self._ev.is_set()
+        return self._fetch(0)
 
 
-def _finish(h, result=None, error=None):
-    h.result = result
-    h.error = error
-    h.t_done = time.perf_counter()
-    h._ev.set()
+class Server:
+    """The native continuous-batching server on the library's device.
+    plan: "auto"/"passes" (cache-blocked passes from 12 qubits), 0 (gate by
+    gate) or a dense fusion width 3..6."""
+
+    def __init__(self, workers=6, small_qubits=SMALL_QUBITS, plan="auto",
+                 budget=None, max_batch=2048, dtype=capi.C128):
+        L = _bind()
+        capi.check(L.dk_init(0))
+        self.dtype = dtype
+        p = C.c_void_p()
+        capi.check(L.dk_server_new(workers, small_qubits, dtype, budget or 0,
+                                   C.byref(p)))
+        self.ptr = p
+        planv = -1 if plan in ("auto", "passes") else int(plan)
+        capi.check(L.dk_server_set(self.ptr, b"plan", planv))
+        capi.check(L.dk_server_set(self.ptr, b"max_batch", int(max_batch)))
+        self._handles = []
+        self._session_nq = {}
+
+    def _reg(self, entry, packed, keep):
+        if entry[0] == "session":
+            sid, steps = entry[1], entry[2]
+            pk = packed if packed is not None else capi.pack_steps(steps)
+            nq = self._session_nq[sid]
+        else:
+            nq, steps = entry
+            sid = 0
+            pk = packed if packed is not None else capi.pack_steps(steps)
+        mf, tg, cm, am, ng = capi.checked_packed(pk)
+        keep.append((mf, tg, cm, am))
+        r = _RegSpec()
+        r.num_qubits = nq
+        r.session = sid
+        r.num_gates = ng
+        r.matrices = mf.ctypes.data_as(C.POINTER(C.c_double))
+        r.targets = tg.ctypes.data_as(C.POINTER(C.c_uint))
+        r.ctrl_masks = cm.ctypes.data_as(C.POINTER(C.c_uint64))
+        r.actrl_masks = am.ctypes.data_as(C.POINTER(C.c_uint64))
+        return r, nq
+
+    def submit(self, spec):
+        op = spec["op"]
+        req = _Request()
+        req.op = OPS[op]
+        regs = spec["regs"]
+        req.num_regs = len(regs)
+        packed = spec.get("packed") or [None] * len(regs)
+        keep = []
+        nqs = []
+        for i, entry in enumerate(regs):
+            r, nq = self._reg(entry, packed[i], keep)
+            req.regs[i] = r
+            nqs.append(nq)
+        rolls = None
+        if op == "measure":
+            rolls = np.ascontiguousarray(np.asarray(spec["rolls"], dtype=np.float64))
+            req.rolls = rolls.ctypes.data_as(C.POINTER(C.c_double))
+        req.keep_mask = capi.mask(spec.get("keep") or ()) if op == "density" else 0
+        req.keep_result = 1 if spec.get("keep_result") or op == "keep" else 0
+        ticket = C.c_uint64()
+        capi.check(_bind().dk_submit_circuit(self.ptr, C.byref(req), C.byref(ticket)))
+        n = sum(nqs) if op == "join" else nqs[0]
+        h = Handle(self, spec, ticket.value, (op, n, len(spec.get("keep") or ())))
+        self._handles.append(h)
+        if req.keep_result:
+            # a kept register's size, for later requests naming its session
+            h._kept_nq = n
+        return h
+
+    def session_of(self, handle):
+        """The session id a keep_result request made (waits for it)."""
+        r = handle.wait()
+        sid = r["session"]
+        self._session_nq[sid] = handle._kept_nq
+        return sid
+
+    def free_session(self, sid):
+        capi.check(_bind().dk_session_free(self.ptr, sid))
+        self._session_nq.pop(sid, None)
+
+    def _stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        capi.check(_bind().dk_server_stats(self.ptr, *[C.byref(x) for x in v]))
+        return [x.value for x in v]
+
+    @property
+    def batches(self):
+        return self._stats()[0]
+
+    @property
+    def batched(self):
+        return self._stats()[1]
+
+    def close(self):
+        if getattr(self, "ptr", None) is None:
+            return
+        for h in self._handles:
+            h._fetch(None)
+        _bind().dk_server_free(self.ptr)
+        self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
-def build_register(nq, packed, dtype, plan="auto"):
-    """|0...0> of nq qubits with the packed circuit applied in place under
-    `plan`: "passes" (cache-blocked passes), a fusion width k (dense blocks
-    of up to k qubits, when the register is wide enough for them), 0 (gate
-    by gate), or "auto" for the tuned plan of (nq, dtype)."""
-    if plan == "auto":
-        plan = tune.best_plan(nq, dtype)
-    st = capi.new_typed(nq, dtype)
-    circ = capi.Circuit(nq)
-    circ.add_packed(packed)
-    if plan == "passes" or (plan and nq >= plan + 6):
-        tune.prepare(circ, plan)
-    circ.run(st, in_place=True)
-    return st
+def serve_trace(server, specs, time_scale=1.0):
+    """Submits the requests at their arrival times (scaled by time_scale) and
+    waits for all of them.  Returns (handles, t0); handle.t_done is on the
+    clock of time.perf_counter."""
+    t0 = time.perf_counter()
+    hs = []
+    for s in specs:
+        due = t0 + s["arrival"] * time_scale
+        now = time.perf_counter()
+        if due > now:
+            time.sleep(due - now)
+        hs.append(server.submit(s))
+    for h in hs:
+        h._fetch(None)
+    return hs, t0
 
 
-def run_operation(spec, states, dtype, one_pass_join=False):
-    """The request's final operation on its resident register(s).  With
-    one_pass_join a join and its marginals are made in one pass over the
-    joined state (dk_join_prob_all), else as a join and then prob_all."""
+def run_operation(spec, states):
+    """The request's final operation on its register(s), through the C ABI
+    one call at a time (the plain path)."""
     op = spec["op"]
     st = states[0]
     if op == "prob":
@@ -125,200 +306,17 @@ def run_operation(spec, states, dtype, one_pass_join=False):
     if op == "density":
         return {"rho": capi.reduced_density(st, spec["keep"])}
     if op == "join":
-        if one_pass_join:
-            _, probs = capi.join_prob_all(states[0], states[1])
-            return {"probs": probs}
         j = capi.join(states[0], states[1])
         return {"probs": capi.prob_all(j)}
     raise ValueError("unknown operation %r" % op)
-
-
-class Server:
-    """Continuous-batching request server on the library's device."""
-
-    def __init__(self, workers=6, small_qubits=SMALL_QUBITS, plan="auto",
-                 budget=None, max_batch=2048, dtype=capi.C128):
-        self.dtype = dtype
-        self.small_qubits = small_qubits
-        self.plan = plan
-        self.max_batch = max_batch
-        if budget is None:
-            b = capi.C.c_uint64()
-            capi.check(capi.lib().dk_memory_budget(capi.C.byref(b), None))
-            budget = int(b.value)
-        self.budget = budget
-        self.used = 0
-        self._lock = threading.Lock()
-        self._pending = collections.deque()
-        self._small = collections.deque()
-        self._small_cv = threading.Condition(self._lock)
-        self._large = collections.deque()
-        self._large_cv = threading.Condition(self._lock)
-        self._stop = False
-        self.batches = 0          # launches of the batched engine
-        self.batched = 0          # requests run in them
-        self._threads = [threading.Thread(target=self._batcher, daemon=True)]
-        self._threads += [threading.Thread(target=self._worker, daemon=True)
-                          for _ in range(workers)]
-        for t in self._threads:
-            t.start()
-
-    # ---- admission -----------------------------------------------------------
-
-    def is_small(self, spec):
-        return (spec["op"] in ("prob", "measure", "density")
-                and spec["regs"][0][0] <= self.small_qubits
-                and len(spec["keep"]) <= 6)
-
-    def submit(self, spec):
-        h = Handle(spec, self.dtype)
-        h.t_submit = time.perf_counter()
-        with self._lock:
-            if h.need > self.budget:
-                _finish(h, error=Rejected(
-                    "request %d needs %d bytes of states; the budget is %d"
-                    % (spec["rid"], h.need, self.budget)))
-                return h
-            self._pending.append(h)
-            self._admit_locked()
-        return h
-
-    def _admit_locked(self):
-        # FIFO: a request waits while the one ahead of it does not fit;
-        # batched requests (a few MiB at most) pass a waiting one when they fit
-        blocked = False
-        keep = collections.deque()
-        while self._pending:
-            h = self._pending.popleft()
-            fits = self.used + h.need <= self.budget
-            small = self.is_small(h.spec)
-            if fits and (not blocked or small):
-                self.used += h.need
-                if small:
-                    self._small.append(h)
-                    self._small_cv.notify()
-                else:
-                    self._large.append(h)
-                    self._large_cv.notify()
-            else:
-                blocked = blocked or not small
-                keep.append(h)
-        self._pending = keep
-
-    def _release(self, need):
-        with self._lock:
-            self.used -= need
-            self._admit_locked()
-
-    # ---- execution -------------------------------------------------------------
-
-    def _batcher(self):
-        lane = capi.Lane()
-        lane.bind()
-        b = capi.Batch(self.dtype, probs=1)
-        while True:
-            with self._lock:
-                while not self._small and not self._stop:
-                    self._small_cv.wait()
-                if self._stop and not self._small:
-                    return
-                items = []
-                while self._small and len(items) < self.max_batch:
-                    items.append(self._small.popleft())
-            t_start = time.perf_counter()
-            for h in items:
-                h.t_start = t_start
-            try:
-                b.clear()
-                for h in items:
-                    s = h.spec
-                    n = s["regs"][0][0]
-                    b.add_packed(n, h.packed[0],
-                                 s["rolls"] if s["op"] == "measure" else None,
-                                 capi.mask(s["keep"]) if s["op"] == "density" else 0)
-                b.run()
-                self.batches += 1
-                self.batched += len(items)
-                for i, h in enumerate(items):
-                    op = h.spec["op"]
-                    if op == "prob":
-                        res = {"probs": b.probs(i)}
-                    elif op == "measure":
-                        outs, p1 = b.outcomes(i)
-                        res = {"outcomes": outs, "p1": p1}
-                    else:
-                        res = {"rho": b.density(i)}
-                    self._release(h.need)
-                    _finish(h, res)
-            except Exception as e:  # noqa: BLE001 - handed to the waiters
-                for h in items:
-                    if not h.done:
-                        self._release(h.need)
-                        _finish(h, error=e)
-
-    def _worker(self):
-        lane = capi.Lane()
-        lane.bind()
-        while True:
-            with self._lock:
-                while not self._large and not self._stop:
-                    self._large_cv.wait()
-                if self._stop and not self._large:
-                    return
-                h = self._large.popleft()
-            h.t_start = time.perf_counter()
-            try:
-                res = self._run_large(h)
-                err = None
-            except Exception as e:  # noqa: BLE001 - handed to the waiter
-                res, err = None, e
-            self._release(h.need)
-            _finish(h, res, err)
-
-    def _run_large(self, h):
-        states = [build_register(n, pk, self.dtype, self.plan)
-                  for (n, _), pk in zip(h.spec["regs"], h.packed)]
-        res = run_operation(h.spec, states, self.dtype, one_pass_join=True)
-        del states  # freed on this thread's lane
-        return res
-
-    def close(self):
-        with self._lock:
-            self._stop = True
-            self._small_cv.notify_all()
-            self._large_cv.notify_all()
-        for t in self._threads:
-            t.join()
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
-
-
-def serve_trace(server, specs, time_scale=1.0):
-    """Submits the requests at their arrival times (scaled by time_scale) and
-    waits for all of them.  Returns (handles, t0)."""
-    t0 = time.perf_counter()
-    hs = []
-    for s in specs:
-        due = t0 + s["arrival"] * time_scale
-        now = time.perf_counter()
-        if due > now:
-            time.sleep(due - now)
-        hs.append(server.submit(s))
-    for h in hs:
-        h._ev.wait()
-    return hs, t0
 
 
 def run_plain(specs, dtype=capi.C128, time_scale=1.0):
     """Plain per-request execution: requests one at a time in arrival order,
     gate by gate out of place, on the default lane.  Returns a list of
     (result or exception, latency seconds) and the makespan."""
-    budget = capi.C.c_uint64()
-    capi.check(capi.lib().dk_memory_budget(capi.C.byref(budget), None))
+    budget = C.c_uint64()
+    capi.check(capi.lib().dk_memory_budget(C.byref(budget), None))
     t0 = time.perf_counter()
     out = []
     for s in specs:
@@ -335,7 +333,7 @@ def run_plain(specs, dtype=capi.C128, time_scale=1.0):
                 for m, t, c, a in steps:
                     st = capi.apply(st, capi.Gate(m), t, c, a)
                 states.append(st)
-            res = run_operation(s, states, dtype)
+            res = run_operation(s, states)
             del states, st
         except Exception as e:  # noqa: BLE001 - reported per request
             res = e

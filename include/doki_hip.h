@@ -383,6 +383,94 @@ int dk_batch_density(const dk_batch *b, size_t i, double *rho);
  * "export_states"). */
 int dk_batch_state(const dk_batch *b, size_t i, double *values);
 
+/* ---- serving -----------------------------------------------------------------
+ * Continuous batching of circuit requests on the library's device.  A
+ * request builds one register (two for a join) -- from |0...0> through a
+ * list of one-qubit gates, or from a session register that stays resident
+ * on the device between requests, with more gates applied to it in place --
+ * and ends in one read-out.  Requests are admitted by the device memory
+ * their states need, first come first served (a small request passes a
+ * large one that does not fit yet); small ones (one new register of at most
+ * small_qubits qubits read by DK_OP_PROB, _MEASURE or _DENSITY) run together
+ * in one batched launch per round, the others on worker threads with
+ * execution lanes of their own, as cache-blocked passes.  Results equal
+ * those of the same request run alone.  No single reference counterpart:
+ * the reference's callers issue registry_new / registry_apply /
+ * registry_prob / registry_measure / registry_density / registry_join
+ * (doki.c:302-1228) one call at a time. */
+typedef struct dk_server dk_server;
+
+#define DK_E_PENDING 10     /* dk_poll: the request has not finished yet */
+
+#define DK_OP_PROB 0        /* P(qubit q = 1) of every qubit -> probs[n] */
+#define DK_OP_MEASURE 1     /* measurement of every qubit, MSB first, with
+			       rolls[n] -> outcomes[n] (registry_measure's
+			       list order), p1[n] */
+#define DK_OP_DENSITY 2     /* reduced density over keep_mask -> rho */
+#define DK_OP_JOIN 3        /* join(regs[0], regs[1]), regs[0] the high
+			       qubits, and the marginals of the result ->
+			       probs[n0 + n1] */
+#define DK_OP_KEEP 4        /* no read-out: the register becomes a session */
+
+/* A register of a request: `session` nonzero names a live session register
+ * (num_qubits is then ignored), else a new |0...0> of num_qubits qubits.
+ * The gates (2x2, as dk_circuit_add_many takes them) are applied to it in
+ * order; for a session register they change the session. */
+typedef struct {
+	unsigned num_qubits;
+	uint64_t session;
+	size_t num_gates;
+	const double *matrices;
+	const unsigned *targets;
+	const uint64_t *ctrl_masks;   /* may be NULL */
+	const uint64_t *actrl_masks;  /* may be NULL */
+} dk_reg_spec;
+
+typedef struct {
+	int op;                  /* DK_OP_* */
+	unsigned num_regs;       /* 2 for DK_OP_JOIN, else 1 */
+	dk_reg_spec regs[2];
+	const double *rolls;     /* DK_OP_MEASURE */
+	uint64_t keep_mask;      /* DK_OP_DENSITY: kept qubits (at most 12) */
+	int keep_result;         /* nonzero: the request's register (the joined
+				    one for a join) stays resident as a
+				    session; DK_OP_KEEP implies it */
+} dk_request;
+
+typedef struct {
+	double *probs;           /* each may be NULL; sizes as the op says */
+	int8_t *outcomes;
+	double *p1;
+	double *rho;
+	uint64_t session;        /* the kept session register, or 0 */
+	double t_submit, t_start, t_done; /* CLOCK_MONOTONIC seconds */
+	char error[256];         /* message of a failed request */
+} dk_result;
+
+/* A server with `workers` worker threads (1..64) and one batcher; budget 0
+ * takes the library's free memory budget. */
+int dk_server_new(int workers, unsigned small_qubits, int dtype,
+		  uint64_t budget, dk_server **out);
+/* Settings: "plan" (-1 cache-blocked passes from 12 qubits on, the default;
+ * 0 gate by gate; 3..6 dense fusion width), "max_batch" (requests per
+ * batched launch, 2048 by default). */
+int dk_server_set(dk_server *s, const char *key, int64_t value);
+/* Copies the request (the caller's arrays may go at once) and queues it;
+ * *ticket names it for dk_poll.  Requests whose states exceed the budget
+ * complete at once with DK_E_BUDGET. */
+int dk_submit_circuit(dk_server *s, const dk_request *r, uint64_t *ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the request: DK_E_PENDING if it
+ * has not finished, else its own return code, with the results copied to
+ * *res (may be NULL); the ticket is then consumed. */
+int dk_poll(dk_server *s, uint64_t ticket, int timeout_ms, dk_result *res);
+/* Releases a session register (after the request using it, if one does). */
+int dk_session_free(dk_server *s, uint64_t session);
+int dk_server_stats(dk_server *s, uint64_t *batches, uint64_t *batched,
+		    uint64_t *completed, uint64_t *rejected);
+/* Runs every admitted request to its end, stops the threads and frees the
+ * session registers; requests not polled by then are dropped. */
+void dk_server_free(dk_server *s);
+
 /* ---- timing, tuning, device ------------------------------------------------
  * No reference counterparts. */
 

@@ -247,3 +247,98 @@ def serve_plain_state(capi, n, steps):
     for m, t, c, a in steps:
         st = capi.apply(st, capi.Gate(m), t, c, a)
     return st
+
+
+def test_sessions_and_live_joins(capi, oracle):
+    """Registers kept resident as sessions, then requests that join two live
+    session registers, apply more gates to one in place, read one out; each
+    result is the same whether the requests run one at a time or all at
+    once, and equals the oracle's."""
+    from doki_amd import serve
+    rng = W.rng_for(4040)
+    sizes = [7, 9, 6, 8, 10, 11, 18]
+    builds = [(n, [W.random_gate_step(rng, n) for _ in range(40)]) for n in sizes]
+    extra = [W.random_gate_step(rng, 10) for _ in range(25)]
+    rolls = rng.random(11).tolist()
+
+    def requests(sid):
+        return [dict(rid=0, arrival=0.0, op="join", regs=[("session", sid[0], []),
+                                                         ("session", sid[1], [])]),
+                dict(rid=1, arrival=0.0, op="join", regs=[("session", sid[2], []),
+                                                         ("session", sid[3], [])],
+                     keep_result=True),
+                dict(rid=2, arrival=0.0, op="prob", regs=[("session", sid[4], extra)]),
+                dict(rid=3, arrival=0.0, op="measure", regs=[("session", sid[5], [])],
+                     rolls=rolls),
+                dict(rid=4, arrival=0.0, op="density", regs=[("session", sid[6], [])],
+                     keep=(0, 5, 17)),
+                dict(rid=5, arrival=0.0, op="join", regs=[("session", sid[6], []),
+                                                         (6, builds[2][1])])]
+
+    def run(one_at_a_time):
+        with serve.Server(workers=3) as srv:
+            hs = [srv.submit(dict(rid=100 + i, arrival=0.0, op="keep", regs=[b]))
+                  for i, b in enumerate(builds)]
+            sid = [srv.session_of(h) for h in hs]
+            reqs = requests(sid)
+            if one_at_a_time:
+                res = [srv.submit(r).wait(120) for r in reqs]
+            else:
+                hh, _ = serve.serve_trace(srv, reqs, time_scale=0.0)
+                res = [h.wait(0) for h in hh]
+            kept = res[1]["session"]
+            srv._session_nq[kept] = sizes[2] + sizes[3]
+            after = srv.submit(dict(rid=9, arrival=0.0, op="prob",
+                                    regs=[("session", kept, [])])).wait(60)
+            for s in sid + [kept]:
+                srv.free_session(s)
+        return res, after
+
+    alone, after_a = run(True)
+    mix, after_m = run(False)
+    for a, m in zip(alone, mix):
+        assert_same({k: v for k, v in a.items() if k != "session"},
+                    {k: v for k, v in m.items() if k != "session"})
+    assert_same(after_a, after_m)
+    st = [oracle_run(oracle, oracle.new(n), steps) for n, steps in builds]
+
+    def probs(o):
+        return np.array([oracle.prob(o, q) for q in range(o.nq)])
+
+    j01 = oracle.join(st[0], st[1])
+    j23 = oracle.join(st[2], st[3])
+    s4 = oracle_run(oracle, st[4], extra)
+    j6 = oracle.join(st[6], oracle_run(oracle, oracle.new(6), builds[2][1]))
+    assert np.max(np.abs(mix[0]["probs"] - probs(j01))) <= ATOL
+    assert np.max(np.abs(mix[1]["probs"] - probs(j23))) <= ATOL
+    assert np.max(np.abs(after_m["probs"] - probs(j23))) <= ATOL
+    assert np.max(np.abs(mix[2]["probs"] - probs(s4))) <= ATOL
+    rc, _, olst = oracle.measure(st[5], (1 << 11) - 1, rolls)
+    if not any(abs(p - r) < 1e-9 for p, r in zip(mix[3]["p1"], rolls)):
+        assert mix[3]["outcomes"] == olst
+    assert np.max(np.abs(mix[4]["rho"] -
+                         reduced_density_einsum(st[6].values(), (0, 5, 17)))) <= ATOL
+    assert np.max(np.abs(mix[5]["probs"] - probs(j6))) <= ATOL
+
+
+def test_server_requests_through_the_c_abi(capi):
+    """The native server's entry points: a finished request's ticket is
+    consumed by dk_poll, unknown tickets and sessions are errors, and a
+    request over the budget completes at once with DK_E_BUDGET."""
+    import ctypes as C
+    from doki_amd import serve
+    L = serve._bind()
+    with serve.Server(workers=1, budget=1 << 22) as srv:
+        h = srv.submit(dict(rid=0, arrival=0.0, op="prob",
+                            regs=[(10, W.random_circuit(1, 10, 20)[0])]))
+        assert h.wait(60)["probs"].shape == (10,)
+        assert L.dk_poll(srv.ptr, h.ticket, 0, None) != 0      # consumed
+        assert L.dk_poll(srv.ptr, 987654, 0, None) != 0
+        assert L.dk_session_free(srv.ptr, 424242) != 0
+        big = srv.submit(dict(rid=1, arrival=0.0, op="prob",
+                              regs=[(20, W.random_circuit(2, 20, 5)[0])]))
+        with pytest.raises(serve.Rejected):
+            big.wait(10)
+        stats = [C.c_uint64() for _ in range(4)]
+        capi.check(L.dk_server_stats(srv.ptr, *[C.byref(x) for x in stats]))
+        assert stats[2].value >= 1 and stats[3].value == 1
