@@ -229,10 +229,13 @@ int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
 
 int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev);
 
-int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev);
+// Reduced densities; amplitudes whose index fails (idx & tmask) == tval
+// (tmask on traced bits only) are left out of the traced sum.
+int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev,
+		      u64 tmask = 0, u64 tval = 0);
 int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 			       const u64 *rows_dev, const u64 *cols_dev,
-			       double2 *out_dev);
+			       double2 *out_dev, u64 tmask = 0, u64 tval = 0);
 
 int dk_launch_fill_zero_state(dk_state *s);
 int dk_launch_set_norm(dk_state *s, double v);

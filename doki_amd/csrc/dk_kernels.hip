@@ -716,7 +716,17 @@ struct DensArgs {
 	unsigned char kpos[TILE_BITS]; // kept positions, ascending
 	int a;              // kept qubits
 	int m;              // traced qubits
+	u64 tmask, tval;    // only amplitudes with (index & tmask) == tval
+	                    // count (tmask: traced bits; 0 = all)
 };
+
+// Amplitude idx of the state as the density reads it: zero where the traced
+// bits fail the constraint (a projection folded into the density).
+template <typename T>
+__device__ __forceinline__ T dens_load(const T *in, u64 idx, const DensArgs &da)
+{
+	return (idx & da.tmask) == da.tval ? in[idx] : czero<T>();
+}
 
 __device__ __forceinline__ u64 traced_index(u64 t, const DensArgs &da)
 {
@@ -753,11 +763,11 @@ __global__ __launch_bounds__(TPB) void k_density_tile(
 		if (t < t1) {
 			const u64 tix = traced_index(t, da);
 			if (ri < dim) {
-				const T a = in[keep_index(ri, da) | tix];
+				const T a = dens_load(in, keep_index(ri, da) | tix, da);
 				vi = make_double2(a.x * inv, a.y * inv);
 			}
 			if (rj < dim) {
-				const T a = in[keep_index(rj, da) | tix];
+				const T a = dens_load(in, keep_index(rj, da) | tix, da);
 				vj = make_double2(a.x * inv, a.y * inv);
 			}
 		}
@@ -827,7 +837,7 @@ __global__ __launch_bounds__(TPB) void k_density_gram(
 		const u64 t0 = c * (u64)cols;
 		for (int e = threadIdx.x; e < ch; e += TPB) {
 			const int i = e / cols, t = e - i * cols;
-			const T v = in[keep_index((u64)i, da) | traced_index(t0 + t, da)];
+			const T v = dens_load(in, keep_index((u64)i, da) | traced_index(t0 + t, da), da);
 			lds[e] = make_double2(v.x * inv, v.y * inv);
 		}
 		__syncthreads();
@@ -893,7 +903,7 @@ __global__ __launch_bounds__(TPB) void k_density_elements(
 	double2 acc = make_double2(0.0, 0.0);
 	for (u64 t = threadIdx.x; t < T_; t += TPB) {
 		const u64 tix = traced_index(t, da);
-		const T a = in[ki | tix], b = in[kj | tix];
+		const T a = dens_load(in, ki | tix, da), b = dens_load(in, kj | tix, da);
 		const double2 x = make_double2(a.x * inv, a.y * inv);
 		const double2 y = make_double2(b.x * inv, -b.y * inv);
 		acc = cmac(acc, x, y);
@@ -1494,9 +1504,12 @@ int dk_launch_first_nonzero(const dk_state *s, u64 start, u64 *dst_dev)
 	return DK_OK;
 }
 
-static void dens_args(const dk_state *s, u64 keep_mask, DensArgs &da)
+static void dens_args(const dk_state *s, u64 keep_mask, DensArgs &da, u64 tmask,
+		      u64 tval)
 {
 	memset(&da, 0, sizeof(da));
+	da.tmask = tmask;
+	da.tval = tval & tmask;
 	int a = 0;
 	for (unsigned p = 0; p < s->nq; ++p)
 		if ((keep_mask >> p) & 1)
@@ -1556,10 +1569,11 @@ static int launch_density_gram(const dk_state *s, const DensArgs &da,
 	return DK_OK;
 }
 
-int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
+int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev,
+		      u64 tmask, u64 tval)
 {
 	DensArgs da;
-	dens_args(s, keep_mask, da);
+	dens_args(s, keep_mask, da, tmask, tval);
 	if (da.a <= 6)
 		return launch_density_gram(s, da, rho_dev);
 	const u64 dim = 1ull << da.a;
@@ -1595,10 +1609,10 @@ int dk_launch_density(const dk_state *s, u64 keep_mask, double2 *rho_dev)
 
 int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 			       const u64 *rows_dev, const u64 *cols_dev,
-			       double2 *out_dev)
+			       double2 *out_dev, u64 tmask, u64 tval)
 {
 	DensArgs da;
-	dens_args(s, keep_mask, da);
+	dens_args(s, keep_mask, da, tmask, tval);
 	if (count == 0)
 		return DK_OK;
 #define L(T)                                                                  \

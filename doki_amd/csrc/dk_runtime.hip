@@ -973,14 +973,22 @@ static int popcount64(u64 x)
 extern "C" int dk_reduced_density(const dk_state *s, uint64_t keep_mask,
 				  double *rho)
 {
+	return dk_reduced_density_masked(s, keep_mask, 0, 0, rho);
+}
+
+extern "C" int dk_reduced_density_masked(const dk_state *s, uint64_t keep_mask,
+					 uint64_t traced_mask,
+					 uint64_t traced_value, double *rho)
+{
 	DK_INIT();
 	if (!s || !rho) {
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
 	const u64 valid = (1ull << s->nq) - 1;
-	if (keep_mask & ~valid) {
-		dk_set_error("kept qubit out of range");
+	if ((keep_mask & ~valid) || (traced_mask & ~valid) ||
+	    (traced_mask & keep_mask)) {
+		dk_set_error("kept or constrained qubit out of range");
 		return DK_E_ARG;
 	}
 	DK_SETTLE(s);
@@ -993,7 +1001,7 @@ extern "C" int dk_reduced_density(const dk_state *s, uint64_t keep_mask,
 	const u64 dim = 1ull << a;
 	double2 *dev = nullptr;
 	DK_TRY(dk_device_alloc(dim * dim * sizeof(double2), (void **)&dev));
-	int rc = dk_launch_density(s, keep_mask, dev);
+	int rc = dk_launch_density(s, keep_mask, dev, traced_mask, traced_value);
 	if (!rc)
 		rc = copy_to_host(rho, dev, 2 * dim * dim);
 	dk_device_free(dev, dim * dim * sizeof(double2));
@@ -1004,14 +1012,24 @@ extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask
 					   size_t count, const uint64_t *rows,
 					   const uint64_t *cols, double *out)
 {
+	return dk_reduced_density_elements_masked(s, keep_mask, 0, 0, count, rows,
+						  cols, out);
+}
+
+extern "C" int dk_reduced_density_elements_masked(
+	const dk_state *s, uint64_t keep_mask, uint64_t traced_mask,
+	uint64_t traced_value, size_t count, const uint64_t *rows,
+	const uint64_t *cols, double *out)
+{
 	DK_INIT();
 	if (!s || (count && (!rows || !cols || !out))) {
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
 	const u64 valid = (1ull << s->nq) - 1;
-	if (keep_mask & ~valid) {
-		dk_set_error("kept qubit out of range");
+	if ((keep_mask & ~valid) || (traced_mask & ~valid) ||
+	    (traced_mask & keep_mask)) {
+		dk_set_error("kept or constrained qubit out of range");
 		return DK_E_ARG;
 	}
 	DK_SETTLE(s);
@@ -1042,7 +1060,7 @@ extern "C" int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask
 	}
 	if (!rc)
 		rc = dk_launch_density_elements(s, keep_mask, count, drows, dcols,
-						dout);
+						dout, traced_mask, traced_value);
 	if (!rc)
 		rc = copy_to_host(out, dout, 2 * count);
 	dk_device_free(buf, bytes);

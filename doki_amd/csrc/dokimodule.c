@@ -1221,6 +1221,10 @@ typedef struct fm_t {
 	uint64_t keep_mask;
 	unsigned keep_count;
 	unsigned char keep_pos[64];
+	/* projections folded in: register qubits in pmask must have the bit
+	 * of pval -- a row condition for kept qubits, a restriction of the
+	 * traced sum for traced ones */
+	uint64_t pmask, pval;
 	double *cache;        /* materialised matrix, or NULL */
 	int e;                /* FM_PTRACE: index bit traced out */
 	int hadamard;         /* FM_WALSH */
@@ -1345,12 +1349,29 @@ static int64_t ins_bit(int value, int64_t pos, int bit)
 	return pos % step + (pos / step) * (step << 1) + value * step;
 }
 
+/* Whether reduced row index i satisfies the projections on kept qubits. */
+static int fm_density_row_ok(const fm_t *f, int64_t i)
+{
+	unsigned b;
+	for (b = 0; b < f->keep_count; b++) {
+		const unsigned q = f->keep_pos[b];
+		if (((f->pmask >> q) & 1) && ((uint64_t)(i >> b) & 1) != ((f->pval >> q) & 1))
+			return 0;
+	}
+	return 1;
+}
+
 static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 {
 	reg_t *r = reg_get(f->state_cap, "NULL pointer to registry");
+	const uint64_t tmask = f->pmask & ~f->keep_mask, tval = f->pval & tmask;
 	int rc;
 	if (!r)
 		return -1;
+	if (!fm_density_row_ok(f, i)) {
+		*out = 0;
+		return 0;
+	}
 	if (f->keep_count <= FM_MATERIALISE_MAX_QUBITS) {
 		if (!f->cache) {
 			size_t n = (size_t)f->r * (size_t)f->r;
@@ -1359,7 +1380,8 @@ static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 				PyErr_NoMemory();
 				return -1;
 			}
-			rc = dk_reduced_density(r->st, f->keep_mask, f->cache);
+			rc = dk_reduced_density_masked(r->st, f->keep_mask, tmask, tval,
+						       f->cache);
 			if (rc) {
 				PyMem_Free(f->cache);
 				f->cache = NULL;
@@ -1375,7 +1397,8 @@ static int fm_density_get(fm_t *f, int64_t i, int64_t j, cplx *out)
 	} else {
 		uint64_t ri = (uint64_t)i, cj = (uint64_t)j;
 		double v[2];
-		rc = dk_reduced_density_elements(r->st, f->keep_mask, 1, &ri, &cj, v);
+		rc = dk_reduced_density_elements_masked(r->st, f->keep_mask, tmask, tval,
+							1, &ri, &cj, v);
 		if (rc) {
 			dk_fail(rc, "[DENSITY] Unknown error");
 			return -1;
@@ -1398,6 +1421,118 @@ static int fm_read(PyObject *cap, int64_t i, int64_t j, cplx *out)
 		return 8;
 	}
 	return fm_get(m, i, j, out);
+}
+
+/* Materialises an FM_APPLY node: reads its input column and its gate (a
+ * failed read stands as NaN, so it spreads to every element that uses it, as
+ * in _ApplyGateFunction, qops.c:449-563), applies the gate to the column as
+ * one in-place dk_apply on the device, and keeps the raw result (not
+ * normalised: the node is a plain matrix-vector product) as f->cache.  A gate
+ * on no qubits (1x1) is a scalar on the rows its controls select: it runs as
+ * the diagonal gate diag(g, g) on a qubit that is neither control nor
+ * anticontrol, or, when there is none, as diag(1, g) / diag(g, 1) on one of
+ * the controls / anticontrols; a zero-qubit column is padded to one qubit. */
+static int fm_apply_materialise(fm_t *f)
+{
+	const fm_buf_t *b = f->buf;
+	const unsigned *ct = b->ids + b->nt, *ac = b->ids + b->nt + b->nc;
+	const int64_t rows = f->r;
+	const unsigned nq = rows > 1 ? (unsigned)__builtin_ctzll((uint64_t)rows) : 1;
+	const int64_t srows = (int64_t)1 << nq;
+	const int64_t dg = (int64_t)1 << b->nt;
+	unsigned tg[6], k = b->nt, u;
+	uint64_t cm = 0, am = 0;
+	double *col = NULL, *mat = NULL, norm;
+	dk_state *st = NULL;
+	dk_gate *gt = NULL;
+	int64_t i, j;
+	int rc;
+	cplx x;
+	if (b->nt > 6) {
+		PyErr_SetString(DokiError, "[APPLY] gates act on at most 6 qubits");
+		return -1;
+	}
+	for (u = 0; u < b->nc; u++)
+		cm |= (uint64_t)1 << ct[u];
+	for (u = 0; u < b->na; u++)
+		am |= (uint64_t)1 << ac[u];
+	col = (double *)PyMem_Calloc((size_t)srows * 2, sizeof(double));
+	mat = (double *)PyMem_Calloc(2 * 64 * 64, sizeof(double));
+	if (!col || !mat) {
+		PyErr_NoMemory();
+		goto fail;
+	}
+	for (i = 0; i < rows; i++) {
+		rc = fm_read(f->a_cap, i, 0, &x);
+		if (rc < 0)
+			goto fail;
+		if (rc)
+			x = NAN;
+		col[2 * i] = creal(x);
+		col[2 * i + 1] = cimag(x);
+	}
+	for (i = 0; i < dg; i++)
+		for (j = 0; j < dg; j++) {
+			rc = fm_read(f->b_cap, i, j, &x);
+			if (rc < 0)
+				goto fail;
+			if (rc)
+				x = NAN;
+			mat[2 * (i * dg + j)] = creal(x);
+			mat[2 * (i * dg + j) + 1] = cimag(x);
+		}
+	for (u = 0; u < b->nt; u++)
+		tg[u] = b->ids[u];
+	if (b->nt == 0) {
+		/* the scalar g = mat[0..1] on a diagonal 2x2 gate */
+		const uint64_t used = cm | am;
+		double one[2] = {1.0, 0.0}, gv[2] = {mat[0], mat[1]};
+		const double *d0 = gv, *d1 = gv;
+		k = 1;
+		tg[0] = 0;
+		while (tg[0] < nq && ((used >> tg[0]) & 1))
+			tg[0]++;
+		if (tg[0] == nq) {
+			/* every qubit is a control or an anticontrol */
+			tg[0] = 0;
+			if ((cm >> 0) & 1) {
+				d0 = one;
+				cm &= ~(uint64_t)1;
+			} else {
+				d1 = one;
+				am &= ~(uint64_t)1;
+			}
+		}
+		memset(mat, 0, 8 * sizeof(double));
+		mat[0] = d0[0];
+		mat[1] = d0[1];
+		mat[6] = d1[0];
+		mat[7] = d1[1];
+	}
+	rc = dk_state_from_host(nq, col, &st);
+	if (!rc)
+		rc = dk_gate_new(k, mat, &gt);
+	if (!rc)
+		rc = dk_apply(st, gt, tg, cm, am, 1, &st);
+	if (!rc)
+		rc = dk_read_raw(st, 0, (uint64_t)srows, col, &norm);
+	if (rc) {
+		dk_fail(rc, "[APPLY] Unknown error");
+		goto fail;
+	}
+	dk_gate_free(gt);
+	dk_state_free(st);
+	PyMem_Free(mat);
+	f->cache = col;
+	return 0;
+fail:
+	if (gt)
+		dk_gate_free(gt);
+	if (st)
+		dk_state_free(st);
+	PyMem_Free(col);
+	PyMem_Free(mat);
+	return -1;
 }
 
 /* Value of a leaf at (i, j), indices already swapped for transposition.
@@ -1492,56 +1627,13 @@ static int fm_leaf(fm_t *f, int64_t i, int64_t j, cplx *out)
 		}
 		return fm_read(f->a_cap, i, j, out) < 0 ? -1 : 0;
 	}
-	case FM_APPLY: {
-		const fm_buf_t *b = f->buf;
-		const unsigned *tg = b->ids, *ct = b->ids + b->nt,
-			       *ac = b->ids + b->nt + b->nc;
-		fm_t *g = fm_of(f->b_cap);
-		int64_t n, reg = i;
-		unsigned k;
-		cplx val = 0, x, y;
-		int r1, r2;
-		(void)j;
-		/* a failed read makes the element NaN (_ApplyGateFunction) */
-		for (k = 0; k < b->nc; k++)
-			if (!((i >> ct[k]) & 1))
-				goto passthrough;
-		for (k = 0; k < b->na; k++)
-			if ((i >> ac[k]) & 1)
-				goto passthrough;
-		if (!g) {
-			*out = NAN;
-			return 0;
-		}
-		for (n = 0; n < g->r; n++) {
-			int64_t row = 0;
-			for (k = 0; k < b->nt; k++) {
-				row += (int64_t)((i >> tg[k]) & 1) << k;
-				if ((n >> k) & 1)
-					reg |= (int64_t)1 << tg[k];
-				else
-					reg &= ~((int64_t)1 << tg[k]);
-			}
-			r1 = fm_read(f->a_cap, reg, 0, &x);
-			r2 = r1 ? 0 : fm_read(f->b_cap, row, n, &y);
-			if (r1 < 0 || r2 < 0)
-				return -1;
-			if (r1 || r2) {
-				*out = NAN;
-				return 0;
-			}
-			val = val + x * y;
-		}
-		*out = val;
-		return 0;
-	passthrough:
-		r1 = fm_read(f->a_cap, i, 0, out);
-		if (r1 < 0)
+	case FM_APPLY:
+		/* the column is materialised on the device at the first read */
+		if (!f->cache && fm_apply_materialise(f) < 0)
 			return -1;
-		if (r1)
-			*out = NAN;
+		*out = f->cache[2 * i] + I * f->cache[2 * i + 1];
+		(void)j;
 		return 0;
-	}
 	default:
 		*out = NAN;
 		return 0;
@@ -1750,6 +1842,8 @@ static PyObject *doki_funmatrix_partialtrace(PyObject *self, PyObject *args)
 				o->keep_pos[w++] = f->keep_pos[k];
 		o->keep_count = w;
 		o->keep_mask = f->keep_mask & ~((uint64_t)1 << f->keep_pos[e]);
+		o->pmask = f->pmask;
+		o->pval = f->pval;
 	} else {
 		o->a_cap = cap;
 		Py_INCREF(cap);
@@ -2199,11 +2293,32 @@ static PyObject *doki_funmatrix_projection(PyObject *self, PyObject *args)
 	f = fm_new(FM_PROJECTION, m->r, m->c);
 	if (!f)
 		return NULL;
+	f->mem = REF_FMATRIX_BYTES + REF_PROJECTION_BYTES + m->mem;
+	if (m->kind == FM_DENSITY && qubit < m->keep_count && m->s == 1.0 &&
+	    !m->transpose && !m->conjugate &&
+	    !(((m->pmask >> m->keep_pos[qubit]) & 1) &&
+	      ((m->pval >> m->keep_pos[qubit]) & 1) != (uint64_t)(value != 0))) {
+		/* a projection of a registry density: the same leaf with one
+		 * more bit condition, evaluated on the GPU (a condition that
+		 * contradicts an earlier one stays a node of its own) */
+		const unsigned q = m->keep_pos[qubit];
+		unsigned k;
+		f->kind = FM_DENSITY;
+		f->state_cap = m->state_cap;
+		Py_INCREF(f->state_cap);
+		f->keep_count = m->keep_count;
+		for (k = 0; k < m->keep_count; k++)
+			f->keep_pos[k] = m->keep_pos[k];
+		f->keep_mask = m->keep_mask;
+		f->pmask = m->pmask | ((uint64_t)1 << q);
+		f->pval = (m->pval & ~((uint64_t)1 << q)) |
+			  ((uint64_t)(value != 0) << q);
+		return fm_wrap(f);
+	}
 	f->a_cap = cap;
 	Py_INCREF(cap);
 	f->qubit = qubit;
 	f->value = value != 0;
-	f->mem = REF_FMATRIX_BYTES + REF_PROJECTION_BYTES + m->mem;
 	return fm_wrap(f);
 }
 

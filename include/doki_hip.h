@@ -251,6 +251,21 @@ int dk_reduced_density_elements(const dk_state *s, uint64_t keep_mask,
 				size_t count, const uint64_t *rows,
 				const uint64_t *cols, double *out);
 
+/* The same two with projections folded in: only the terms of the traced
+ * sum whose state index satisfies (index & traced_mask) == traced_value are
+ * added (traced_mask names traced qubits only).  This is the partial trace
+ * of a projected density: a projection keeps the rows whose bit has the
+ * given value (funmatrix.c:837-918), so tracing that qubit out afterwards
+ * (funmatrix.c:920-1037) keeps only the term with that value. */
+int dk_reduced_density_masked(const dk_state *s, uint64_t keep_mask,
+			      uint64_t traced_mask, uint64_t traced_value,
+			      double *rho);
+int dk_reduced_density_elements_masked(const dk_state *s, uint64_t keep_mask,
+				       uint64_t traced_mask,
+				       uint64_t traced_value, size_t count,
+				       const uint64_t *rows, const uint64_t *cols,
+				       double *out);
+
 /* ---- gate lists and fusion ------------------------------------------------
  * No single reference counterpart: a caller (QSimov's circuit layer) issues
  * the same gates one registry_apply at a time (doki.c:722-941).  A circuit

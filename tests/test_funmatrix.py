@@ -7,11 +7,12 @@ getitem and the node functions), so results are compared bit-for-bit with
 the reference's own outputs stored in golden/funmatrix.* (written by
 golden/make_golden.py from the reference's C sources).
 
-projection and apply are pinned by mathematics instead (a row mask; the full
-unitary of a controlled gate): the reference's compiled parsers for those two
-calls are not usable (see make_golden.funmatrix_fixture).  Chains whose
-leaves are density matrices of GPU registries are covered in
-test_gpu_parity.py.
+projection is pinned by mathematics instead (a row mask): the reference's
+compiled parsers for projection and apply are not usable (see
+make_golden.funmatrix_fixture).  apply nodes are evaluated on the GPU (one
+gate application over the input column), so only their validation runs
+here; their values, and chains whose leaves are density matrices of GPU
+registries, are covered in test_gpu_funmatrix.py and test_gpu_parity.py.
 """
 import json
 
@@ -107,78 +108,6 @@ def test_projection_past_row_bits_and_trace(d):
                           dense(d, b))
     assert not dense(d, F(d, "projection", b, 5, True)).any()
     assert F(d, "trace", F(d, "projection", b, 0, True)) == 4
-
-
-def full_unitary(n, g, targets, controls, acontrols):
-    """2^n x 2^n matrix of gate g on `targets` (target k = bit k of g's
-    row/column index) under controls and anticontrols."""
-    N = 1 << n
-    U = np.zeros((N, N), dtype=np.complex128)
-    for i in range(N):
-        if not (all((i >> c) & 1 for c in controls)
-                and not any((i >> a) & 1 for a in acontrols)):
-            U[i, i] = 1
-            continue
-        row = sum(((i >> t) & 1) << k for k, t in enumerate(targets))
-        for m in range(g.shape[0]):
-            j = i
-            for k, t in enumerate(targets):
-                j = (j | (1 << t)) if (m >> k) & 1 else (j & ~(1 << t))
-            U[i, j] += g[row, m]
-    return U
-
-
-def column(d, psi):
-    """A column-vector node holding psi: create(M) @ statezero(n), where
-    M's first column is psi."""
-    n = int(np.log2(len(psi)))
-    M = np.zeros((len(psi), len(psi)), dtype=np.complex128)
-    M[:, 0] = psi
-    return F(d, "matmul", F(d, "create", M.tolist()), F(d, "statezero", n))
-
-
-def test_apply_equals_full_unitary(d):
-    rng = np.random.default_rng(5)
-    for _ in range(40):
-        n = int(rng.integers(2, 6))
-        k = int(rng.integers(1, min(3, n) + 1))
-        qs = [int(x) for x in rng.permutation(n)]
-        targets, rest = qs[:k], qs[k:]
-        nc = int(rng.integers(0, len(rest) + 1))
-        na = int(rng.integers(0, len(rest) - nc + 1))
-        ctl, act = set(rest[:nc]), set(rest[nc:nc + na])
-        g = (rng.standard_normal((1 << k, 1 << k))
-             + 1j * rng.standard_normal((1 << k, 1 << k)))
-        psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
-        node = F(d, "apply", column(d, psi), F(d, "create", g.tolist()),
-                 targets, ctl, act)
-        assert F(d, "shape", node) == (1 << n, 1)
-        got = dense(d, node)[:, 0]
-        want = full_unitary(n, g, targets, sorted(ctl), sorted(act)) @ psi
-        assert np.max(np.abs(got - want)) <= 1e-13
-
-
-def test_apply_known_answers(d):
-    h = F(d, "hadamard", 1)
-    x = F(d, "create", [[0, 1], [1, 0]])
-    sz = F(d, "statezero", 2)
-    # H on both qubits of |00>: uniform amplitudes 1/2
-    hh = F(d, "apply", F(d, "apply", sz, h, [0], None, None), h, [1], None, None)
-    assert np.allclose(dense(d, hh)[:, 0], 0.5, atol=1e-15)
-    # CNOT(control 0, target 1) after H on 0: Bell state (|00> + |11>)/sqrt2
-    bell = F(d, "apply", F(d, "apply", sz, h, [0], set(), set()),
-             x, [1], {0}, set())
-    r = 1 / np.sqrt(2)
-    assert np.allclose(dense(d, bell)[:, 0], [r, 0, 0, r], atol=1e-15)
-    # anticontrol: X on 1 when qubit 0 is 0, from |00> gives |10> (index 2)
-    v = dense(d, F(d, "apply", sz, x, [1], set(), {0}))[:, 0]
-    assert np.array_equal(v, [0, 0, 1, 0])
-    # a 2-qubit gate with swapped targets: SWAP of |01> (index 1) -> index 2
-    swap = F(d, "create", [[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0],
-                           [0, 0, 0, 1]])
-    e1 = column(d, np.array([0, 1, 0, 0], dtype=complex))
-    assert np.array_equal(dense(d, F(d, "apply", e1, swap, [1, 0], None,
-                                     None))[:, 0], [0, 0, 1, 0])
 
 
 @pytest.mark.parametrize("call,msg", [
