@@ -168,16 +168,16 @@ extern "C" void dk_circuit_free(dk_circuit *c)
 	delete c;
 }
 
-extern "C" int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
-			      const unsigned *targets, uint64_t ctrl_mask,
-			      uint64_t actrl_mask)
+// Checks one gate against the circuit and builds its record.
+static int make_gate(const dk_circuit *c, unsigned k, const double *matrix,
+		     const unsigned *targets, uint64_t ctrl_mask,
+		     uint64_t actrl_mask, CGate &g)
 {
-	if (!c || !matrix || !targets || k < 1 || k > DK_MAX_GATE_QUBITS) {
+	if (!matrix || !targets || k < 1 || k > DK_MAX_GATE_QUBITS) {
 		dk_set_error("gates act on 1..%d qubits", DK_MAX_GATE_QUBITS);
 		return DK_E_ARG;
 	}
 	const u64 valid = c->nq >= 64 ? ~0ull : ((1ull << c->nq) - 1);
-	CGate g;
 	g.k = k;
 	u64 tm = 0;
 	for (unsigned b = 0; b < k; ++b) {
@@ -203,15 +203,37 @@ extern "C" int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
 	g.m.resize(n);
 	for (size_t i = 0; i < n; ++i)
 		g.m[i] = cd(matrix[2 * i], matrix[2 * i + 1]);
-	c->gates.push_back(std::move(g));
-	// a new gate invalidates any previous plan
+	return DK_OK;
+}
+
+// New gates invalidate any previous plan and its device copies.
+static void invalidate_plan(dk_circuit *c)
+{
 	drop_device_gates(c);
 	c->blocks.clear();
 	c->fuse_k = 0;
 	c->passes_planned = false;
+}
+
+extern "C" int dk_circuit_add(dk_circuit *c, unsigned k, const double *matrix,
+			      const unsigned *targets, uint64_t ctrl_mask,
+			      uint64_t actrl_mask)
+{
+	if (!c) {
+		dk_set_error("NULL circuit");
+		return DK_E_ARG;
+	}
+	CGate g;
+	const int rc = make_gate(c, k, matrix, targets, ctrl_mask, actrl_mask, g);
+	if (rc)
+		return rc;
+	c->gates.push_back(std::move(g));
+	invalidate_plan(c);
 	return DK_OK;
 }
 
+// All or nothing: every gate is checked before any is appended, and the
+// plan is invalidated once per call.
 extern "C" int dk_circuit_add_many(dk_circuit *c, size_t count,
 				   const double *matrices, const unsigned *targets,
 				   const uint64_t *ctrl_masks,
@@ -221,17 +243,20 @@ extern "C" int dk_circuit_add_many(dk_circuit *c, size_t count,
 		dk_set_error("NULL argument");
 		return DK_E_ARG;
 	}
-	const size_t before = c->gates.size();
+	std::vector<CGate> add(count);
 	for (size_t i = 0; i < count; ++i) {
-		const int rc = dk_circuit_add(c, 1, matrices + 8 * i, targets + i,
-					      ctrl_masks ? ctrl_masks[i] : 0,
-					      actrl_masks ? actrl_masks[i] : 0);
-		if (rc) {
-			// all or nothing: drop the gates of this call
-			c->gates.resize(before);
+		const int rc = make_gate(c, 1, matrices + 8 * i, targets + i,
+					 ctrl_masks ? ctrl_masks[i] : 0,
+					 actrl_masks ? actrl_masks[i] : 0, add[i]);
+		if (rc)
 			return rc;
-		}
 	}
+	if (!count)
+		return DK_OK;
+	c->gates.reserve(c->gates.size() + count);
+	for (auto &g : add)
+		c->gates.push_back(std::move(g));
+	invalidate_plan(c);
 	return DK_OK;
 }
 

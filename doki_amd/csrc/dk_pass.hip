@@ -431,7 +431,8 @@ __device__ __forceinline__ u64 slot_index(unsigned slots, const u64 (&hb)[HIGH_B
 // Exchange buffer.  complex128: half a tile (32 KiB; exchanges run in two
 // rounds), so that LDS admits the three workgroups per CU the kernel's
 // registers allow; complex64: the whole tile (32 KiB; one round), since its
-// registers already cap the CU at four workgroups.
+// registers already cap the CU at four workgroups.  32 KiB exactly: any
+// other LDS variable would cost a workgroup per CU.
 template <typename T> constexpr bool exch_halves() { return sizeof(T) == 16; }
 template <typename T> constexpr int exch_entries()
 {
@@ -597,6 +598,7 @@ __device__ __forceinline__ void tile_run(
 	if (!partials)
 		return;
 	s = wave_sum(s);
+	__syncthreads(); // red aliases the exchange buffer: its reads are done
 	if (lane == 0)
 		red[w] = s;
 	__syncthreads();
@@ -633,15 +635,24 @@ __device__ __forceinline__ void pass_tile(
 	tile_run<T>(out, pd, ops, partials, tile, base, hb, lane, w, vr, vi, lds, red);
 }
 
+// Occupancy: the exchange buffer is the kernel's only LDS (32 KiB: five
+// workgroups per CU), so registers set the waves per SIMD.  complex128 asks
+// for three: the op code then fits 168 VGPRs without spilling (left alone,
+// the round kernel took 201 and ran at two; four waves spill the tile).
+// complex64 runs at four in about 110 VGPRs; five spill (measured slower).
+template <typename T> constexpr int pass_waves() { return sizeof(T) == 16 ? 3 : 4; }
+
 // LOOP: tiles beyond the grid (launches of more than 2^31 threads).
 template <typename T, bool LOOP>
-__global__ __launch_bounds__(THREADS) void k_pass(
+__global__ __launch_bounds__(THREADS)
+__attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass(
 	const T *in, T *out, const double *__restrict__ nrm, PassDesc pd,
 	const PassOp *__restrict__ ops, double *__restrict__ partials, u64 ntiles)
 {
 	typedef typename real_of<T>::type R;
 	__shared__ T lds[exch_entries<T>()];
-	__shared__ double red[THREADS / 64];
+	// the per-wave partial sums reuse the exchange buffer
+	double *red = reinterpret_cast<double *>(lds);
 	const R inv = (R)(1.0 / nrm[DK_SCAL_NORM]);
 	if constexpr (LOOP) {
 		for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -660,30 +671,26 @@ __global__ __launch_bounds__(THREADS) void k_pass(
 // first round, whose tiles start from |0...0> instead of the pool.  No
 // renormalisation between passes (the caller normalises at the end).
 template <typename T>
-__global__ __launch_bounds__(THREADS) void k_pass_round(
+__global__ __launch_bounds__(THREADS)
+__attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass_round(
 	T *pool, const PassEntry *__restrict__ ents, unsigned nents,
 	const PassOp *__restrict__ ops, u64 nitems, int init)
 {
 	typedef typename real_of<T>::type R;
 	__shared__ T lds[exch_entries<T>()];
-	__shared__ double red[THREADS / 64];
-	__shared__ unsigned ent_sh;
 	const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	for (u64 it = blockIdx.x; it < nitems; it += gridDim.x) {
-		if (threadIdx.x == 0) {
-			unsigned lo = 0, hi = nents - 1;
-			while (lo < hi) {
-				const unsigned mid = (lo + hi + 1) / 2;
-				if (ents[mid].first <= it)
-					lo = mid;
-				else
-					hi = mid - 1;
-			}
-			ent_sh = lo;
+		// the entry holding item `it`, found by every wave on its own
+		// (uniform values: scalar loads, no LDS, no barrier)
+		unsigned lo = 0, hi = nents - 1;
+		while (lo < hi) {
+			const unsigned mid = (lo + hi + 1) / 2;
+			if (ents[mid].first <= it)
+				lo = mid;
+			else
+				hi = mid - 1;
 		}
-		__syncthreads();
-		// the same entry for the whole workgroup: scalar loads of it
-		const unsigned e = __builtin_amdgcn_readfirstlane(ent_sh);
+		const unsigned e = __builtin_amdgcn_readfirstlane(lo);
 		const PassDesc pd = ents[e].pd;
 		T *st = pool + ents[e].off;
 		const PassOp *eops = ops + ents[e].op_begin;
@@ -705,7 +712,7 @@ __global__ __launch_bounds__(THREADS) void k_pass_round(
 			tile_fetch(st, base, hb, lane, w, vr, vi);
 		}
 		tile_run<T>(st, pd, eops, nullptr, tile, base, hb, lane, w, vr, vi,
-			    lds, red);
+			    lds, nullptr);
 		__syncthreads();
 	}
 }
