@@ -3,10 +3,11 @@
 // An LDS tile holds 64 groups of D = 2^k amplitudes.  The tile is the
 // complex matrix Psi (D rows x 64 columns); the gate computes O = M * Psi,
 // with M the row-major gate (row = output basis state).  Complex products use
-// the exact four-real-product form
+// the four-real-product form
 //     Re O = Re M Re Psi - Im M Im Psi,   Im O = Re M Im Psi + Im M Re Psi,
 // each a sequence of 16x16x4 MFMAs accumulating in the amplitude's precision
-// (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32).
+// (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32); the streaming
+// form for k = 6 uses three real products instead (see k_dense_stream).
 //
 // A block owns one tile of 64 * CW groups (64 * CW * D amplitudes: the
 // gate's targets plus the 6 + log2(CW) lowest other index bits).  Its waves
@@ -257,7 +258,7 @@ __device__ __forceinline__ void stv(float2 *p, float2 v)
 	__builtin_nontemporal_store(x, (fv2 *)p);
 }
 
-template <typename T, int K, int CT>
+template <typename T, int K, int CT, bool G3, int RH>
 __global__ __launch_bounds__(SW * 64) void k_dense_stream(
 	const T *in, T *out, const double *__restrict__ nrm,
 	const T *__restrict__ M, StreamArgs sa, double *__restrict__ partials,
@@ -330,6 +331,71 @@ __global__ __launch_bounds__(SW * 64) void k_dense_stream(
 				for (int c = 0; c < CT; ++c)
 					bn[c][ks] = ldv(in + (nb | col[c] | roff[ks * 4 + flk]));
 		}
+		if constexpr (G3) {
+			// three real products per complex one (Gauss):
+			//   P = Re A Re B, Q = Im A Im B, S = (Re A + Im A)(Re B + Im B),
+			//   Re O = P - Q,  Im O = S - P - Q;
+			// RH row tiles at a time keep the three accumulator sets in
+			// the register budget of the four-product form (a rolled
+			// loop: unrolled, the sets of consecutive row tiles overlap)
+#pragma nounroll
+			for (int rh = 0; rh < RT; rh += RH) {
+				acc_t P[CT][RH], Q[CT][RH], S[CT][RH];
+#pragma unroll
+				for (int c = 0; c < CT; ++c)
+#pragma unroll
+					for (int j = 0; j < RH; ++j) {
+						P[c][j] = acc_t{0, 0, 0, 0};
+						Q[c][j] = acc_t{0, 0, 0, 0};
+						S[c][j] = acc_t{0, 0, 0, 0};
+					}
+#pragma unroll
+				for (int ks = 0; ks < KS; ++ks) {
+					T a[RH];
+					R bs[CT];
+#pragma unroll
+					for (int j = 0; j < RH; ++j)
+						a[j] = afr[((rh + j) * KS + ks) * 64 + fl];
+#pragma unroll
+					for (int c = 0; c < CT; ++c) {
+						// per row tile pass: hoisted out of the rolled
+						// loop, the sums would hold KS more registers
+						bs[c] = bc[c][ks].x + bc[c][ks].y;
+						asm volatile("" : "+v"(bs[c]));
+					}
+#pragma unroll
+					for (int j = 0; j < RH; ++j)
+#pragma unroll
+						for (int c = 0; c < CT; ++c)
+							P[c][j] = MF::op(a[j].x, bc[c][ks].x, P[c][j]);
+#pragma unroll
+					for (int j = 0; j < RH; ++j)
+#pragma unroll
+						for (int c = 0; c < CT; ++c)
+							Q[c][j] = MF::op(a[j].y, bc[c][ks].y, Q[c][j]);
+#pragma unroll
+					for (int j = 0; j < RH; ++j)
+#pragma unroll
+						for (int c = 0; c < CT; ++c)
+							S[c][j] = MF::op(a[j].x + a[j].y, bs[c], S[c][j]);
+				}
+#pragma unroll
+				for (int j = 0; j < RH; ++j)
+#pragma unroll
+					for (int r = 0; r < 4; ++r) {
+						const u64 ro = roff[(rh + j) * 16 + MF::out_row(flk, r)];
+#pragma unroll
+						for (int c = 0; c < CT; ++c) {
+							const R re = P[c][j][r] - Q[c][j][r];
+							const R im = (S[c][j][r] - P[c][j][r]) - Q[c][j][r];
+							const T o = mk<T>(re * inv, im * inv);
+							stv(out + (base | col[c] | ro), o);
+							nacc += cabs2d(o);
+						}
+					}
+			}
+			continue;
+		}
 		acc_t cr[CT][RT], ci[CT][RT];
 #pragma unroll
 		for (int c = 0; c < CT; ++c)
@@ -389,13 +455,25 @@ template <typename T, int K> constexpr int stream_ct()
 	return sizeof(T) == 16 ? (K == 4 ? 2 : 1) : (K == 4 ? 4 : K == 5 ? 2 : 1);
 }
 
+// The three-product form where the matrix cores bound the kernel (k = 6;
+// measured at 30 qubits: complex128 8.02 -> 7.10 ms, complex64 4.54 ->
+// 3.87-4.02 ms; k = 5 moved by 1% or less, and keeps the four-product form,
+// whose sums match the tiled kernel's bit for bit), and the row tiles it
+// takes at a time.
+template <typename T, int K> constexpr bool stream_g3() { return K == 6; }
+template <typename T, int K> constexpr int stream_rh() { return 1; }
+
+template <typename T, int K>
+constexpr auto stream_kernel = k_dense_stream<T, K, stream_ct<T, K>(), stream_g3<T, K>(),
+					      stream_rh<T, K>()>;
+
 // Grid of the streaming form: as many blocks as are resident at once (the
 // blocks are persistent and take equal shares), fewer for small states.
 template <typename T, int K> unsigned stream_grid(u64 ncg)
 {
 	static int bpc = 0;
 	if (!bpc && hipOccupancyMaxActiveBlocksPerMultiprocessor(
-			    &bpc, k_dense_stream<T, K, stream_ct<T, K>()>, SW * 64, 0) !=
+			    &bpc, stream_kernel<T, K>, SW * 64, 0) !=
 			    hipSuccess)
 		bpc = 1;
 	const u64 want = (u64)std::max(1, g_ctx.num_cus) * (u64)std::max(1, bpc);
@@ -406,7 +484,7 @@ template <typename T, int K>
 void launch_stream(const T *src, T *dst, const double *scal, const T *M,
 		   const StreamArgs &sa, double *parts, u64 ncg, unsigned grid)
 {
-	hipLaunchKernelGGL((k_dense_stream<T, K, stream_ct<T, K>()>), dim3(grid),
+	hipLaunchKernelGGL((stream_kernel<T, K>), dim3(grid),
 			   dim3(SW * 64), 0, DK_STREAM, src, dst, scal, M, sa, parts, ncg);
 }
 
@@ -552,12 +630,25 @@ int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 			  double *scratch, u64 scratch_cap)
 {
 	// the streaming form where it measured faster (30 qubits, gfx950):
-	// complex128 k = 5 6.79 against 8.04 ms, k = 6 8.16 against 11.28 ms
-	// (67 TF/s); the tiles stay ahead for complex128 k = 4 (6.23 against
-	// 6.72 ms) and for complex64 (k = 4 / 5 / 6: 3.3 / 4.2 / 6.4 against
-	// 8.0 / 6.9 / 7.2 ms)
-	const bool stream = g_ctx.dense_stream == 2 ||
-			    (g_ctx.dense_stream == 1 && in->dtype == DK_C128 && g->k >= 5);
+	// complex128 k = 5 6.3-7.6 against 7.8-8.2 ms, k = 6 7.1 against 11.3
+	// ms (77 TF/s), whatever the targets; the tiles stay ahead for k = 4.
+	// complex64 streams well only while the 16 groups a wave reads are
+	// long runs: with no target among qubits 0..2, k = 5 takes 3.2-3.4
+	// against 4.1 ms and k = 6 3.9-4.0 against 6.35 ms (137-142 TF/s).
+	// With low targets it falls to 5-13 ms (a wave's 8-byte reads touch a
+	// quarter of each line) and the tiles run.
+	bool stream = false;
+	if (g_ctx.dense_stream == 2) {
+		stream = true;
+	} else if (g_ctx.dense_stream == 1 && g->k >= 5) {
+		stream = in->dtype == DK_C128;
+		if (!stream) {
+			stream = true;
+			for (unsigned b = 0; b < g->k; ++b)
+				if (targets[b] < 3)
+					stream = false;
+		}
+	}
 	if (stream)
 		return launch_dense_stream(in, out, g, targets, parts, nparts, scratch,
 					   scratch_cap);

@@ -507,7 +507,8 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
  * (dense k >= 4 gates on MFMA, else on the vector ALUs), "apply1_group"
  * (1, 2 or 4 pair groups per block of the 1-qubit gate kernels),
  * "dense_stream" (dense gates on MFMA: 0 LDS tiles, 1 the faster form per
- * amplitude type and k (default), 2 the streaming form), "dense_k4_cw"
+ * amplitude type, k and low target qubits (default), 2 the streaming
+ * form), "dense_k4_cw"
  * (1, 2 or 4 sets of 64 amplitude groups per dense 4-qubit block, 0 =
  * by amplitude type), "arena_max_chunks"
  * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the
