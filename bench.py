@@ -95,10 +95,10 @@ def gpu_leg(args):
     by_target = {k: round(bpg / (sum(v) / len(v) * 1e-3) / 1e9)
                  for k, v in sorted(by_target.items(), key=lambda kv: (kv[0][:2], int(kv[0][2:])))}
 
-    # per-target kernel durations: five back-to-back gates per target
+    # per-target kernel durations: five back-to-back gates on every target
     per_target = {}
     u3 = layers[-1][0][0]
-    for t in sorted({0, 3, 6, 12, n // 2, n - 10, n - 5, n - 1}):
+    for t in range(n):
         for _ in range(5):
             reg = d.registry_apply(reg, u3, [t], set(), None, -1, False)
         d.registry_sync(reg, False)
@@ -675,11 +675,113 @@ def cpu_leg(args):
             "c0_12q_200_gates": c0, "batch_subset": batch}
 
 
+def traffic_child(n):
+    """Workload of the counter passes (run as a child under rocprofv3): a
+    clone of an n-qubit complex128 register (the calibration kernel: it
+    reads and writes exactly 16 B per amplitude) and one U3 on every target,
+    in place (every form of the one-qubit kernel)."""
+    import numpy as np
+    import doki_amd
+    from doki_amd import workloads as W
+    d = doki_amd.doki
+    d.execution_mode("gates", False)
+    reg = d.registry_new(n, False)
+    rng = np.random.default_rng(7)
+    for t in range(n):
+        g = d.gate_new(1, W.gate_u3(*rng.uniform(0, 2 * np.pi, 3)).tolist(), False)
+        reg = d.registry_apply(reg, g, [t], set(), None, -1, False)
+    d.registry_sync(reg, False)
+    c = d.registry_clone(reg, -1, False)
+    d.registry_sync(c, False)
+    print("traffic child done", flush=True)
+
+
+def kernel_short_name(full):
+    """'void (anonymous namespace)::k_x<HIP_vector_type<double, 2u>, 1>(...)'
+    -> 'k_x<double2, 1>'."""
+    s = full.replace("(anonymous namespace)::", "")
+    s = s.replace("HIP_vector_type<double, 2u>", "double2")
+    s = s.replace("HIP_vector_type<float, 2u>", "float2")
+    if s.startswith("void "):
+        s = s[5:]
+    return s.split("(")[0].strip()
+
+
+def _pmc_pass(counter, n, tmp):
+    """Per-dispatch values of one counter over the traffic child's kernels:
+    {kernel name: [KiB per dispatch, ...]}, or None."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    out = os.path.join(tmp, counter.lower())
+    cmd = ["timeout", "-s", "KILL", "120", exe, "--pmc", counter,
+           "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-child",
+           "--qubits", str(n)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if r.returncode != 0 or not files:
+        log(f"[bench] {counter} pass failed (rc {r.returncode}): {r.stderr[-400:]}")
+        return None
+    vals = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = kernel_short_name(row["Kernel_Name"])
+            vals.setdefault(name, []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def live_traffic(n):
+    """HBM bytes per launch of the one-qubit gate kernels, measured by this
+    run: two rocprofv3 passes (FETCH_SIZE, then WRITE_SIZE, in KiB per
+    dispatch) over the traffic child, each scaled by what the counter reads
+    for the state copy kernel (16 * 2^n B read and written).  Returns
+    (bytes per launch averaged over every one-qubit dispatch, a summary
+    dict), or (None, reason)."""
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="dk_traffic_", dir="/tmp")
+    fetch = _pmc_pass("FETCH_SIZE", n, tmp)
+    write = _pmc_pass("WRITE_SIZE", n, tmp) if fetch else None
+    if not fetch or not write:
+        return None, "rocprofv3 counter passes unavailable"
+    copy = [k for k in fetch if k.startswith("k_normalised_copy")]
+    gates = [k for k in fetch if k.startswith("k_apply1_")]
+    if not copy or not gates:
+        return None, f"kernels not found in the counter passes: {sorted(fetch)}"
+    state_kib = 16 * (1 << n) / 1024
+    fscale = state_kib / fetch[copy[0]][0]
+    wscale = state_kib / write[copy[0]][0]
+    per = {}
+    tot, cnt = 0.0, 0
+    for k in gates:
+        fs, ws = fetch[k], write.get(k, [])
+        if len(fs) != len(ws):
+            return None, f"dispatch counts differ for {k}"
+        b = [(f * fscale + w * wscale) * 1024 for f, w in zip(fs, ws)]
+        per[k] = {"dispatches": len(b), "bytes_per_launch": round(sum(b) / len(b)),
+                  "over_algorithmic": round(sum(b) / len(b) / (32 * (1 << n)), 4)}
+        tot += sum(b)
+        cnt += len(b)
+    return round(tot / cnt), {
+        "source": "measured by this run: rocprofv3 --pmc FETCH_SIZE, then "
+                  "WRITE_SIZE, over a clone and one U3 per target at "
+                  f"{n} qubits (KiB per dispatch, scaled by the copy kernel: "
+                  f"fetch x{fscale:.4f}, write x{wscale:.4f})",
+        "by_kernel": per}
+
+
 def measured_traffic(n):
-    """HBM bytes per launch of the 1-qubit gate kernel, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (copy-kernel
-    calibrated), and where they come from (file and the commit measured);
-    (None, None) when no profile covers this register size."""
+    """HBM bytes per launch of the 1-qubit gate kernel from the committed
+    rocprofv3 summary in profiles/ (copy-kernel calibrated) and where they
+    come from; (None, None) when no profile covers this register size.  The
+    fallback when this run cannot take its own counter passes."""
     name = f"apply1_{n}q_c128_hbm_traffic.txt"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -690,7 +792,7 @@ def measured_traffic(n):
             if ":" in line and not line.startswith("#"):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
-    src = f"profiles/{name}, measured at commit {vals.get('measured_at_commit', 'unknown')}"
+    src = f"profiles/{name} (the committed profile, not measured by this run)"
     return int(float(vals["traffic_bytes_per_launch"])), src
 
 
@@ -715,6 +817,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=60.0)
     ap.add_argument("--serve-workers", type=int, default=6)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="take roofline.traffic from the committed profile "
+                         "instead of two rocprofv3 counter passes")
+    ap.add_argument("--traffic-child", action="store_true",
+                    help=argparse.SUPPRESS)
     ap.add_argument("--serve-curve", type=float, nargs="*",
                     default=[30.0, 120.0, 240.0, 480.0],
                     help="further offered rates (requests/s) of the load curve")
@@ -726,6 +833,21 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
+    if args.traffic_child:
+        traffic_child(args.qubits)
+        return
+
+    # the counter passes run first, while this process has not touched the
+    # GPU (their counters would otherwise see its kernels too)
+    traffic, traffic_info = (None, None) if args.no_traffic else live_traffic(args.qubits)
+    if traffic is None:
+        if traffic_info:
+            log(f"[bench] live traffic: {traffic_info}")
+        traffic, traffic_src = measured_traffic(args.qubits)
+        traffic_info = {"source": traffic_src}
+    else:
+        log(f"[bench] live traffic {traffic} B per launch "
+            f"({traffic / (32 * (1 << args.qubits)):.4f} x algorithmic)")
 
     g = gpu_leg(args)
     big = big_leg(args) if args.big_qubits else None
@@ -738,7 +860,6 @@ def main():
     if cpu:
         log(f"[bench] cpu baseline {cpu}")
     value = g["ngates"] * g["bpg"] / g["elapsed"] / 1e9
-    traffic, traffic_src = measured_traffic(args.qubits)
     achieved = g["bpg"] / (g["kavg_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -770,7 +891,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": traffic_src,
+            "traffic_source": traffic_info,
         },
         "cpu_baseline": cpu,
         "extras": {

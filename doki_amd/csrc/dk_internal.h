@@ -123,6 +123,7 @@ struct dk_ctx {
 	int dense_k4_cw = 0;    // column sets per dense k = 4 block (0 = auto)
 	int dense_stream = 1;   // dense form: 0 tiles, 1 by type and k, 2 streaming
 	u64 arena_max_chunks = 0;     // cap on 2 GiB state-arena chunks
+	u64 arena_sub_bytes = 0;  // physical pieces of an arena chunk (0: one piece)
 	u64 arena_default_chunks = 0; // the cap at init: free memory less 6 GiB
 	// device address of the file-scope scratch array of dk_runtime.hip
 	// (block partials of dk_apply_noalloc), taken once at init with the

@@ -44,7 +44,15 @@ enum : u64 {
 static_assert(dkb::PIECE == DK_ARENA_MIN, "arena pieces start at DK_ARENA_MIN");
 using dkb::order_of;
 
-typedef hipMemGenericAllocationHandle_t Handle;
+// A chunk is CHUNK bytes of physical memory made of `subs.size()` equal
+// handles.  They are mapped in a scrambled order (virtual piece j onto handle
+// (j * SCRAMBLE) mod n), so that amplitudes a power-of-two distance of one
+// piece or more apart do not sit at the same power-of-two distance in
+// physical memory.
+struct Handle {
+	std::vector<hipMemGenericAllocationHandle_t> subs;
+};
+constexpr unsigned SCRAMBLE = 77;  // odd: a bijection mod any power of two
 
 struct Cached {
 	void *p;
@@ -77,6 +85,57 @@ hipMemAllocationProp chunk_prop()
 	prop.location.type = hipMemLocationTypeDevice;
 	prop.location.id = g_ctx.device;
 	return prop;
+}
+
+void chunk_release(Handle &h)
+{
+	for (auto s : h.subs)
+		(void)hipMemRelease(s);
+	h.subs.clear();
+}
+
+hipError_t chunk_create(Handle *out)
+{
+	const hipMemAllocationProp prop = chunk_prop();
+	u64 sub = g_ctx.arena_sub_bytes;
+	if (sub == 0 || sub > CHUNK || CHUNK % sub)
+		sub = CHUNK;
+	Handle h;
+	for (u64 off = 0; off < CHUNK; off += sub) {
+		hipMemGenericAllocationHandle_t s;
+		hipError_t e = hipMemCreate(&s, sub, &prop, 0);
+		if (e != hipSuccess) {
+			chunk_release(h);
+			return e;
+		}
+		h.subs.push_back(s);
+	}
+	*out = std::move(h);
+	return hipSuccess;
+}
+
+// Maps a chunk at va (CHUNK bytes of reserved addresses); on failure nothing
+// stays mapped.
+hipError_t chunk_map(void *va, const Handle &h)
+{
+	const u64 n = h.subs.size(), sub = CHUNK / n;
+	for (u64 j = 0; j < n; ++j) {
+		hipError_t e = hipMemMap((char *)va + j * sub, sub, 0,
+					 h.subs[(j * SCRAMBLE) & (n - 1)], 0);
+		if (e != hipSuccess) {
+			for (u64 i = 0; i < j; ++i)
+				(void)hipMemUnmap((char *)va + i * sub, sub);
+			return e;
+		}
+	}
+	return hipSuccess;
+}
+
+void chunk_unmap(void *va, const Handle &h)
+{
+	const u64 n = h.subs.size(), sub = CHUNK / n;
+	for (u64 j = 0; j < n; ++j)
+		(void)hipMemUnmap((char *)va + j * sub, sub);
 }
 
 Cached released_here(void *p)
@@ -137,9 +196,9 @@ bool unsplit_free_chunks()
 {
 	return g_buddy.drain_whole([](u64 base) {
 		auto sp = g_split.find(base);
-		(void)hipMemUnmap((void *)base, CHUNK);
+		chunk_unmap((void *)base, sp->second.h);
 		(void)hipMemAddressFree(sp->second.resv, 2 * CHUNK);
-		g_free_chunks.push_back(sp->second.h);
+		g_free_chunks.push_back(std::move(sp->second.h));
 		g_split.erase(sp);
 	});
 }
@@ -151,10 +210,11 @@ void unmap_whole(void *va)
 	auto it = g_mapped.find(va);
 	if (it == g_mapped.end())
 		return;
-	(void)hipMemUnmap(va, it->second.bytes);
+	for (size_t i = 0; i < it->second.chunks.size(); ++i)
+		chunk_unmap((char *)va + i * CHUNK, it->second.chunks[i]);
 	(void)hipMemAddressFree(va, it->second.bytes);
-	for (auto h : it->second.chunks)
-		g_free_chunks.push_back(h);
+	for (auto &h : it->second.chunks)
+		g_free_chunks.push_back(std::move(h));
 	g_mapped.erase(it);
 }
 
@@ -175,16 +235,15 @@ bool harvest_whole()
 // then chunks of cached blocks and pieces.
 int gather_chunks(size_t need, u64 bytes)
 {
-	const hipMemAllocationProp prop = chunk_prop();
 	bool trimmed = false;
 	while (g_free_chunks.size() < need) {
 		const bool capped = g_chunks_total >= g_ctx.arena_max_chunks;
 		hipError_t e = hipErrorOutOfMemory;
 		if (!capped) {
 			Handle h;
-			e = hipMemCreate(&h, CHUNK, &prop, 0);
+			e = chunk_create(&h);
 			if (e == hipSuccess) {
-				g_free_chunks.push_back(h);
+				g_free_chunks.push_back(std::move(h));
 				++g_chunks_total;
 				continue;
 			}
@@ -230,11 +289,10 @@ int alloc_whole(u64 bytes, void **out)
 	Mapped m;
 	m.bytes = bytes;
 	for (size_t i = 0; i < need && e == hipSuccess; ++i) {
-		Handle h = g_free_chunks.back();
-		e = hipMemMap((char *)va + i * CHUNK, CHUNK, 0, h, 0);
+		e = chunk_map((char *)va + i * CHUNK, g_free_chunks.back());
 		if (e == hipSuccess) {
+			m.chunks.push_back(std::move(g_free_chunks.back()));
 			g_free_chunks.pop_back();
-			m.chunks.push_back(h);
 		}
 	}
 	if (e == hipSuccess) {
@@ -275,8 +333,7 @@ int alloc_piece(u64 bytes, void **out)
 		if (e != hipSuccess)
 			return oom(bytes, e);
 		void *va = (void *)(((u64)resv + CHUNK - 1) & ~(CHUNK - 1));
-		Handle h = g_free_chunks.back();
-		e = hipMemMap(va, CHUNK, 0, h, 0);
+		e = chunk_map(va, g_free_chunks.back());
 		if (e == hipSuccess) {
 			const hipMemAllocationProp prop = chunk_prop();
 			hipMemAccessDesc ad = {};
@@ -284,14 +341,14 @@ int alloc_piece(u64 bytes, void **out)
 			ad.flags = hipMemAccessFlagsProtReadWrite;
 			e = hipMemSetAccess(va, CHUNK, &ad, 1);
 			if (e != hipSuccess)
-				(void)hipMemUnmap(va, CHUNK);
+				chunk_unmap(va, g_free_chunks.back());
 		}
 		if (e != hipSuccess) {
 			(void)hipMemAddressFree(resv, 2 * CHUNK);
 			return oom(bytes, e);
 		}
+		g_split.emplace((u64)va, Split{std::move(g_free_chunks.back()), resv});
 		g_free_chunks.pop_back();
-		g_split.emplace((u64)va, Split{h, resv});
 		g_buddy.release((u64)va, dkb::ORDERS - 1);
 		g_buddy.take(o, &addr);
 	}
@@ -312,7 +369,7 @@ hipError_t relieve_pool(F retry)
 				break;
 			continue;
 		}
-		(void)hipMemRelease(g_free_chunks.back());
+		chunk_release(g_free_chunks.back());
 		g_free_chunks.pop_back();
 		--g_chunks_total;
 		e = retry();
@@ -381,8 +438,8 @@ extern "C" int dk_memory_trim(void)
 	}
 	flush_pieces();
 	unsplit_free_chunks();
-	for (auto h : g_free_chunks)
-		(void)hipMemRelease(h);
+	for (auto &h : g_free_chunks)
+		chunk_release(h);
 	g_chunks_total -= g_free_chunks.size();
 	g_free_chunks.clear();
 	hipMemPool_t pool;

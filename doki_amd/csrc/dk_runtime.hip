@@ -1196,6 +1196,14 @@ extern "C" int dk_tune(const char *key, int64_t value)
 		g_ctx.dense_k4_cw = (int)value;
 		return DK_OK;
 	}
+	if (!strcmp(key, "arena_sub_mib")) {
+		if (value < 0 || value > 2048 || (value & (value - 1))) {
+			dk_set_error("arena_sub_mib is 0 (whole chunks) or a power of two up to 2048");
+			return DK_E_ARG;
+		}
+		g_ctx.arena_sub_bytes = (u64)value << 20;
+		return DK_OK;
+	}
 	if (!strcmp(key, "arena_max_chunks")) {
 		g_ctx.arena_max_chunks = value > 0 ? (u64)value
 						   : g_ctx.arena_default_chunks;

@@ -110,3 +110,39 @@ def test_pieces_and_chunk_recycling(capi):
     finally:
         del held
         capi.tune("arena_max_chunks", 0)
+
+
+def test_scrambled_chunk_pieces_are_invisible(capi):
+    """Chunks made of 16 MiB physical pieces mapped in a scrambled order
+    (dk_tune "arena_sub_mib") hold states exactly as whole chunks do: the
+    same gates on a whole-chunk state (27 qubits) and on buddy pieces (25
+    qubits; four fill a chunk) give bit-identical amplitudes on either kind
+    of chunk.  The scrambled states come first, after a trim, so that they
+    take newly created chunks."""
+    rng = np.random.default_rng(11)
+    gates = [(capi.Gate(W.gate_u3(*rng.uniform(0, 2 * np.pi, 3))), int(t))
+             for t in rng.permutation(27)]
+
+    def run(n):
+        st = capi.new(n)
+        for g, t in gates:
+            if t < n:
+                capi.apply(st, g, [t], in_place=True)
+        return st
+
+    capi.synchronize()
+    capi.memory_trim()
+    try:
+        capi.tune("arena_sub_mib", 16)
+        scrambled = [run(n) for n in (27, 25, 25, 25, 25)]
+    finally:
+        capi.tune("arena_sub_mib", 0)
+    plain = [run(n) for n in (27, 25)]
+    for st in scrambled:
+        ref = plain[0] if st.nq == 27 else plain[1]
+        inner, mx = capi.compare(ref, st)
+        assert mx == 0.0, (st.nq, mx)
+        assert abs(inner - 1.0) <= 1e-12
+    del plain, scrambled, st, ref
+    capi.synchronize()
+    capi.memory_trim()
