@@ -38,6 +38,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class phase:
+    """Logs the host clock (CLOCK_MONOTONIC and CLOCK_BOOTTIME, ns) at the
+    start and end of a bench phase, so that a kernel trace of the run can be
+    cut into the phases."""
+
+    def __init__(self, name):
+        self.name = name
+
+    @staticmethod
+    def now():
+        return time.clock_gettime_ns(time.CLOCK_MONOTONIC), \
+            time.clock_gettime_ns(time.CLOCK_BOOTTIME)
+
+    def __enter__(self):
+        self.t0 = self.now()
+
+    def __exit__(self, *exc):
+        t1 = self.now()
+        log(f"[bench] phase {self.name} monotonic_ns {self.t0[0]} {t1[0]} "
+            f"boottime_ns {self.t0[1]} {t1[1]}")
+
+
 def run_layer(d, reg, layer):
     for g, t, c in layer:
         reg = d.registry_apply(reg, g, t, c, None, -1, False)
@@ -512,9 +534,11 @@ def serve_leg(args):
     log(f"[bench] serve: {out['served']}")
     plain_res, out["plain"] = plain_run(1.0)
     log(f"[bench] serve plain: {out['plain']}")
-    _, out["served_burst"] = served(0.0)
+    with phase("served_burst"):
+        _, out["served_burst"] = served(0.0)
     log(f"[bench] serve burst: {out['served_burst']}")
-    _, out["plain_burst"] = plain_run(0.0)
+    with phase("plain_burst"):
+        _, out["plain_burst"] = plain_run(0.0)
     log(f"[bench] serve plain burst: {out['plain_burst']}")
     # load curve: the same trace offered at other rates (arrival times
     # scaled), up to saturation; plain beyond its capacity only queues
@@ -704,7 +728,7 @@ def kernel_short_name(full):
     s = s.replace("HIP_vector_type<float, 2u>", "float2")
     if s.startswith("void "):
         s = s[5:]
-    return s.split("(")[0].strip()
+    return s.split("(")[0].strip().replace(" >", ">")
 
 
 def _pmc_pass(counter, n, tmp):

@@ -512,7 +512,9 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
  * (1, 2 or 4 sets of 64 amplitude groups per dense 4-qubit block, 0 =
  * by amplitude type), "arena_max_chunks"
  * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the
- * free memory at init less 6 GiB). */
+ * free memory at init less 6 GiB), "arena_sub_mib" (chunks created from
+ * then on are made of physical pieces of this many MiB, a power of two,
+ * mapped in a scrambled order; 0, the default, one piece per chunk). */
 int dk_tune(const char *key, int64_t value);
 /* Compute units and architecture name of the library's device. */
 int dk_device_info(int *num_cus, char *arch, size_t arch_len);
