@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -374,34 +375,46 @@ void batcher(dk_server *S)
 		int rc = brc;
 		if (!rc)
 			rc = dk_batch_clear(b);
-		for (SReq *q : items) {
+		// a register the engine refuses fails its own request only (the
+		// batch is left as it was); idx: the register's place in the batch
+		std::vector<size_t> idx(items.size(), SIZE_MAX);
+		size_t added = 0;
+		for (size_t i = 0; i < items.size() && !rc; ++i) {
+			SReq *q = items[i];
 			q->t_start = t0;
-			if (rc)
-				break;
 			const SReg &r = q->regs[0];
-			rc = dk_batch_add(b, r.nq, r.tg.size(), r.mats.data(), r.tg.data(),
-					  r.cm.data(), r.am.data(),
-					  q->op == DK_OP_MEASURE ? q->rolls.data() : nullptr,
-					  q->op == DK_OP_DENSITY ? q->keep_mask : 0, nullptr);
+			const int ra = dk_batch_add(b, r.nq, r.tg.size(), r.mats.data(),
+						    r.tg.data(), r.cm.data(), r.am.data(),
+						    q->op == DK_OP_MEASURE ? q->rolls.data() : nullptr,
+						    q->op == DK_OP_DENSITY ? q->keep_mask : 0, &idx[i]);
+			if (ra) {
+				q->rc = ra;
+				q->err = dk_last_error();
+				idx[i] = SIZE_MAX;
+			} else {
+				++added;
+			}
 		}
-		if (!rc)
+		if (!rc && added)
 			rc = dk_batch_run(b);
 		std::string err = rc ? dk_last_error() : "";
 		for (size_t i = 0; i < items.size() && !rc; ++i) {
 			SReq *q = items[i];
+			if (idx[i] == SIZE_MAX)
+				continue;
 			const unsigned n = q->regs[0].nq;
 			int r2 = DK_OK;
 			if (q->op == DK_OP_PROB) {
 				q->probs.resize(n);
-				r2 = dk_batch_probs(b, i, q->probs.data());
+				r2 = dk_batch_probs(b, idx[i], q->probs.data());
 			} else if (q->op == DK_OP_MEASURE) {
 				q->outcomes.resize(n);
 				q->p1.resize(n);
-				r2 = dk_batch_outcomes(b, i, q->outcomes.data(), q->p1.data());
+				r2 = dk_batch_outcomes(b, idx[i], q->outcomes.data(), q->p1.data());
 			} else {
 				const unsigned a = (unsigned)__builtin_popcountll(q->keep_mask);
 				q->rho.resize((size_t)2 << (2 * a));
-				r2 = dk_batch_density(b, i, q->rho.data());
+				r2 = dk_batch_density(b, idx[i], q->rho.data());
 			}
 			if (r2) {
 				q->rc = r2;
@@ -409,9 +422,9 @@ void batcher(dk_server *S)
 			}
 		}
 		std::lock_guard<std::mutex> lk(S->mu);
-		if (!rc) {
+		if (!rc && added) {
 			S->batches++;
-			S->batched += items.size();
+			S->batched += added;
 		}
 		for (SReq *q : items) {
 			release(S, q);
@@ -522,15 +535,26 @@ extern "C" int dk_submit_circuit(dk_server *S, const dk_request *r, uint64_t *ti
 			q->need += state_bytes(g.nq, S->dtype);
 		}
 		nq[i] = g.nq;
-		for (unsigned t : g.tg)
-			if (!rc && t >= g.nq)
+		// every gate as the batched engine and the circuits take it, so
+		// that an admitted request cannot fail another one's launch
+		const u64 valid = g.nq >= 64 ? ~0ull : (1ull << g.nq) - 1;
+		for (size_t k = 0; k < g.tg.size() && !rc; ++k) {
+			const unsigned t = g.tg[k];
+			const u64 cm = g.cm[k], am = g.am[k];
+			if (t >= g.nq || ((cm | am) & ~valid) || (cm & am) ||
+			    (((cm | am) >> t) & 1))
 				rc = DK_E_ARG;
+		}
 	}
 	if (!rc && q->op == DK_OP_MEASURE) {
-		if (!r->rolls)
+		if (!r->rolls) {
 			rc = DK_E_ROLL;
-		else
+		} else {
 			q->rolls.assign(r->rolls, r->rolls + nq[0]);
+			for (double v : q->rolls)
+				if (v < 0.0 || v >= 1.0)  // NaN passes, as the reference takes it
+					rc = DK_E_ROLL;
+		}
 		q->need += state_bytes(nq[0], S->dtype); // the collapse chain
 	}
 	if (!rc && q->op == DK_OP_DENSITY &&

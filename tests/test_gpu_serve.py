@@ -342,3 +342,42 @@ def test_server_requests_through_the_c_abi(capi):
         stats = [C.c_uint64() for _ in range(4)]
         capi.check(L.dk_server_stats(srv.ptr, *[C.byref(x) for x in stats]))
         assert stats[2].value >= 1 and stats[3].value == 1
+
+
+def test_server_refuses_malformed_requests(capi):
+    """A request whose gates or rolls the engines would refuse is refused
+    at submission (DK_E_ARG, DK_E_ROLL), so it can never fail the batched
+    launch it would have shared; the well-formed requests around it give
+    the results they give alone."""
+    from doki_amd import serve
+    good = [dict(rid=i, arrival=0.0, op="prob",
+                 regs=[(9, W.random_circuit(100 + i, 9, 30)[0])]) for i in range(6)]
+    h_gate = W.gate_h()
+    bad = [
+        # control beyond the register
+        dict(rid=90, arrival=0.0, op="prob", regs=[(9, [(h_gate, [0], {11}, set())])]),
+        # control on the target itself
+        dict(rid=91, arrival=0.0, op="prob", regs=[(9, [(h_gate, [3], {3}, set())])]),
+        # a qubit both control and anticontrol
+        dict(rid=92, arrival=0.0, op="prob", regs=[(9, [(h_gate, [0], {2}, {2})])]),
+        # target beyond the register
+        dict(rid=93, arrival=0.0, op="prob", regs=[(9, [(h_gate, [9], set(), set())])]),
+    ]
+    roll_bad = dict(rid=94, arrival=0.0, op="measure",
+                    regs=[(9, W.random_circuit(5, 9, 10)[0])], rolls=[0.5] * 8 + [1.5])
+    with serve.Server(workers=2) as srv:
+        alone = [srv.submit(s).wait(60)["probs"] for s in good]
+    with serve.Server(workers=2) as srv:
+        hs = []
+        for i, s in enumerate(good):
+            hs.append(srv.submit(s))
+            if i < len(bad):
+                with pytest.raises(capi.DokiHipError) as e:
+                    srv.submit(bad[i])
+                assert e.value.code == 5
+        with pytest.raises(capi.DokiHipError) as e:
+            srv.submit(roll_bad)
+        assert e.value.code == 6
+        mixed = [h.wait(60)["probs"] for h in hs]
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)
