@@ -170,11 +170,31 @@ static PyObject *reg_detach(reg_t *r)
 	return p;
 }
 
+/* Drops a reference to registry capsule p and, while that was the last
+ * one, to its pending ancestors in turn: each is detached from its parent
+ * before it is destroyed, so a long chain of pending registries goes
+ * without recursion. */
+static void reg_release_chain(PyObject *p)
+{
+	while (p) {
+		PyObject *pp = NULL;
+		if (Py_REFCNT(p) == 1) {
+			reg_t *pr = reg_of(p);
+			if (pr)
+				pp = reg_detach(pr);
+			else
+				PyErr_Clear();
+		}
+		Py_DECREF(p);
+		p = pp;
+	}
+}
+
 static void reg_free(reg_t *r)
 {
 	if (!r)
 		return;
-	Py_XDECREF(reg_detach(r));
+	reg_release_chain(reg_detach(r));
 	if (r->st)
 		dk_state_free(r->st);
 	PyMem_Free(r->cache);
@@ -494,7 +514,7 @@ static PyObject *doki_registry_del(PyObject *self, PyObject *args)
 			if (r->st)
 				dk_state_free(r->st);
 			r->st = NULL;
-			Py_XDECREF(reg_detach(r));
+			reg_release_chain(reg_detach(r));
 		}
 		PyMem_Free(r->cache);
 		r->cache = NULL;

@@ -228,3 +228,39 @@ def test_execution_mode_errors(doki):
     with pytest.raises(Exception) as e:
         doki.registry_sync(None, False)
     assert "NULL pointer to registry" in str(e.value)
+
+
+def test_dropped_long_chain_releases_without_recursion(doki, capi):
+    """A chain of FUSE_MAX_PENDING - 1 pending registries that nobody reads
+    is released link by link when its last registry goes: in a thread with a
+    128 KiB stack (a recursive release would need a C frame chain per link),
+    and with the source's memory back afterwards."""
+    import threading
+    prev = doki.execution_mode("fuse", False)
+    try:
+        base = used_bytes(capi)
+        x = doki.gate_new(1, W.gate_x().tolist(), False)
+        out = {}
+
+        def work():
+            try:
+                reg = doki.registry_new(14, False)
+                for i in range(2047):
+                    reg = doki.registry_apply(reg, x, [i % 14], set(), set(), 1, False)
+                del reg
+                out["ok"] = True
+            except BaseException as e:       # reported below
+                out["err"] = e
+
+        old = threading.stack_size(128 * 1024)
+        try:
+            t = threading.Thread(target=work)
+            t.start()
+            t.join(120)
+        finally:
+            threading.stack_size(old)
+        assert not t.is_alive()
+        assert out.get("ok"), out.get("err")
+        assert used_bytes(capi) == base
+    finally:
+        doki.execution_mode(prev, False)
