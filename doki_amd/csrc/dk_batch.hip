@@ -12,7 +12,7 @@
 // consecutive gates whose targets fit one 2^LB-amplitude tile; the workgroup
 // streams the register through LDS tile by tile, applying the whole run to
 // each tile (controls outside the tile are decided per tile).  Registers of
-// at least "split_qubits" qubits (17 by default) would hold one workgroup
+// at least "split_qubits" qubits (14 by default) would hold one workgroup
 // for long (a 2^20-amplitude register streams at about one CU's share of
 // HBM), so their gates run before the persistent launch as rounds of
 // cache-blocked passes, planned per register as a circuit's passes are
@@ -634,7 +634,7 @@ struct dk_batch {
 	int use_graph = 0;
 	int pass_qubits = 0; // 0 = tile size
 	int probs = 0;       // marginal probabilities of every register
-	int split_qubits = 17; // pooled registers this large run their passes
+	int split_qubits = 14; // pooled registers this large run their passes
 			       // as grid-wide rounds (0 = never)
 	std::vector<BGate> orig;          // every register's gates as added
 	std::vector<size_t> orig_begin;   // first gate of register i in orig

@@ -585,7 +585,13 @@ static int launch_dense_stream(const dk_state *in, dk_state *out, const dk_gate 
 		ct = c128 ? stream_ct<double2, 5>() : stream_ct<float2, 5>();
 	else
 		ct = c128 ? stream_ct<double2, 6>() : stream_ct<float2, 6>();
-	const u64 ncg = (in->len >> (k + 4)) / (u64)ct;  // blocks of 16 ct groups
+	const u64 ngroups = in->len >> (k + 4);            // 16-group units
+	if (ngroups == 0 || ngroups % (u64)ct) {
+		dk_set_error("dense gate of %u qubits: state of %u qubits too small "
+			     "for the streaming form", k, in->nq);
+		return DK_E_ARG;
+	}
+	const u64 ncg = ngroups / (u64)ct;  // blocks of 16 ct groups
 	unsigned grid = 1;
 	if (k == 4)
 		grid = c128 ? stream_grid<double2, 4>(ncg) : stream_grid<float2, 4>(ncg);

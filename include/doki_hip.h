@@ -358,7 +358,8 @@ void dk_batch_free(dk_batch *b);
  * "probs" (every register reports its marginal qubit probabilities),
  * "split_qubits" (registers of at least this many qubits, and larger than
  * one LDS tile, run their passes as grid-wide rounds before the persistent
- * launch; 17 by default, 0 = never; results are the same either way). */
+ * launch; 14 by default, 0 = never; results agree to rounding either way,
+ * the rounds merging neighbouring gates). */
 int dk_batch_set(dk_batch *b, const char *key, int64_t value);
 
 /* Drops every register and keeps the device and pinned buffers for the next

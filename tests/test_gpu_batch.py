@@ -141,8 +141,9 @@ def test_batch_split_rounds_equal_single_workgroup(capi, oracle, dtype, split, p
 
 
 def test_batch_default_split_vs_oracle(capi, oracle):
-    """The default threshold (17 qubits) with the persistent registers beside
-    it, against the oracle."""
+    """The default threshold (14 qubits: the 16-, 17- and 18-qubit registers
+    run as rounds) with the persistent registers beside it, against the
+    oracle."""
     specs = [(17, *W.random_circuit(9801, 17, 60)), (18, *W.random_circuit(9802, 18, 45)),
              (12, *W.random_circuit(9803, 12, 30)), (16, *W.random_circuit(9804, 16, 30))]
     check_batch(capi, oracle, capi.C128, specs, ATOL)
