@@ -214,6 +214,7 @@ int dk_launch_prob_partials(const dk_state *s, unsigned q, int *nparts_out);
 
 // Scaled copy (and type conversion): dst = src * (1 / src.norm), norm 1.
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst);
+int dk_launch_values_copy(const dk_state *src, dk_state *dst);
 
 // Measurement step: decide outcome from device partials, write the norm of
 // the collapsed state; collapse (with fused partials for the next qubit).

@@ -1415,6 +1415,41 @@ int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
 	return DK_OK;
 }
 
+// out = in / norm per component (an IEEE division, as state_get divides,
+// qstate.h:86), into a complex128 state whose norm becomes 1.
+template <typename TI>
+__global__ __launch_bounds__(TPB) void k_values_copy(
+	const TI *__restrict__ in, const double *__restrict__ nrm,
+	double2 *__restrict__ out, double *out_scal, u64 len)
+{
+	const double n = nrm[DK_SCAL_NORM];
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < len;
+	     i += (u64)gridDim.x * TPB) {
+		const TI a = ld_stream(in + i);
+		double2 o;
+		o.x = (double)a.x / n;
+		o.y = (double)a.y / n;
+		st_stream(out + i, o);
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		out_scal[DK_SCAL_NORM] = 1.0;
+}
+
+int dk_launch_values_copy(const dk_state *src, dk_state *dst)
+{
+	const unsigned nb = grid_full(src->len, TPB);
+	if (src->dtype == DK_C128)
+		hipLaunchKernelGGL((k_values_copy<double2>), dim3(nb), dim3(TPB), 0,
+				   DK_STREAM, src->a128(), src->scal, dst->a128(),
+				   dst->scal, src->len);
+	else
+		hipLaunchKernelGGL((k_values_copy<float2>), dim3(nb), dim3(TPB), 0,
+				   DK_STREAM, src->a64(), src->scal, dst->a128(),
+				   dst->scal, src->len);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
 int dk_launch_normalised_copy(const dk_state *src, dk_state *dst)
 {
 	const unsigned nb = grid_full(src->len, TPB);

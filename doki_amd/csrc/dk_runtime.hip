@@ -402,6 +402,74 @@ extern "C" int dk_state_from_host(unsigned num_qubits, const double *values,
 	return DK_OK;
 }
 
+extern "C" int dk_state_from_device(unsigned num_qubits, const void *values,
+				    dk_state **out)
+{
+	DK_INIT();
+	if (!values || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	dk_state *s;
+	int rc = dk_alloc_state(num_qubits, DK_C128, &s);
+	if (rc)
+		return rc;
+	// the values may come from work on any stream of the device (another
+	// library's): wait for all of it, then copy on this lane
+	hipError_t e = hipDeviceSynchronize();
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(s->amp, values, s->len * sizeof(double2),
+				   hipMemcpyDeviceToDevice, DK_STREAM);
+	if (e != hipSuccess) {
+		dk_set_error("device copy failed: %s", hipGetErrorString(e));
+		dk_release_state(s);
+		return DK_E_DEVICE;
+	}
+	rc = dk_launch_set_norm(s, 1.0);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" int dk_state_values(const dk_state *src, dk_state **out)
+{
+	if (!src || !out) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	DK_SETTLE(src);
+	dk_state *s;
+	int rc = dk_alloc_state(src->nq, DK_C128, &s);
+	if (rc)
+		return rc;
+	rc = dk_launch_values_copy(src, s);
+	if (rc) {
+		dk_release_state(s);
+		return rc;
+	}
+	*out = s;
+	return DK_OK;
+}
+
+extern "C" int dk_state_device_address(const dk_state *s, void **amp, int *device)
+{
+	if (!s || !amp) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	DK_SETTLE(s);
+	// the work queued on the state completes before its address is handed
+	// to a caller that reads it on streams of its own
+	DK_HIP(hipStreamSynchronize(DK_STREAM));
+	*amp = s->amp;
+	if (device)
+		*device = g_ctx.device;
+	return DK_OK;
+}
+
 extern "C" int dk_state_clone(const dk_state *src, dk_state **out)
 {
 	if (!src) {
@@ -1211,6 +1279,17 @@ extern "C" int dk_tune(const char *key, int64_t value)
 	}
 	dk_set_error("unknown tunable %s", key);
 	return DK_E_ARG;
+}
+
+extern "C" int dk_device_ordinal(int *device)
+{
+	DK_INIT();
+	if (!device) {
+		dk_set_error("NULL argument");
+		return DK_E_ARG;
+	}
+	*device = g_ctx.device;
+	return DK_OK;
 }
 
 extern "C" int dk_device_info(int *num_cus, char *arch, size_t arch_len)

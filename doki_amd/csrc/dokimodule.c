@@ -615,6 +615,229 @@ static PyObject *doki_registry_new_data(PyObject *self, PyObject *args)
 	return reg_wrap(st);
 }
 
+/*
+ * DLPack interchange of registry values with other libraries on the device
+ * (PyTorch, CuPy, ...), without a host round trip.  The structures follow
+ * the public DLPack ABI (dlpack.h, version 0.8): a "dltensor" capsule holds
+ * a DLManagedTensor whose deleter the consumer calls when it is done; a
+ * consumer that takes the tensor renames the capsule "used_dltensor".
+ */
+typedef struct {
+	int32_t device_type;   /* DLDeviceType: 1 CPU, 3 CUDA host, 10 ROCm, 11 ROCm host */
+	int32_t device_id;
+} dl_device;
+typedef struct {
+	uint8_t code;          /* 2 float, 5 complex */
+	uint8_t bits;
+	uint16_t lanes;
+} dl_dtype;
+typedef struct {
+	void *data;
+	dl_device device;
+	int32_t ndim;
+	dl_dtype dtype;
+	int64_t *shape;
+	int64_t *strides;      /* in elements; NULL for a compact row-major tensor */
+	uint64_t byte_offset;
+} dl_tensor;
+typedef struct dl_managed {
+	dl_tensor dl;
+	void *manager_ctx;
+	void (*deleter)(struct dl_managed *self);
+} dl_managed;
+enum { DL_CPU = 1, DL_CUDA_HOST = 3, DL_ROCM = 10, DL_ROCM_HOST = 11 };
+enum { DL_FLOAT = 2, DL_COMPLEX = 5 };
+
+/* What a produced tensor owns: the state holding the values and the shape. */
+typedef struct {
+	dl_managed m;
+	dk_state *st;
+	int64_t shape[1];
+} dl_export;
+
+static void dl_export_delete(dl_managed *m)
+{
+	dl_export *e = (dl_export *)m;
+	if (e->st)
+		dk_state_free(e->st);
+	PyMem_RawFree(e);
+}
+
+static void dl_capsule_destroy(PyObject *cap)
+{
+	/* a capsule nobody consumed still owns its tensor */
+	if (PyCapsule_IsValid(cap, "dltensor")) {
+		dl_managed *m = (dl_managed *)PyCapsule_GetPointer(cap, "dltensor");
+		if (m && m->deleter)
+			m->deleter(m);
+	}
+}
+
+/* registry_to_dlpack(registry, verbose): a "dltensor" capsule with the
+ * registry's values (as registry_get returns them, bit for bit) as a 1-D
+ * complex128 tensor of 2^n elements in device memory, for
+ * torch.utils.dlpack.from_dlpack and the like.  The tensor holds a copy made
+ * on the device (registries are values: writes to the tensor do not change
+ * the registry). */
+static PyObject *doki_registry_to_dlpack(PyObject *self, PyObject *args)
+{
+	PyObject *cap, *out;
+	int verbose, rc, device = 0;
+	reg_t *r;
+	dl_export *e;
+	void *amp = NULL;
+	dk_state *vals = NULL;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &cap, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_to_dlpack(registry, verbose)");
+		return NULL;
+	}
+	r = reg_get(cap, "NULL pointer to registry");
+	if (!r)
+		return NULL;
+	rc = dk_state_values(r->st, &vals);
+	if (!rc)
+		rc = dk_state_device_address(vals, &amp, &device);
+	if (rc) {
+		if (vals)
+			dk_state_free(vals);
+		return dk_fail(rc, "Failed to export registry");
+	}
+	e = (dl_export *)PyMem_RawCalloc(1, sizeof(dl_export));
+	if (!e) {
+		dk_state_free(vals);
+		return PyErr_NoMemory();
+	}
+	e->st = vals;
+	e->shape[0] = (int64_t)1 << r->nq;
+	e->m.dl.data = amp;
+	e->m.dl.device.device_type = DL_ROCM;
+	e->m.dl.device.device_id = device;
+	e->m.dl.ndim = 1;
+	e->m.dl.dtype.code = DL_COMPLEX;
+	e->m.dl.dtype.bits = 128;
+	e->m.dl.dtype.lanes = 1;
+	e->m.dl.shape = e->shape;
+	e->m.dl.strides = NULL;
+	e->m.dl.byte_offset = 0;
+	e->m.manager_ctx = e;
+	e->m.deleter = dl_export_delete;
+	out = PyCapsule_New(&e->m, "dltensor", dl_capsule_destroy);
+	if (!out)
+		dl_export_delete(&e->m);
+	return out;
+}
+
+/* Whether t is a compact row-major tensor. */
+static int dl_compact(const dl_tensor *t)
+{
+	int64_t want = 1;
+	if (!t->strides)
+		return 1;
+	for (int32_t d = t->ndim - 1; d >= 0; d--) {
+		if (t->shape[d] != 1 && t->strides[d] != want)
+			return 0;
+		want *= t->shape[d];
+	}
+	return 1;
+}
+
+/* registry_from_dlpack(tensor, verbose): a registry holding the values of a
+ * DLPack tensor (a "dltensor" capsule, or an object with __dlpack__): 2^n
+ * complex128 elements (1-D), or 2^n x 2 float64 (re, im), compact, in
+ * device memory of this library's device or in host memory.  Values are
+ * taken as given, not normalised, as registry_new_data takes them
+ * (doki.c:625-720); the copy stays on the device for device tensors. */
+static PyObject *doki_registry_from_dlpack(PyObject *self, PyObject *args)
+{
+	PyObject *obj, *cap = NULL;
+	int verbose, rc, mydev = 0;
+	dl_managed *m;
+	const dl_tensor *t;
+	uint64_t len;
+	unsigned nq = 0;
+	dk_state *st = NULL;
+	const char *why = NULL;
+	(void)self;
+	if (!PyArg_ParseTuple(args, "Op", &obj, &verbose)) {
+		PyErr_SetString(DokiError, "Syntax: registry_from_dlpack(tensor, verbose)");
+		return NULL;
+	}
+	if (PyCapsule_CheckExact(obj)) {
+		cap = obj;
+		Py_INCREF(cap);
+	} else if (PyObject_HasAttrString(obj, "__dlpack__")) {
+		cap = PyObject_CallMethod(obj, "__dlpack__", NULL);
+		if (!cap)
+			return NULL;
+	} else {
+		PyErr_SetString(DokiError, "tensor has to be a DLPack capsule or "
+					   "provide __dlpack__");
+		return NULL;
+	}
+	if (!PyCapsule_IsValid(cap, "dltensor")) {
+		Py_DECREF(cap);
+		PyErr_SetString(DokiError, "not an unconsumed DLPack capsule");
+		return NULL;
+	}
+	m = (dl_managed *)PyCapsule_GetPointer(cap, "dltensor");
+	t = &m->dl;
+	len = 1;
+	for (int32_t d = 0; d < t->ndim; d++)
+		len *= (uint64_t)t->shape[d];
+	if (t->dtype.lanes != 1)
+		why = "vector element types are not supported";
+	else if (t->dtype.code == DL_COMPLEX && t->dtype.bits == 128 && t->ndim == 1)
+		;
+	else if (t->dtype.code == DL_FLOAT && t->dtype.bits == 64 && t->ndim == 2 &&
+		 t->shape[1] == 2)
+		len /= 2;
+	else
+		why = "values have to be complex128 (n) or float64 (n, 2)";
+	if (!why && !dl_compact(t))
+		why = "the tensor has to be contiguous";
+	if (!why) {
+		while (nq < 63 && ((uint64_t)1 << nq) < len)
+			nq++;
+		if (len < 2 || ((uint64_t)1 << nq) != len || nq > DK_MAX_QUBITS)
+			why = "the tensor's length has to be 2^n, 1 <= n <= 62";
+	}
+	if (!why) {
+		const char *data = (const char *)t->data + t->byte_offset;
+		if (t->device.device_type == DL_ROCM) {
+			rc = dk_device_ordinal(&mydev);
+			if (!rc && t->device.device_id != mydev)
+				why = "the tensor is on another device";
+			else if (!rc)
+				rc = dk_state_from_device(nq, data, &st);
+		} else if (t->device.device_type == DL_CPU ||
+			   t->device.device_type == DL_CUDA_HOST ||
+			   t->device.device_type == DL_ROCM_HOST) {
+			rc = dk_state_from_host(nq, (const double *)data, &st);
+		} else {
+			why = "the tensor has to be in host memory or on this device";
+		}
+		if (!why && rc) {
+			PyCapsule_SetName(cap, "used_dltensor");
+			if (m->deleter)
+				m->deleter(m);
+			Py_DECREF(cap);
+			return dk_fail(rc, "Unknown error when creating state");
+		}
+	}
+	if (why) {
+		Py_DECREF(cap);
+		PyErr_SetString(DokiError, why);
+		return NULL;
+	}
+	/* consumed: the values are copied, so the producer's tensor goes now */
+	PyCapsule_SetName(cap, "used_dltensor");
+	if (m->deleter)
+		m->deleter(m);
+	Py_DECREF(cap);
+	return reg_wrap(st);
+}
+
 static PyObject *doki_registry_get(PyObject *self, PyObject *args)
 {
 	PyObject *cap;
@@ -2582,6 +2805,10 @@ static PyMethodDef DokiMethods[] = {
 	{"registry_get", doki_registry_get, METH_VARARGS, "Get value from registry"},
 	{"registry_get_all", doki_registry_get_all, METH_VARARGS,
 	 "Get every value of a registry as a numpy array (one copy)"},
+	{"registry_to_dlpack", doki_registry_to_dlpack, METH_VARARGS,
+	 "The values of a registry as a DLPack tensor on the device (a copy)"},
+	{"registry_from_dlpack", doki_registry_from_dlpack, METH_VARARGS,
+	 "A registry holding the values of a DLPack tensor"},
 	{"registry_apply", doki_registry_apply, METH_VARARGS, "Apply a gate"},
 	{"registry_join", doki_registry_join, METH_VARARGS, "Merges two registries"},
 	{"registry_measure", doki_registry_measure, METH_VARARGS,

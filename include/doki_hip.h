@@ -113,6 +113,27 @@ int dk_state_from_host(unsigned num_qubits, const double *values,
  * qstate.c:90-105 (registry_clone, doki.c:343-398). */
 int dk_state_clone(const dk_state *src, dk_state **out);
 
+/* A complex128 state holding 2^n values that are already in device memory
+ * of the library's device (another library's tensor, for example), copied as
+ * given, norm 1 (as dk_state_from_host).  Waits for all work queued on the
+ * device first, whatever stream queued it.  Replaces registry_new_data,
+ * doki.c:625-720, for device-resident input (no host round trip). */
+int dk_state_from_device(unsigned num_qubits, const void *values,
+			 dk_state **out);
+
+/* A new complex128 state holding src's values (each raw component divided
+ * by the norm, as state_get reads them, qstate.h:86), norm 1: its raw
+ * amplitudes are the values, equal bit for bit to what registry_get
+ * returns.  For handing a state to another library.  No reference
+ * counterpart. */
+int dk_state_values(const dk_state *src, dk_state **out);
+
+/* Device address of s's raw amplitudes (2^n interleaved complex of s's
+ * type) and the HIP device ordinal holding them, once the work queued on s
+ * is complete (the calling thread waits).  Valid until s is freed.  No
+ * reference counterpart (the reference's states never leave the host). */
+int dk_state_device_address(const dk_state *s, void **amp, int *device);
+
 /* dk_state_new with a storage type (DK_C128 or DK_C64).  No reference
  * counterpart for DK_C64 (the reference fixes PRECISION 2, platform.h:199). */
 int dk_state_new_typed(unsigned num_qubits, int zero_state, int dtype,
@@ -519,6 +540,8 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
 int dk_tune(const char *key, int64_t value);
 /* Compute units and architecture name of the library's device. */
 int dk_device_info(int *num_cus, char *arch, size_t arch_len);
+/* The HIP device ordinal the library runs on (initialising it). */
+int dk_device_ordinal(int *device);
 
 #ifdef __cplusplus
 }
