@@ -954,6 +954,9 @@ static int batch_upload(dk_batch *b)
 			}
 			maxp = std::max(maxp, descs[i].size());
 		}
+		// index bits each register's state may have set so far: every
+		// register starts from |0...0>, and a pass widens it by its tile
+		std::vector<u64> support(b->regs.size(), 0);
 		for (size_t r = 0; r < maxp; ++r) {
 			b->round_off.push_back(b->rounds.size());
 			u64 items = 0;
@@ -966,6 +969,13 @@ static int batch_upload(dk_batch *b)
 				e.first = items;
 				e.off = R.off;
 				e.pd = descs[i][r];
+				// tiles outside the support hold zeros and keep them
+				const u64 valid = (1ull << R.nq) - 1;
+				e.pd.zmask = e.pd.finite ? (~support[i] & valid) : 0;
+				u64 tmask = (1ull << dkp::LANE_BITS) - 1;
+				for (int h = 0; h < dkp::HIGH_BITS; ++h)
+					tmask |= 1ull << e.pd.hpos[h];
+				support[i] |= tmask;
 				e.op_begin = obeg[i][r];
 				e.nq = R.nq;
 				b->rounds.push_back(e);

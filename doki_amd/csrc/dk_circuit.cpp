@@ -17,6 +17,7 @@
 // move ahead of gates left for later passes only when they commute with
 // them (checked on the qubits each gate uses non-diagonally).
 #include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstring>
 #include <vector>
@@ -842,6 +843,12 @@ void build_pass(dk_circuit *c, std::vector<unsigned> chosen,
 			o.tcm |= 1ull << o.a;
 	}
 	pp.desc.nops = (uint32_t)(c->pops.size() - pp.first_op);
+	pp.desc.finite = 1;
+	for (size_t i = pp.first_op; i < c->pops.size(); ++i)
+		for (int j = 0; j < 8; ++j)
+			if (!std::isfinite(c->pops[i].m[j]))
+				pp.desc.finite = 0;
+	pp.desc.zmask = 0;
 	c->passes.push_back(pp);
 	c->plan.push_back({true, (int)c->passes.size() - 1});
 }

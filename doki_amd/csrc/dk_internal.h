@@ -63,6 +63,10 @@ struct dk_state {
 	double *scal;   // device scalars
 	void *amp;      // device amplitudes (raw), double2 or float2
 	u64 bytes;      // bytes of the allocation
+	// index bits that may be set in a nonzero amplitude: every amplitude
+	// whose index has a bit outside `support` is zero (~0: no knowledge;
+	// a new |0...0> state has 0, gates and passes widen it)
+	u64 support;
 	double2 *a128() const { return (double2 *)amp; }
 	float2 *a64() const { return (float2 *)amp; }
 };

@@ -1218,6 +1218,11 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	}
 	if (rc)
 		return rc;
+	// the gate mixes amplitudes only along its targets
+	u64 tmask = 0;
+	for (unsigned b = 0; b < g->k; ++b)
+		tmask |= 1ull << targets[b];
+	out->support = in->support | tmask;
 	DK_TRY(dk_time_mark(1));
 	if (g->dmat)
 		DK_TRY(dk_gate_mark_use(g));

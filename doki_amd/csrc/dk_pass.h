@@ -70,6 +70,11 @@ struct PassDesc {
 	uint8_t hpos[HIGH_BITS];    // index bit of high slot bit h
 	uint8_t hsorted[HIGH_BITS]; // the same bits, ascending
 	uint32_t nops;
+	uint32_t finite;            // every op's matrix entries are finite
+	// index bits known to be zero in the input (set at launch): a tile
+	// whose fixed bits meet it holds only zeros, and so does its output,
+	// which is written without reading the tile or running the ops
+	uint64_t zmask;
 };
 
 // One register's pass within a round of batched passes: its tiles are the

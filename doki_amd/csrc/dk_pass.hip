@@ -466,6 +466,17 @@ __device__ __forceinline__ void tile_fetch(const T *in, u64 base,
 	}
 }
 
+// Zeros into a thread's 16 amplitudes of the tile at `base` (load mapping).
+template <typename T>
+__device__ __forceinline__ void tile_zero(T *out, u64 base, const u64 (&hb)[HIGH_BITS],
+					  unsigned lane, unsigned w)
+{
+	const u64 tb = base | lane | slot_index(w << REG_BITS, hb);
+#pragma unroll
+	for (int r = 0; r < REGS; ++r)
+		st_stream(out + (tb | slot_index((unsigned)r, hb)), czero<T>());
+}
+
 // The pass's ops on one tile held in vr/vi (normalised values, load
 // mapping), then the store and the tile's partial |out|^2 sum.
 template <typename T, typename R>
@@ -625,6 +636,13 @@ __device__ __forceinline__ void pass_tile(
 	for (int h = 0; h < HIGH_BITS; ++h)
 		hb[h] = 1ull << pd.hpos[h];
 	const u64 base = tile_base(tile, pd);
+	if (base & pd.zmask) {
+		// a tile of zeros stays zeros: written without a read or the ops
+		tile_zero(out, base, hb, lane, w);
+		if (partials && threadIdx.x == 0)
+			partials[tile] = 0.0;
+		return;
+	}
 	R vr[REGS], vi[REGS];
 	tile_fetch(in, base, hb, lane, w, vr, vi);
 #pragma unroll
@@ -700,6 +718,11 @@ __attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass_round(
 		for (int h = 0; h < HIGH_BITS; ++h)
 			hb[h] = 1ull << pd.hpos[h];
 		const u64 base = tile_base(tile, pd);
+		if (base & pd.zmask) {
+			// uniform for the workgroup: every thread skips the barrier
+			tile_zero(st, base, hb, lane, w);
+			continue;
+		}
 		R vr[REGS], vi[REGS];
 		if (init) {
 #pragma unroll
@@ -767,14 +790,21 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 		return DK_E_ARG;
 	}
 	const u64 ntiles = in->len >> PASS_TILE_BITS;
+	// tiles outside the input's support hold zeros and keep them (only
+	// when every matrix entry is finite: 0 * inf would not be zero)
+	PassDesc d = pd;
+	const u64 valid = in->len - 1;
+	d.zmask = pd.finite ? (~in->support & valid) : 0;
+	const u64 tmask = ((1ull << LANE_BITS) - 1) | seen;
 	double *parts = nullptr;
 	DK_TRY(dk_partials_reserve(ntiles, &parts));
 	DK_TRY(dk_time_mark(0));
 	int rc = in->dtype == DK_C64
-			 ? launch_pass_typed<float2>(in, out, pd, ops_dev, parts, ntiles)
-			 : launch_pass_typed<double2>(in, out, pd, ops_dev, parts, ntiles);
+			 ? launch_pass_typed<float2>(in, out, d, ops_dev, parts, ntiles)
+			 : launch_pass_typed<double2>(in, out, d, ops_dev, parts, ntiles);
 	if (rc)
 		return rc;
+	out->support = in->support | tmask;
 	DK_TRY(dk_time_mark(1));
 	return dk_finish_partials(parts, ntiles, out->scal + DK_SCAL_NORM, 1);
 }

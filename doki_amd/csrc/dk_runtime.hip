@@ -290,6 +290,7 @@ int dk_alloc_state(unsigned nq, int dtype, dk_state **out)
 	s->amp = base;
 	s->scal = (double *)(split ? sbase : base + abytes);
 	s->bytes = bytes;
+	s->support = ~0ull;
 	*out = s;
 	return DK_OK;
 }
@@ -329,6 +330,8 @@ extern "C" int dk_state_new_typed(unsigned num_qubits, int zero_state,
 		dk_release_state(s);
 		return rc;
 	}
+	if (zero_state)
+		s->support = 0;
 	*out = s;
 	return DK_OK;
 }
@@ -369,6 +372,8 @@ extern "C" int dk_state_new(unsigned num_qubits, int zero_state, dk_state **out)
 		dk_release_state(s);
 		return rc;
 	}
+	if (zero_state)
+		s->support = 0;
 	*out = s;
 	return DK_OK;
 }

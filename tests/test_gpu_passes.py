@@ -147,3 +147,48 @@ def test_time_ring_records_every_launch(capi):
         assert len(capi.time_ring_read()) == min(2, nsteps)
     finally:
         capi.tune("time_ring", 0)
+
+
+def zero_state_without_support(capi, n, dtype):
+    """|0...0> made from host values: the library knows nothing of its zeros,
+    so every pass reads and computes every tile."""
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1.0
+    s = capi.from_host(v)
+    return s if dtype == capi.C128 else capi.convert(s, dtype)
+
+
+@pytest.mark.parametrize("n,dtype", [(14, 0), (18, 0), (20, 0), (18, 1)])
+def test_zero_tiles_of_fresh_states(capi, oracle, n, dtype):
+    """From a new |0...0> state the passes write tiles outside the state's
+    support as zeros without reading them or running the ops: the result
+    equals the same plan run on an input the library cannot know to be
+    sparse (bit for bit, zeros of either sign alike), and the oracle."""
+    print(f"[passes] zero tiles n={n} dtype={dtype}", flush=True)
+    steps = mixed_steps(700 + n, n, 180)
+    circ = capi.Circuit(n, steps)
+    circ.passes()
+    fresh = circ.run(capi.new_typed(n, dtype), in_place=True)
+    dense = circ.run(zero_state_without_support(capi, n, dtype), in_place=True)
+    a, b = fresh.values(), dense.values()
+    assert np.array_equal(a, b)
+    assert fresh.norm == dense.norm
+    if dtype == capi.C128:
+        want = oracle_run(oracle, oracle.new(n), steps).values()
+        assert maxabs(a, want) <= ATOL
+
+
+def test_zero_tiles_not_assumed_with_non_finite_gates(capi):
+    """A pass whose matrices hold an infinity computes every tile, as the
+    reference does (0 * inf is NaN): the result equals the dense run's,
+    NaN for NaN."""
+    n = 14
+    rng = np.random.default_rng(3)
+    steps = [(W.gate_h(), [q], set(), set()) for q in range(n)]
+    bad = np.array([[np.inf, 0.0], [0.0, 1.0]], dtype=np.complex128)
+    steps.insert(3, (bad, [int(rng.integers(n))], set(), set()))
+    circ = capi.Circuit(n, steps)
+    circ.passes()
+    fresh = circ.run(capi.new(n), in_place=True).raw()[0]
+    dense = circ.run(zero_state_without_support(capi, n, capi.C128), in_place=True).raw()[0]
+    assert np.array_equal(fresh, dense, equal_nan=True)
