@@ -972,10 +972,14 @@ static int batch_upload(dk_batch *b)
 				// tiles outside the support hold zeros and keep them
 				const u64 valid = (1ull << R.nq) - 1;
 				e.pd.zmask = e.pd.finite ? (~support[i] & valid) : 0;
+				// the first round writes the zeros over the pool's old
+				// contents; later rounds find them in place
+				e.pd.zwrite = r == 0;
 				u64 tmask = (1ull << dkp::LANE_BITS) - 1;
 				for (int h = 0; h < dkp::HIGH_BITS; ++h)
 					tmask |= 1ull << e.pd.hpos[h];
-				support[i] |= tmask;
+				// (after a non-finite entry nothing is known any more)
+				support[i] = e.pd.finite ? (support[i] | tmask) : ~0ull;
 				e.op_begin = obeg[i][r];
 				e.nq = R.nq;
 				b->rounds.push_back(e);

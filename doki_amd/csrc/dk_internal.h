@@ -90,6 +90,7 @@ struct dk_gate {
 	mutable std::vector<std::pair<dk_lane *, hipEvent_t>> uses;
 	double2 hmat[64 * 64];
 	float2 hmat32[64 * 64]; // source of the FP32 upload
+	bool finite = true;     // every matrix entry is finite
 };
 
 // ---- library context ---------------------------------------------------------

@@ -1218,11 +1218,12 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 	}
 	if (rc)
 		return rc;
-	// the gate mixes amplitudes only along its targets
+	// the gate mixes amplitudes only along its targets (a non-finite entry
+	// times a zero amplitude is NaN: then nothing is known any more)
 	u64 tmask = 0;
 	for (unsigned b = 0; b < g->k; ++b)
 		tmask |= 1ull << targets[b];
-	out->support = in->support | tmask;
+	out->support = g->finite ? (in->support | tmask) : ~0ull;
 	DK_TRY(dk_time_mark(1));
 	if (g->dmat)
 		DK_TRY(dk_gate_mark_use(g));

@@ -71,9 +71,12 @@ struct PassDesc {
 	uint8_t hsorted[HIGH_BITS]; // the same bits, ascending
 	uint32_t nops;
 	uint32_t finite;            // every op's matrix entries are finite
-	// index bits known to be zero in the input (set at launch): a tile
-	// whose fixed bits meet it holds only zeros, and so does its output,
-	// which is written without reading the tile or running the ops
+	// zwrite, zmask (set at launch): index bits known to be zero in the
+	// input; a tile whose fixed bits meet zmask holds only zeros, and so
+	// does its output, which is written as zeros (zwrite) or, where the
+	// output already holds those zeros (in place), not written at all,
+	// without reading the tile or running the ops
+	uint32_t zwrite;
 	uint64_t zmask;
 };
 

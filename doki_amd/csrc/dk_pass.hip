@@ -637,8 +637,9 @@ __device__ __forceinline__ void pass_tile(
 		hb[h] = 1ull << pd.hpos[h];
 	const u64 base = tile_base(tile, pd);
 	if (base & pd.zmask) {
-		// a tile of zeros stays zeros: written without a read or the ops
-		tile_zero(out, base, hb, lane, w);
+		// a tile of zeros stays zeros: no read, no ops
+		if (pd.zwrite)
+			tile_zero(out, base, hb, lane, w);
 		if (partials && threadIdx.x == 0)
 			partials[tile] = 0.0;
 		return;
@@ -720,7 +721,8 @@ __attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass_round(
 		const u64 base = tile_base(tile, pd);
 		if (base & pd.zmask) {
 			// uniform for the workgroup: every thread skips the barrier
-			tile_zero(st, base, hb, lane, w);
+			if (pd.zwrite)
+				tile_zero(st, base, hb, lane, w);
 			continue;
 		}
 		R vr[REGS], vi[REGS];
@@ -795,6 +797,8 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 	PassDesc d = pd;
 	const u64 valid = in->len - 1;
 	d.zmask = pd.finite ? (~in->support & valid) : 0;
+	// in place, the zeros are already there
+	d.zwrite = in != out;
 	const u64 tmask = ((1ull << LANE_BITS) - 1) | seen;
 	double *parts = nullptr;
 	DK_TRY(dk_partials_reserve(ntiles, &parts));
@@ -804,7 +808,9 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 			 : launch_pass_typed<double2>(in, out, d, ops_dev, parts, ntiles);
 	if (rc)
 		return rc;
-	out->support = in->support | tmask;
+	// (a non-finite entry times a zero amplitude is NaN: then nothing is
+	// known any more)
+	out->support = pd.finite ? (in->support | tmask) : ~0ull;
 	DK_TRY(dk_time_mark(1));
 	return dk_finish_partials(parts, ntiles, out->scal + DK_SCAL_NORM, 1);
 }

@@ -579,6 +579,9 @@ extern "C" int dk_gate_new(unsigned num_qubits, const double *matrix,
 	g->lane = dk_cur();
 	const size_t n = (size_t)g->dim * g->dim;
 	memcpy(g->hmat, matrix, n * sizeof(double2));
+	for (size_t i = 0; i < n; ++i)
+		if (!std::isfinite(g->hmat[i].x) || !std::isfinite(g->hmat[i].y))
+			g->finite = false;
 	if (num_qubits > 1) {
 		// FP64 and FP32 copies in one stream-ordered allocation of the
 		// calling lane; the host sources live in the gate itself

@@ -192,3 +192,21 @@ def test_zero_tiles_not_assumed_with_non_finite_gates(capi):
     fresh = circ.run(capi.new(n), in_place=True).raw()[0]
     dense = circ.run(zero_state_without_support(capi, n, capi.C128), in_place=True).raw()[0]
     assert np.array_equal(fresh, dense, equal_nan=True)
+
+
+def test_zero_tiles_after_a_non_finite_lone_gate(capi):
+    """A multi-qubit gate with a NaN entry, run alone between passes, leaves
+    nothing known of the state's zeros: the passes after it compute every
+    tile, and the result equals the dense run's, NaN for NaN."""
+    n = 16
+    bad = np.eye(4, dtype=np.complex128)
+    bad[1, 2] = np.nan
+    steps = [(W.gate_h(), [q], set(), set()) for q in range(0, n, 2)]
+    steps.append((bad, [3, 11], set(), set()))
+    steps += [(W.gate_h(), [q], set(), set()) for q in range(1, n, 2)]
+    circ = capi.Circuit(n, steps)
+    circ.passes()
+    assert any(kind == "gate" for kind, _ in circ.plan())
+    fresh = circ.run(capi.new(n), in_place=True).raw()[0]
+    dense = circ.run(zero_state_without_support(capi, n, capi.C128), in_place=True).raw()[0]
+    assert np.array_equal(fresh, dense, equal_nan=True)
