@@ -613,6 +613,15 @@ __global__ __launch_bounds__(CT) void k_chunk_sums(
 	}
 }
 
+// Zeros over the first `len` amplitudes of the pool (before the rounds).
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_pool(T *__restrict__ pool, u64 len)
+{
+	for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < len;
+	     i += (u64)gridDim.x * 256)
+		st_stream(pool + i, czero<T>());
+}
+
 __global__ void k_reset_counter(unsigned *c)
 {
 	if (threadIdx.x == 0)
@@ -654,6 +663,7 @@ struct dk_batch {
 	u64 dbuf_bytes = 0;
 	void *pool = nullptr;
 	u64 pool_bytes = 0;
+	u64 pool_amps = 0;   // amplitudes of the pool the registers use
 	char *hbuf = nullptr;
 	u64 hbuf_bytes = 0;
 	BReg *d_regs = nullptr;
@@ -972,9 +982,10 @@ static int batch_upload(dk_batch *b)
 				// tiles outside the support hold zeros and keep them
 				const u64 valid = (1ull << R.nq) - 1;
 				e.pd.zmask = e.pd.finite ? (~support[i] & valid) : 0;
-				// the first round writes the zeros over the pool's old
-				// contents; later rounds find them in place
-				e.pd.zwrite = r == 0;
+				// the pool is cleared before the first round, so no
+				// round writes zeros: a finite first pass runs only the
+				// tile holding amplitude 0 (tile 0)
+				e.pd.zwrite = 0;
 				u64 tmask = (1ull << dkp::LANE_BITS) - 1;
 				for (int h = 0; h < dkp::HIGH_BITS; ++h)
 					tmask |= 1ull << e.pd.hpos[h];
@@ -983,7 +994,9 @@ static int batch_upload(dk_batch *b)
 				e.op_begin = obeg[i][r];
 				e.nq = R.nq;
 				b->rounds.push_back(e);
-				items += 1ull << (R.nq - (unsigned)dkp::PASS_TILE_BITS);
+				items += r == 0 && e.pd.finite
+						 ? 1
+						 : 1ull << (R.nq - (unsigned)dkp::PASS_TILE_BITS);
 			}
 			b->round_items.push_back(items);
 		}
@@ -1027,6 +1040,7 @@ static int batch_upload(dk_batch *b)
 			return rc;
 		b->dbuf_bytes = total;
 	}
+	b->pool_amps = pool_amps;
 	const u64 pool_need = std::max<u64>(256, pool_amps * asz);
 	if (b->pool_bytes < pool_need) {
 		if (b->pool)
@@ -1108,6 +1122,20 @@ static int batch_enqueue(dk_batch *b)
 	hipStream_t st = DK_STREAM;
 	const unsigned nregs = (unsigned)b->regs.size();
 	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
+	if (!b->round_items.empty()) {
+		// the rounds skip tiles outside each register's support, which
+		// must hold zeros: clear the pool first (a kernel, not a memset,
+		// so that the launch sequence can be captured in a graph)
+		const unsigned zg = (unsigned)std::min<u64>(
+			std::max<u64>(1, b->pool_amps / 256), (u64)std::max(1, g_ctx.num_cus) * 16);
+		if (b->dtype == DK_C64)
+			hipLaunchKernelGGL(k_zero_pool<float2>, dim3(zg), dim3(256), 0, st,
+					   (float2 *)b->pool, b->pool_amps);
+		else
+			hipLaunchKernelGGL(k_zero_pool<double2>, dim3(zg), dim3(256), 0, st,
+					   (double2 *)b->pool, b->pool_amps);
+		DK_LAUNCHED();
+	}
 	for (size_t r = 0; r < b->round_items.size(); ++r) {
 		const size_t e1 = r + 1 < b->round_off.size() ? b->round_off[r + 1]
 							       : b->rounds.size();
