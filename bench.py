@@ -305,9 +305,19 @@ def fused_leg(args):
     n = args.qubits
     out = {"dense_kernel": {}}
     rng = W.rng_for(4242)
-    capi.tune("time_ring", 16)
-    for dtype, name, amp_bytes in ((capi.C128, "f64", 16), (capi.C64, "f32", 8)):
+    hgate = capi.Gate(W.gate_h())
+
+    def dense_state(dtype):
+        """|+...+>: every amplitude nonzero (and known to be), so that no
+        kernel can skip tiles of zeros."""
         s = capi.new_typed(n, dtype)
+        for q in range(n):
+            capi.apply(s, hgate, [q], in_place=True)
+        return s
+
+    for dtype, name, amp_bytes in ((capi.C128, "f64", 16), (capi.C64, "f32", 8)):
+        s = dense_state(dtype)
+        capi.tune("time_ring", 16)
         for k in (4, 5, 6):
             g = capi.Gate(W.haar_unitary(rng, k))
             for rep in range(3):
@@ -319,9 +329,9 @@ def fused_leg(args):
                 "kernel_ms": round(t * 1e3, 3),
                 "GBps": round(2 * amp_bytes * (1 << n) / t / 1e9, 1),
                 "TFLOPs": round(8 * (1 << k) * (1 << n) / t / 1e12, 2)}
+        capi.tune("time_ring", 0)
         del s
         capi.synchronize()
-    capi.tune("time_ring", 0)
     log(f"[bench] dense kernels {out['dense_kernel']}")
 
     # circuits of Haar-random k-qubit blocks (four layers of blocks across
@@ -348,6 +358,15 @@ def fused_leg(args):
             capi.synchronize()
             rk[f"{name}_seconds"] = round(time.perf_counter() - t0, 4)
             states[name] = s
+            # the same blocks on a dense input (no tiles of zeros to skip)
+            d = dense_state(dtype)
+            capi.synchronize()
+            t0 = time.perf_counter()
+            for g, tg in blocks:
+                capi.apply(d, g, tg, in_place=True)
+            capi.synchronize()
+            rk[f"{name}_seconds_dense_input"] = round(time.perf_counter() - t0, 4)
+            del d
         inner, mx = capi.compare(states["f64"], states["f32"])
         rk["f32_max_abs_err_vs_f64"] = mx
         rk["f32_fidelity_vs_f64"] = abs(inner) ** 2
@@ -388,6 +407,18 @@ def fused_leg(args):
         if label != "plain_f64":
             del st
         capi.synchronize()
+        if label != "plain_f64":
+            # the same plan on a dense input: from |0...0> the first passes
+            # (blocks) find most tiles outside the state's support and skip
+            # them; here every tile is computed
+            d = dense_state(dtype)
+            capi.synchronize()
+            t0 = time.perf_counter()
+            circ.run(d, in_place=True)
+            capi.synchronize()
+            res[label]["seconds_dense_input"] = round(time.perf_counter() - t0, 4)
+            del d
+            capi.synchronize()
         log(f"[bench] {label}: {res[label]}")
     out["brickwork_circuit"] = {"qubits": n, "depth": depth, "gates": len(steps), **res}
     del ref_state

@@ -1126,8 +1126,10 @@ static int batch_enqueue(dk_batch *b)
 		// the rounds skip tiles outside each register's support, which
 		// must hold zeros: clear the pool first (a kernel, not a memset,
 		// so that the launch sequence can be captured in a graph)
+		// one amplitude per thread over the whole pool (a capped
+		// grid-stride loop streams at two thirds of the rate)
 		const unsigned zg = (unsigned)std::min<u64>(
-			std::max<u64>(1, b->pool_amps / 256), (u64)std::max(1, g_ctx.num_cus) * 16);
+			std::max<u64>(1, (b->pool_amps + 255) / 256), (1ull << 31) / 256);
 		if (b->dtype == DK_C64)
 			hipLaunchKernelGGL(k_zero_pool<float2>, dim3(zg), dim3(256), 0, st,
 					   (float2 *)b->pool, b->pool_amps);

@@ -81,13 +81,26 @@ __device__ __forceinline__ void dense_tile(
 	const int rt = wv % RT, cw = wv / RT;
 	const int li = lane & 15, lk = lane >> 4;
 	const u64 base = tile_base(tile, ta);
+	constexpr int PER = TS / NT; // 16 slots per thread
+	if (base & ta.zmask) {
+		// a tile of zeros stays zeros: no read, no product
+		if (ta.zwrite) {
+			const u64 tp = base | ta.dep_lo[threadIdx.x & 63] |
+				       ta.dep_hi[threadIdx.x >> 6];
+#pragma unroll
+			for (int j = 0; j < PER; ++j)
+				st_stream(out + (tp | ta.dep_hi[(j * NT) >> 6]), czero<T>());
+		}
+		if (threadIdx.x == 0)
+			partials[tile] = 0.0;
+		return;
+	}
 	// complex128 k = 4 tiles load straight into LDS (measured on gfx950 at
 	// 30 qubits: 6.39 against 6.67 ms; k = 5 and 6 ran 4% and 23% slower)
 	constexpr bool LDS_DIRECT = sizeof(T) == 16 && K == 4;
 
 	// Slot s = threadIdx.x + j * NT: its index bits are this thread's part
 	// (computed once) OR the part of j * NT (the same for the whole block).
-	constexpr int PER = TS / NT; // 16 slots per thread
 	const u64 tpart = base | ta.dep_lo[threadIdx.x & 63] |
 			  ta.dep_hi[threadIdx.x >> 6];
 	// issue the tile's loads first, then set up fragments while they fly
@@ -226,6 +239,11 @@ __global__ __launch_bounds__((4 << K) * CW) void k_apply_dense(
 struct StreamArgs {
 	u64 rowoff[64];            // gate row / column j -> its target index bits
 	unsigned char tsorted[8];  // target bit positions, ascending
+	// index bits known to be zero in the input: a column block whose base
+	// meets zmask holds only zeros, and so does its output (written as
+	// zeros when zwrite, else left as it is)
+	int zwrite;
+	u64 zmask;
 };
 
 constexpr int SW = 4;              // waves per streaming block
@@ -302,7 +320,7 @@ __global__ __launch_bounds__(SW * 64) void k_dense_stream(
 	};
 	double nacc = 0.0;
 	T bn[CT][KS];
-	if (gw < ncg) {
+	if (gw < ncg && !(block_base(gw) & sa.zmask)) {
 		const u64 nb = block_base(gw);
 #pragma unroll
 		for (int ks = 0; ks < KS; ++ks)
@@ -323,13 +341,26 @@ __global__ __launch_bounds__(SW * 64) void k_dense_stream(
 			for (int ks = 0; ks < KS; ++ks)
 				bc[c][ks] = bn[c][ks];
 		const u64 base = block_base(cg);
-		if (cg + W < ncg) {
+		if (cg + W < ncg && !(block_base(cg + W) & sa.zmask)) {
 			const u64 nb = block_base(cg + W);
 #pragma unroll
 			for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
 				for (int c = 0; c < CT; ++c)
 					bn[c][ks] = ldv(in + (nb | col[c] | roff[ks * 4 + flk]));
+		}
+		if (base & sa.zmask) {
+			// a block of zeros stays zeros: no product (the wave's rows
+			// j = lk + 4m of its groups, when written)
+			if (sa.zwrite) {
+#pragma unroll
+				for (int m = 0; m < D / 4; ++m)
+#pragma unroll
+					for (int c = 0; c < CT; ++c)
+						stv(out + (base | col[c] | roff[4 * m + flk]),
+						    czero<T>());
+			}
+			continue;
 		}
 		if constexpr (G3) {
 			// three real products per complex one (Gauss):
@@ -555,6 +586,13 @@ unsigned dk_dense_tile_bits(unsigned k, int dtype)
 	return k + 6 + cwb;
 }
 
+// Index bits known to be zero in the input (none when the gate has a
+// non-finite entry: 0 * inf is not zero).
+static u64 dense_zmask(const dk_state *in, const dk_gate *g)
+{
+	return g->finite ? (~in->support & (in->len - 1)) : 0;
+}
+
 // Streaming dense launch: grid, partials (one per wave) and the gate's
 // index tables.
 static int launch_dense_stream(const dk_state *in, dk_state *out, const dk_gate *g,
@@ -577,6 +615,8 @@ static int launch_dense_stream(const dk_state *in, dk_state *out, const dk_gate 
 	std::sort(ts, ts + k);
 	for (unsigned b = 0; b < k; ++b)
 		sa.tsorted[b] = (unsigned char)ts[b];
+	sa.zmask = dense_zmask(in, g);
+	sa.zwrite = in != out;
 	const bool c128 = in->dtype != DK_C64;
 	int ct = 1;
 	if (k == 4)
@@ -667,6 +707,8 @@ int dk_launch_apply_dense(const dk_state *in, dk_state *out, const dk_gate *g,
 	}
 	TileArgs ta;
 	dk_tile_args(in->nq, g->k, targets, ta, (int)tb);
+	ta.zmask = dense_zmask(in, g);
+	ta.zwrite = in != out;
 	const u64 nblocks = in->len >> ta.npos;
 	int rc = scratch ? dk_partials_in(scratch, scratch_cap, nblocks, parts)
 			 : dk_partials_reserve(nblocks, parts);

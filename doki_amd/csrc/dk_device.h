@@ -126,6 +126,11 @@ struct TileArgs {
 	unsigned char pos[TILE_BITS]; // tile positions, ascending
 	unsigned char tslot[8];    // slot positions of the targets, ascending
 	int npos;
+	// index bits known to be zero in the input (dense gates): a tile whose
+	// fixed bits meet zmask holds only zeros, and so does its output,
+	// written as zeros when zwrite (out of place), else left as it is
+	int zwrite;
+	u64 zmask;
 };
 
 __device__ __forceinline__ u64 tile_base(u64 tile, const TileArgs &ta)

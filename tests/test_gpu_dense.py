@@ -184,3 +184,37 @@ def test_dense_k4_column_sets_vs_oracle(capi, oracle, cw):
                 assert maxabs(d.values(), o.values()) <= tol, (dtype, targets)
     finally:
         capi.tune("dense_k4_cw", 0)
+
+
+@pytest.mark.parametrize("form", [0, 2])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_dense_zero_blocks_of_fresh_states(capi, oracle, form, dtype, k):
+    """Dense gates on a new |0...0> state skip the tiles (column blocks)
+    outside the state's support, out of place (zeros written) and in place
+    (zeros left where they are): every result equals the one on an input the
+    library cannot know to be sparse, and the oracle."""
+    n = 16
+    dt = capi.C128 if dtype == "c128" else capi.C64
+    tol = ATOL if dt == capi.C128 else C64_TOL
+    rng = W.rng_for(9500 + 7 * k + form)
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1.0
+    fresh = capi.new_typed(n, dt)
+    dense = capi.convert(capi.from_host(v), dt)
+    o = oracle.new(n)
+    capi.tune("dense_stream", form)
+    try:
+        for i, targets in enumerate((list(range(n - k, n)), list(range(3, 3 + k)),
+                                     [int(x) for x in rng.choice(n, size=k, replace=False)])):
+            u = W.haar_unitary(rng, k)
+            g = capi.Gate(u)
+            in_place = i % 2 == 1
+            fresh = capi.apply(fresh, g, targets, in_place=in_place)
+            dense = capi.apply(dense, g, targets, in_place=in_place)
+            o = oracle.apply(o, u, targets)
+            a = fresh.values()
+            assert np.array_equal(a, dense.values()), targets
+            assert maxabs(a, o.values()) <= tol, targets
+    finally:
+        capi.tune("dense_stream", 1)
