@@ -30,6 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# target sets of the dense kernels' by-position table (low, mixed, middle,
+# high and spread index bits)
+DENSE_TARGET_SETS = ((0, 1, 2, 3, 4, 5), (3, 4, 5, 6, 7, 8), (10, 11, 12, 13, 14, 15),
+                     (20, 21, 22, 23, 24, 25), (24, 25, 26, 27, 28, 29),
+                     (1, 7, 13, 19, 25, 29))
 METRIC = ("HBM GB/s per gate at 30q complex128; served circuits/s + p99 "
           "latency vs plain")
 
@@ -303,7 +308,8 @@ def fused_leg(args):
 
     from doki_amd import capi, workloads as W
     n = args.qubits
-    out = {"dense_kernel": {}}
+    out = {"dense_kernel": {}, "dense_by_target_ms": {}}
+    dense_by_target = out["dense_by_target_ms"]
     rng = W.rng_for(4242)
     hgate = capi.Gate(W.gate_h())
 
@@ -329,10 +335,25 @@ def fused_leg(args):
                 "kernel_ms": round(t * 1e3, 3),
                 "GBps": round(2 * amp_bytes * (1 << n) / t / 1e9, 1),
                 "TFLOPs": round(8 * (1 << k) * (1 << n) / t / 1e12, 2)}
+            # the same gate by target position and kernel form (median of
+            # three launches; the k leading targets of each set)
+            for form, fname in ((0, "tiles"), (2, "streaming"), (1, "auto")):
+                capi.tune("dense_stream", form)
+                row = {}
+                for ts in DENSE_TARGET_SETS:
+                    if max(ts[:k]) >= n:
+                        continue
+                    for rep in range(3):
+                        capi.apply(s, g, list(ts[:k]), in_place=True)
+                    row[",".join(map(str, ts[:k]))] = round(
+                        float(np.median(capi.time_ring_read())), 3)
+                dense_by_target[f"{name}_k{k}_{fname}"] = row
+            capi.tune("dense_stream", 1)
         capi.tune("time_ring", 0)
         del s
         capi.synchronize()
     log(f"[bench] dense kernels {out['dense_kernel']}")
+    log(f"[bench] dense kernels by target (ms) {dense_by_target}")
 
     # circuits of Haar-random k-qubit blocks (four layers of blocks across
     # the register, shifted per layer), in FP64 and in complex64/FP32; the
