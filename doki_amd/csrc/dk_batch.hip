@@ -420,21 +420,76 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			const int m = (int)n - a;
 			const int dim = 1 << a;
 			const u64 T_ = 1ull << m;
-			for (int e = threadIdx.x; e < dim * dim; e += BT) {
-				const u64 ki = deposit((u64)(e / dim), kp, a);
-				const u64 kj = deposit((u64)(e % dim), kp, a);
-				double2 s = make_double2(0.0, 0.0);
-				for (u64 t = 0; t < T_; ++t) {
-					u64 td = t;
-					for (int b = 0; b < a; ++b)
-						td = ins0(td, kp[b]);
-					const T x = amp_at(tile, st, resident, ki | td);
-					const T y = amp_at(tile, st, resident, kj | td);
-					const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
-					s.x = fma(xr, yr, fma(xi, yi, s.x));
-					s.y = fma(xi, yr, fma(-xr, yi, s.y));
+			if (resident) {
+				for (int e = threadIdx.x; e < dim * dim; e += BT) {
+					const u64 ki = deposit((u64)(e / dim), kp, a);
+					const u64 kj = deposit((u64)(e % dim), kp, a);
+					double2 s = make_double2(0.0, 0.0);
+					for (u64 t = 0; t < T_; ++t) {
+						u64 td = t;
+						for (int b = 0; b < a; ++b)
+							td = ins0(td, kp[b]);
+						const T x = tile[lds_pos<T>((int)(ki | td))];
+						const T y = tile[lds_pos<T>((int)(kj | td))];
+						const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+						s.x = fma(xr, yr, fma(xi, yi, s.x));
+						s.y = fma(xi, yr, fma(-xr, yi, s.y));
+					}
+					dens[R.dens_off + e] = make_double2(s.x / norm2, s.y / norm2);
 				}
-				dens[R.dens_off + e] = make_double2(s.x / norm2, s.y / norm2);
+			} else {
+				// a pooled state: its t-vectors (the dim kept-qubit
+				// amplitudes of one value t of the other bits) gathered
+				// into LDS a chunk at a time and read there by every entry
+				// (one thread per entry reading the pool took ~1.7 ms for
+				// a 14-qubit register, latency-bound); each entry still
+				// sums over t in ascending order, with the same operations.
+				// Up to 8 entries per thread (dim <= 64), four at a time:
+				// the state is gathered once per four
+				constexpr int EPR = 4;
+				const int tc = LT / dim;         // t-vectors per chunk
+				for (int e0 = 0; e0 < dim * dim; e0 += EPR * BT) {
+					double2 acc[EPR];
+#pragma unroll
+					for (int k = 0; k < EPR; ++k)
+						acc[k] = make_double2(0.0, 0.0);
+					for (u64 t0 = 0; t0 < T_; t0 += (u64)tc) {
+						const int nt = T_ - t0 < (u64)tc ? (int)(T_ - t0) : tc;
+						for (int s = threadIdx.x; s < nt * dim; s += BT) {
+							u64 td = t0 + (u64)(s / dim);
+							for (int b = 0; b < a; ++b)
+								td = ins0(td, kp[b]);
+							tile[lds_pos<T>(s)] =
+								st[deposit((u64)(s % dim), kp, a) | td];
+						}
+						__syncthreads();
+#pragma unroll
+						for (int k = 0; k < EPR; ++k) {
+							const int e = e0 + (int)threadIdx.x + k * BT;
+							if (e < dim * dim) {
+								const int i = e / dim, j = e % dim;
+								double2 s = acc[k];
+								for (int tt = 0; tt < nt; ++tt) {
+									const T x = tile[lds_pos<T>(tt * dim + i)];
+									const T y = tile[lds_pos<T>(tt * dim + j)];
+									const double xr = x.x, xi = x.y;
+									const double yr = y.x, yi = y.y;
+									s.x = fma(xr, yr, fma(xi, yi, s.x));
+									s.y = fma(xi, yr, fma(-xr, yi, s.y));
+								}
+								acc[k] = s;
+							}
+						}
+						__syncthreads();
+					}
+#pragma unroll
+					for (int k = 0; k < EPR; ++k) {
+						const int e = e0 + (int)threadIdx.x + k * BT;
+						if (e < dim * dim)
+							dens[R.dens_off + e] = make_double2(
+								acc[k].x / norm2, acc[k].y / norm2);
+					}
+				}
 			}
 			__syncthreads();
 		}

@@ -147,3 +147,28 @@ def test_batch_default_split_vs_oracle(capi, oracle):
     specs = [(17, *W.random_circuit(9801, 17, 60)), (18, *W.random_circuit(9802, 18, 45)),
              (12, *W.random_circuit(9803, 12, 30)), (16, *W.random_circuit(9804, 16, 30))]
     check_batch(capi, oracle, capi.C128, specs, ATOL)
+
+
+@pytest.mark.parametrize("dtype_name", ["c128", "c64"])
+def test_batch_densities_scattered_keeps(capi, oracle, dtype_name):
+    """Reduced densities over scattered kept qubits (one to six, the top
+    qubit among them) of registers held in LDS, in the pool and split into
+    rounds, against the contraction of the oracle's final state; the pooled
+    and split ones are summed from chunks of the state gathered into LDS."""
+    dtype = capi.C128 if dtype_name == "c128" else capi.C64
+    tol = ATOL if dtype == capi.C128 else 5e-6
+    cases = [(9, (0, 8)), (12, (3, 11)), (13, (1, 4, 9, 12)), (13, (0,)),
+             (14, (0, 3, 5, 8, 10, 13)), (14, (13,)), (15, (2, 7)),
+             (16, (0, 1, 2, 3, 4, 15)), (17, (5, 6, 16))]
+    b = capi.Batch(dtype)
+    specs = []
+    for i, (n, keep) in enumerate(cases):
+        steps, rolls = W.random_circuit(7700 + i, n, 40)
+        specs.append((n, steps, keep))
+        b.add(n, steps, rolls, keep)
+    b.run()
+    for i, (n, steps, keep) in enumerate(specs):
+        want = oracle_run(oracle, oracle.new(n), steps).values()
+        rho = b.density(i)
+        assert rho.shape == (1 << len(keep), 1 << len(keep))
+        assert np.max(np.abs(rho - reduced_density_einsum(want, keep))) <= tol, (i, n, keep)
