@@ -67,6 +67,9 @@ struct BReg {
 	u64 off;        // amplitude offset of its pool slot (pooled registers)
 	u64 coff;       // split registers: first of its chunks' sum records
 	u64 keep;       // density: kept qubits (0 = no density)
+	u64 zmask;      // split registers: index bits outside the final state's
+	                // support (amplitudes there are zeros the pool may not
+	                // hold: they are taken as zero, never read)
 	unsigned nq;
 	unsigned pass_begin, pass_end;
 	unsigned roll_off;  // first roll; also first outcome slot
@@ -459,8 +462,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 							u64 td = t0 + (u64)(s / dim);
 							for (int b = 0; b < a; ++b)
 								td = ins0(td, kp[b]);
+							const u64 ix = deposit((u64)(s % dim), kp, a) | td;
 							tile[lds_pos<T>(s)] =
-								st[deposit((u64)(s % dim), kp, a) | td];
+								(ix & R.zmask) ? czero<T>() : st[ix];
 						}
 						__syncthreads();
 #pragma unroll
@@ -519,7 +523,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			}
 			const T *chunk = st + (prefix << CHUNK_BITS);
 			for (int i = threadIdx.x; i < CHUNK; i += BT)
-				tile[lds_pos<T>(i)] = chunk[i];
+				tile[lds_pos<T>(i)] = (((prefix << CHUNK_BITS) | (u64)i) & R.zmask)
+							      ? czero<T>() : chunk[i];
 			__syncthreads();
 			u64 lp = 0;
 			for (; step < n; ++step) {
@@ -585,6 +590,7 @@ struct ChunkEntry {
 	u64 first;      // first item (chunk) of this register
 	u64 off;        // its amplitude offset in the pool
 	u64 coff;       // its first chunk record
+	u64 zmask;      // index bits outside its final support (zeros, not read)
 };
 
 template <typename T>
@@ -610,13 +616,24 @@ __global__ __launch_bounds__(CT) void k_chunk_sums(
 		__syncthreads();
 		const unsigned e = __builtin_amdgcn_readfirstlane(ent_sh);
 		const u64 c = it - ents[e].first;
+		const u64 zm = ents[e].zmask;
+		if ((c << CHUNK_BITS) & zm) {
+			// a chunk outside the support: zeros, not read (the pool
+			// may not hold them)
+			if (threadIdx.x < CREC)
+				sums[(ents[e].coff + c) * CREC + threadIdx.x] = 0.0;
+			__syncthreads();   // ent_sh is rewritten next
+			continue;
+		}
+		const unsigned zl = (unsigned)zm & (CHUNK - 1);
 		const T *src = pool + ents[e].off + (c << CHUNK_BITS);
 		// amplitude k * CT + threadIdx.x: bits 0..7 from the thread, bits
 		// 8..11 from k
 		double tot = 0.0, hb[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
 		for (int k = 0; k < CHUNK / CT; ++k) {
-			const double a = cabs2d(src[k * CT + threadIdx.x]);
+			const unsigned j = k * CT + threadIdx.x;
+			const double a = (j & zl) ? 0.0 : cabs2d(src[j]);
 			tot += a;
 #pragma unroll
 			for (int b = 0; b < 4; ++b)
@@ -677,6 +694,17 @@ __global__ __launch_bounds__(256) void k_zero_pool(T *__restrict__ pool, u64 len
 		st_stream(pool + i, czero<T>());
 }
 
+// NaN over the whole pool (the "poison_pool" setting: a run that read an
+// amplitude no pass wrote would carry the NaN into its results)
+template <typename T>
+__global__ __launch_bounds__(256) void k_poison_pool(T *__restrict__ pool, u64 len)
+{
+	const u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+	if (i < len)
+		pool[i] = mk<T>((typename real_of<T>::type)__builtin_nan(""),
+				(typename real_of<T>::type)__builtin_nan(""));
+}
+
 __global__ void k_reset_counter(unsigned *c)
 {
 	if (threadIdx.x == 0)
@@ -700,6 +728,8 @@ struct dk_batch {
 	int probs = 0;       // marginal probabilities of every register
 	int split_qubits = 14; // pooled registers this large run their passes
 			       // as grid-wide rounds (0 = never)
+	int poison_pool = 0;   // NaN over the pool before each run (unless
+			       // states are exported, which needs it cleared)
 	std::vector<BGate> orig;          // every register's gates as added
 	std::vector<size_t> orig_begin;   // first gate of register i in orig
 	std::vector<dkp::PassEntry> rounds; // every round's entries, in order
@@ -810,6 +840,8 @@ extern "C" int dk_batch_set(dk_batch *b, const char *key, int64_t value)
 		b->probs = value != 0;
 	else if (!strcmp(key, "split_qubits"))
 		b->split_qubits = (int)std::max<int64_t>(0, value);
+	else if (!strcmp(key, "poison_pool"))
+		b->poison_pool = value != 0;
 	else {
 		dk_set_error("unknown batch setting %s", key);
 		return DK_E_ARG;
@@ -957,6 +989,7 @@ static int batch_upload(dk_batch *b)
 		R.flags = (R.flags & ~(BF_EXPORT | BF_PROBS | BF_SPLIT)) |
 			  (b->export_states ? BF_EXPORT : 0) | (b->probs ? BF_PROBS : 0) |
 			  (split ? BF_SPLIT : 0);
+		R.zmask = 0;   // set below for split registers
 		const u64 len = 1ull << R.nq;
 		if (R.nq > (unsigned)lb || b->export_states) {
 			R.off = pool_amps;
@@ -1034,13 +1067,18 @@ static int batch_upload(dk_batch *b)
 				e.first = items;
 				e.off = R.off;
 				e.pd = descs[i][r];
-				// tiles outside the support hold zeros and keep them
+				// tiles outside the support are zeros: skipped, neither
+				// read nor written (the pool is not cleared, so no
+				// later reader may read them either: the final support
+				// goes to the chunk sums and the persistent launch).  A
+				// finite first pass therefore runs only the tile holding
+				// amplitude 0 (tile 0); inside a tile that runs, the
+				// amplitudes outside the support are taken as zero
+				// without a read
 				const u64 valid = (1ull << R.nq) - 1;
 				e.pd.zmask = e.pd.finite ? (~support[i] & valid) : 0;
-				// the pool is cleared before the first round, so no
-				// round writes zeros: a finite first pass runs only the
-				// tile holding amplitude 0 (tile 0)
 				e.pd.zwrite = 0;
+				e.pd.kmask = ~support[i] & valid;
 				u64 tmask = (1ull << dkp::LANE_BITS) - 1;
 				for (int h = 0; h < dkp::HIGH_BITS; ++h)
 					tmask |= 1ull << e.pd.hpos[h];
@@ -1055,6 +1093,9 @@ static int batch_upload(dk_batch *b)
 			}
 			b->round_items.push_back(items);
 		}
+		for (size_t i = 0; i < b->regs.size(); ++i)
+			if (b->regs[i].flags & BF_SPLIT)
+				b->regs[i].zmask = ~support[i] & ((1ull << b->regs[i].nq) - 1);
 	}
 	// chunk sums of every split register's final state
 	b->chunks.clear();
@@ -1065,7 +1106,7 @@ static int batch_upload(dk_batch *b)
 		if (!(R.flags & BF_SPLIT))
 			continue;
 		R.coff = b->chunk_items;
-		b->chunks.push_back(ChunkEntry{b->chunk_items, R.off, R.coff});
+		b->chunks.push_back(ChunkEntry{b->chunk_items, R.off, R.coff, R.zmask});
 		b->chunk_items += 1ull << (R.nq - (unsigned)CHUNK_BITS);
 	}
 	auto al = [](u64 x) { return (x + 255) & ~255ull; };
@@ -1177,20 +1218,23 @@ static int batch_enqueue(dk_batch *b)
 	hipStream_t st = DK_STREAM;
 	const unsigned nregs = (unsigned)b->regs.size();
 	const unsigned grid = (unsigned)std::max(1, g_ctx.num_cus) * 2;
-	if (!b->round_items.empty()) {
-		// the rounds skip tiles outside each register's support, which
-		// must hold zeros: clear the pool first (a kernel, not a memset,
-		// so that the launch sequence can be captured in a graph)
-		// one amplitude per thread over the whole pool (a capped
-		// grid-stride loop streams at two thirds of the rate)
+	if (!b->round_items.empty() && (b->export_states || b->poison_pool)) {
+		// the rounds leave every amplitude outside a register's support
+		// unwritten; exported states need those zeros in the pool.  One
+		// amplitude per thread over the whole pool (a kernel, not a
+		// memset, so that the sequence can be captured in a graph; a
+		// capped grid-stride loop streams at two thirds of the rate)
 		const unsigned zg = (unsigned)std::min<u64>(
 			std::max<u64>(1, (b->pool_amps + 255) / 256), (1ull << 31) / 256);
+		const bool zero = b->export_states != 0;
 		if (b->dtype == DK_C64)
-			hipLaunchKernelGGL(k_zero_pool<float2>, dim3(zg), dim3(256), 0, st,
-					   (float2 *)b->pool, b->pool_amps);
+			hipLaunchKernelGGL(zero ? k_zero_pool<float2> : k_poison_pool<float2>,
+					   dim3(zg), dim3(256), 0, st, (float2 *)b->pool,
+					   b->pool_amps);
 		else
-			hipLaunchKernelGGL(k_zero_pool<double2>, dim3(zg), dim3(256), 0, st,
-					   (double2 *)b->pool, b->pool_amps);
+			hipLaunchKernelGGL(zero ? k_zero_pool<double2> : k_poison_pool<double2>,
+					   dim3(zg), dim3(256), 0, st, (double2 *)b->pool,
+					   b->pool_amps);
 		DK_LAUNCHED();
 	}
 	for (size_t r = 0; r < b->round_items.size(); ++r) {

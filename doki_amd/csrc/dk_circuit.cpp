@@ -849,6 +849,7 @@ void build_pass(dk_circuit *c, std::vector<unsigned> chosen,
 			if (!std::isfinite(c->pops[i].m[j]))
 				pp.desc.finite = 0;
 	pp.desc.zmask = 0;
+	pp.desc.kmask = 0;
 	c->passes.push_back(pp);
 	c->plan.push_back({true, (int)c->passes.size() - 1});
 }

@@ -78,6 +78,11 @@ struct PassDesc {
 	// without reading the tile or running the ops
 	uint32_t zwrite;
 	uint64_t zmask;
+	// kmask: index bits known to be zero in the input where the input
+	// memory need not hold those zeros (the batched engine's pool is not
+	// cleared): an amplitude whose index meets kmask is taken as zero
+	// instead of read.  0 reads every amplitude of a tile it runs
+	uint64_t kmask;
 };
 
 // One register's pass within a round of batched passes: its tiles are the

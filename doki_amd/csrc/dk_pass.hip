@@ -466,6 +466,30 @@ __device__ __forceinline__ void tile_fetch(const T *in, u64 base,
 	}
 }
 
+// tile_fetch taking as zero every amplitude whose index meets kmask (index
+// bits known to be zero, whose memory may never have been written): such
+// amplitudes are not read.  The register part of an index is the same for
+// the whole workgroup, so the per-register test is uniform.
+template <typename T, typename R>
+__device__ __forceinline__ void tile_fetch_known(const T *in, u64 base,
+						 const u64 (&hb)[HIGH_BITS], unsigned lane,
+						 unsigned w, u64 kmask, R (&xr)[REGS],
+						 R (&xi)[REGS])
+{
+	const u64 tb = base | lane | slot_index(w << REG_BITS, hb);
+	const bool none = (tb & kmask) != 0;
+#pragma unroll
+	for (int r = 0; r < REGS; ++r) {
+		const u64 ri = slot_index((unsigned)r, hb);
+		xr[r] = xi[r] = (R)0;
+		if (!none && !(ri & kmask)) {
+			const T x = ld_stream(in + (tb | ri));
+			xr[r] = x.x;
+			xi[r] = x.y;
+		}
+	}
+}
+
 // Zeros into a thread's 16 amplitudes of the tile at `base` (load mapping).
 template <typename T>
 __device__ __forceinline__ void tile_zero(T *out, u64 base, const u64 (&hb)[HIGH_BITS],
@@ -687,7 +711,9 @@ __attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass(
 // tile it - first of the entry holding it (entries ascend by `first`);
 // workgroups take items blockIdx.x, blockIdx.x + gridDim.x, ...  The launch
 // boundary orders one round's passes before the next round's.  `init`: the
-// first round, whose tiles start from |0...0> instead of the pool.  No
+// first round, whose tiles start from |0...0> instead of the pool.  An
+// entry's kmask marks the amplitudes its register has not written yet (the
+// pool is not cleared): they are zeros, taken as such without a read.  No
 // renormalisation between passes (the caller normalises at the end).
 template <typename T>
 __global__ __launch_bounds__(THREADS)
@@ -733,6 +759,8 @@ __attribute__((amdgpu_waves_per_eu(pass_waves<T>()))) void k_pass_round(
 			// amplitude 0: tile 0, lane 0, wave 0, register 0
 			if (base == 0 && lane == 0 && w == 0)
 				vr[0] = (R)1;
+		} else if (pd.kmask) {
+			tile_fetch_known(st, base, hb, lane, w, pd.kmask, vr, vi);
 		} else {
 			tile_fetch(st, base, hb, lane, w, vr, vi);
 		}
@@ -799,6 +827,8 @@ int dk_launch_pass(const dk_state *in, dk_state *out, const PassDesc &pd,
 	d.zmask = pd.finite ? (~in->support & valid) : 0;
 	// in place, the zeros are already there
 	d.zwrite = in != out;
+	// a state's memory holds its zeros: every amplitude of a tile is read
+	d.kmask = 0;
 	const u64 tmask = ((1ull << LANE_BITS) - 1) | seen;
 	double *parts = nullptr;
 	DK_TRY(dk_partials_reserve(ntiles, &parts));

@@ -380,7 +380,10 @@ void dk_batch_free(dk_batch *b);
  * "split_qubits" (registers of at least this many qubits, and larger than
  * one LDS tile, run their passes as grid-wide rounds before the persistent
  * launch; 14 by default, 0 = never; results agree to rounding either way,
- * the rounds merging neighbouring gates). */
+ * the rounds merging neighbouring gates), "poison_pool" (NaN over the
+ * device pool before each run; the pool is cleared only when states are
+ * exported, and no result reads an amplitude the run did not write, which
+ * this setting lets a test show). */
 int dk_batch_set(dk_batch *b, const char *key, int64_t value);
 
 /* Drops every register and keeps the device and pinned buffers for the next

@@ -96,3 +96,33 @@ def test_full_batch_sample_vs_oracle(capi, oracle):
             rho = b.density(i)
             assert np.max(np.abs(rho - reduced_density_einsum(want, keep))) <= ATOL
         print(f"[batch-full] register {i} ({n} qubits) matches the oracle", flush=True)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_full_batch_unwritten_pool_never_read(capi, dtype):
+    """Without exported states the pool is not cleared: the rounds leave
+    every amplitude outside a register's support unwritten and everything
+    after them takes those as zeros without reading them.  With NaN over the
+    pool before each run ("poison_pool") every result equals, bit for bit,
+    that of a run whose pool was cleared (exported states), in a graph
+    replay too."""
+    dt = capi.C128 if dtype == "c128" else capi.C64
+    sp = specs()
+    runs = []
+    for settings in (dict(export_states=1), dict(poison_pool=1)):
+        b = capi.Batch(dt, probs=1, use_graph=1, **settings)
+        for n, steps, rolls, keep in sp:
+            b.add(n, steps, rolls, keep)
+        b.run()
+        b.run()
+        runs.append(b)
+    clean, poisoned = runs
+    for i, (n, steps, rolls, keep) in enumerate(sp):
+        o0, p0 = clean.outcomes(i)
+        o1, p1 = poisoned.outcomes(i)
+        assert o1 == o0 and np.array_equal(p1, p0), (i, n)
+        pr = poisoned.probs(i)
+        assert not np.isnan(pr).any() and np.array_equal(pr, clean.probs(i)), (i, n)
+        if keep:
+            assert np.array_equal(poisoned.density(i), clean.density(i)), (i, n)
+    print(f"[batch-full] {dtype}: poisoned pool, {len(sp)} registers equal", flush=True)
