@@ -467,21 +467,21 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 								(ix & R.zmask) ? czero<T>() : st[ix];
 						}
 						__syncthreads();
+						// a thread's entries share their column j (BT is a
+						// multiple of dim): y is read once per t
+						const int j = (int)threadIdx.x & (dim - 1);
+						for (int tt = 0; tt < nt; ++tt) {
+							const T y = tile[lds_pos<T>(tt * dim + j)];
+							const double yr = y.x, yi = y.y;
 #pragma unroll
-						for (int k = 0; k < EPR; ++k) {
-							const int e = e0 + (int)threadIdx.x + k * BT;
-							if (e < dim * dim) {
-								const int i = e / dim, j = e % dim;
-								double2 s = acc[k];
-								for (int tt = 0; tt < nt; ++tt) {
-									const T x = tile[lds_pos<T>(tt * dim + i)];
-									const T y = tile[lds_pos<T>(tt * dim + j)];
+							for (int k = 0; k < EPR; ++k) {
+								const int e = e0 + (int)threadIdx.x + k * BT;
+								if (e < dim * dim) {
+									const T x = tile[lds_pos<T>(tt * dim + (e >> a))];
 									const double xr = x.x, xi = x.y;
-									const double yr = y.x, yi = y.y;
-									s.x = fma(xr, yr, fma(xi, yi, s.x));
-									s.y = fma(xi, yr, fma(-xr, yi, s.y));
+									acc[k].x = fma(xr, yr, fma(xi, yi, acc[k].x));
+									acc[k].y = fma(xi, yr, fma(-xr, yi, acc[k].y));
 								}
-								acc[k] = s;
 							}
 						}
 						__syncthreads();
