@@ -187,7 +187,7 @@ def join_leg(n):
         capi.event_record(1)
         one.append(capi.event_elapsed_ms(0, 1))
         del j
-    assert np.array_equal(p1, p2)
+    assert np.max(np.abs(p1 - p2)) <= 1e-14    # the same sums, other orders
     written = 16 << n
     out = {"qubits": n, "join_then_prob_all_ms": round(min(two), 3),
            "one_pass_ms": round(min(one), 3),
