@@ -173,3 +173,37 @@ def test_batch_densities_scattered_keeps(capi, oracle, dtype_name):
         rho = b.density(i)
         assert rho.shape == (1 << len(keep), 1 << len(keep))
         assert np.max(np.abs(rho - reduced_density_einsum(want, keep))) <= tol, (i, n, keep)
+
+
+def test_batch_nonfinite_pass_over_unwritten_pool(capi, oracle):
+    """A gate with an infinite entry, controlled by a qubit no gate sets: as
+    the reference computes it, the entry multiplies the zero amplitudes under
+    the control (0 * inf is NaN) and the normalised state is NaN throughout.
+    Its pass in the rounds runs every tile (a non-finite pass skips none),
+    over a pool that is cleared (exported states) or full of NaN (not
+    cleared): both runs give the reference's result, NaN for NaN, and the
+    same outcomes, probabilities and densities."""
+    inf_gate = np.array([[np.inf, 0.0], [0.0, 1.0]], dtype=np.complex128)
+    specs = []
+    for i, (n, at) in enumerate(((15, 3), (16, 25), (17, 60), (14, 0))):
+        rng = W.rng_for(8800 + i)
+        steps = [W.random_gate_step(rng, n - 1) for _ in range(70)]
+        steps.insert(at, (inf_gate, [int(rng.integers(0, n - 1))], {n - 1}, set()))
+        specs.append((n, steps, rng.random(n).tolist()))
+    runs = []
+    for settings in (dict(export_states=1), dict(poison_pool=1)):
+        b = capi.Batch(capi.C128, probs=1, **settings)
+        for n, steps, rolls in specs:
+            b.add(n, steps, rolls, (0, 2, n - 1))
+        b.run()
+        runs.append(b)
+    clean, poisoned = runs
+    for i, (n, steps, rolls) in enumerate(specs):
+        want = oracle_run(oracle, oracle.new(n), steps).values()
+        assert np.isnan(want).all()
+        assert np.array_equal(clean.state(i), want, equal_nan=True), (i, n)
+        o0, p0 = clean.outcomes(i)
+        o1, p1 = poisoned.outcomes(i)
+        assert o1 == o0 and np.array_equal(p1, p0, equal_nan=True), (i, n)
+        assert np.array_equal(poisoned.probs(i), clean.probs(i), equal_nan=True), (i, n)
+        assert np.array_equal(poisoned.density(i), clean.density(i), equal_nan=True), (i, n)
