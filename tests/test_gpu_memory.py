@@ -68,6 +68,39 @@ def test_arena_chunks_move_between_sizes(capi):
         capi.tune("arena_max_chunks", 0)
 
 
+def test_whole_blocks_carved_and_merged(capi):
+    """A released 16 GiB block serves smaller whole-chunk states from its
+    front without new chunks (8, 4 and 4 GiB, distinct and intact side by
+    side); released again in any order, its parts merge back and serve a 16
+    GiB state once more, within the same chunks."""
+    check_ghz_like(capi, capi.new(30), 30)
+    cap = pinned_chunks(capi) + 8
+    capi.tune("arena_max_chunks", cap)
+    try:
+        a = capi.new(30)
+        del a
+        capi.synchronize()
+        assert capi.memory_stats()["cached_bytes"] == 16 << 30
+        parts = [capi.new(n) for n in (29, 28, 28)]
+        st = capi.memory_stats()
+        assert st["arena_chunks"] == cap and st["cached_bytes"] == 0
+        parts = [check_ghz_like(capi, s_, n) for s_, n in zip(parts, (29, 28, 28))]
+        # each still holds its own state after the others were written
+        for s_, n in zip(parts, (29, 28, 28)):
+            p = capi.prob_all(s_)
+            assert abs(p[0] - 0.5) <= 1e-12 and abs(p[n - 1] - 0.5) <= 1e-12
+            assert np.max(np.abs(p[1:n - 1])) <= 1e-12
+        for i in (1, 0, 2):
+            parts[i] = None
+        del s_
+        capi.synchronize()
+        assert capi.memory_stats()["cached_bytes"] == 16 << 30
+        check_ghz_like(capi, capi.new(30), 30)
+        assert capi.memory_stats()["arena_chunks"] == cap
+    finally:
+        capi.tune("arena_max_chunks", 0)
+
+
 def test_budget_refuses_and_recovers(capi):
     import ctypes as C
     L = capi.lib()
