@@ -1196,12 +1196,16 @@ static int batch_upload(dk_batch *b)
 	const u64 h_dens = al(std::max<u64>(1, b->dens_total) * sizeof(double2));
 	const u64 hb = 2 * h_roll + h_dens + al(std::max<u64>(1, nroll));
 	if (b->hbuf_bytes < hb) {
+		// grown geometrically from 8 MiB: hipHostFree waits for the whole
+		// device, every lane's work included
+		const u64 want = std::max<u64>(std::max<u64>(hb, 2 * b->hbuf_bytes),
+					       8ull << 20);
 		if (b->hbuf)
 			(void)hipHostFree(b->hbuf);
 		b->hbuf = nullptr;
 		b->hbuf_bytes = 0;
-		DK_HIP(hipHostMalloc((void **)&b->hbuf, hb, hipHostMallocDefault));
-		b->hbuf_bytes = hb;
+		DK_HIP(hipHostMalloc((void **)&b->hbuf, want, hipHostMallocDefault));
+		b->hbuf_bytes = want;
 	}
 	char *hp = b->hbuf;
 	memset(hp, 0, hb);
