@@ -125,6 +125,12 @@ struct dk_circuit {
 	dkp::PassOp *pops_dev = nullptr;
 	size_t pops_dev_bytes = 0;
 	dk_lane *pops_lane = nullptr;
+	// one-qubit gates of a planned circuit on a small state, for the
+	// one-workgroup kernel (device copy made on the first run)
+	std::vector<dk_small_gate> sgates;
+	dk_small_gate *sgates_dev = nullptr;
+	size_t sgates_dev_bytes = 0;
+	dk_lane *sgates_lane = nullptr;
 };
 
 extern "C" int dk_circuit_new(unsigned num_qubits, dk_circuit **out)
@@ -149,6 +155,15 @@ static void drop_device_gates(dk_circuit *c)
 		c->pops_dev = nullptr;
 		c->pops_dev_bytes = 0;
 	}
+	if (c->sgates_dev) {
+		if (!dk_closed()) {
+			(void)hipStreamSynchronize(c->sgates_lane->stream);
+			dk_device_free(c->sgates_dev, c->sgates_dev_bytes);
+		}
+		c->sgates_dev = nullptr;
+		c->sgates_dev_bytes = 0;
+	}
+	c->sgates.clear();
 	for (auto &b : c->blocks)
 		if (b.dev) {
 			dk_gate_free(b.dev);
@@ -1163,7 +1178,94 @@ extern "C" int dk_circuit_run(dk_circuit *c, dk_state *st, int in_place,
 		first = false;
 		return (int)DK_OK;
 	};
-	if (c->passes_planned) {
+	// A planned circuit on a state smaller than a pass tile (where the plan
+	// runs every gate alone): every run of one-qubit gates in one launch of
+	// one workgroup holding the state in LDS; multi-qubit gates alone
+	// between.  (At 12 qubits the one-tile passes measured faster: 200
+	// gates in 0.34 against 0.40 ms through the drop-in module.)
+	auto small_run = [&]() {
+		if (c->sgates.empty()) {
+			for (const CGate &g : c->gates) {
+				if (g.k != 1)
+					continue;
+				dk_small_gate sg;
+				memset(&sg, 0, sizeof(sg));
+				for (int j = 0; j < 4; ++j) {
+					sg.m[2 * j] = g.m[(size_t)j].real();
+					sg.m[2 * j + 1] = g.m[(size_t)j].imag();
+				}
+				sg.cm = g.cm;
+				sg.am = g.am;
+				sg.t = g.t[0];
+				c->sgates.push_back(sg);
+			}
+		}
+		if (!c->sgates.empty() && !c->sgates_dev) {
+			const size_t bytes = c->sgates.size() * sizeof(dk_small_gate);
+			int r = dk_device_alloc(bytes, (void **)&c->sgates_dev);
+			if (r)
+				return r;
+			c->sgates_dev_bytes = bytes;
+			c->sgates_lane = dk_cur();
+			// the host records stay until the device copy is dropped,
+			// which waits for this lane first
+			hipError_t e = hipMemcpyAsync(c->sgates_dev, c->sgates.data(), bytes,
+						      hipMemcpyHostToDevice, DK_STREAM);
+			if (e != hipSuccess) {
+				dk_set_error("gate upload failed: %s", hipGetErrorString(e));
+				return (int)DK_E_DEVICE;
+			}
+		} else if (c->sgates_dev && c->sgates_lane != dk_cur()) {
+			DK_HIP(hipStreamSynchronize(c->sgates_lane->stream));
+			c->sgates_lane = dk_cur();
+		}
+		size_t si = 0;
+		for (size_t gi = 0; gi < c->gates.size();) {
+			CGate &g = c->gates[gi];
+			if (g.k != 1) {
+				if (!g.dev) {
+					int r = gate_to_device(g.k, g.m, &g.dev);
+					if (r)
+						return r;
+				}
+				int r = step(g.dev, g.t, g.cm, g.am);
+				if (r)
+					return r;
+				++gi;
+				continue;
+			}
+			size_t n1 = 0;
+			while (gi + n1 < c->gates.size() && c->gates[gi + n1].k == 1)
+				++n1;
+			dk_state *o = cur;
+			if (first) {
+				int r = dk_alloc_state(cur->nq, cur->dtype, &o);
+				if (r)
+					return r;
+			}
+			int r = dk_time_mark(0);
+			if (!r)
+				r = dk_launch_small_run(cur, o, c->sgates_dev + si, (unsigned)n1);
+			if (!r)
+				r = dk_time_mark(1);
+			if (r) {
+				if (o != cur)
+					dk_release_state(o);
+				return r;
+			}
+			o->lane = dk_cur();
+			o->support = ~0ull;
+			cur = o;
+			first = false;
+			si += n1;
+			gi += n1;
+		}
+		return (int)DK_OK;
+	};
+	if (c->passes_planned && c->nq < (unsigned)dkp::PASS_TILE_BITS &&
+	    c->nq <= dk_small_qubits(st->dtype)) {
+		rc = small_run();
+	} else if (c->passes_planned) {
 		if (!c->pops.empty() && !c->pops_dev) {
 			const size_t bytes = c->pops.size() * sizeof(PassOp);
 			rc = dk_device_alloc(bytes, (void **)&c->pops_dev);

@@ -196,6 +196,24 @@ int dk_launch_apply(const dk_state *in, dk_state *out, const dk_gate *g,
 // Records that the calling lane's queued work reads gate g's device copy.
 int dk_gate_mark_use(const dk_gate *g);
 
+// A one-qubit gate of a circuit run whole in one workgroup's LDS.
+struct dk_small_gate {
+	double m[8];      // 2x2 row-major, interleaved re/im
+	u64 cm, am;       // control / anticontrol index bits
+	unsigned t;       // target qubit
+	unsigned pad;
+};
+
+// Most qubits of a state one workgroup holds in LDS (64 KiB of amplitudes).
+unsigned dk_small_qubits(int dtype);
+
+// `ngates` one-qubit gates (device array) on `in`, into `out` (may be `in`),
+// in one launch of one workgroup: the state is read through its norm, every
+// gate applied in order in LDS, the result written with its norm.  For
+// states of at most dk_small_qubits(dtype) qubits.
+int dk_launch_small_run(const dk_state *in, dk_state *out,
+			const dk_small_gate *gates, unsigned ngates);
+
 // Dense k-qubit gate (4 <= k <= 6, no controls) on MFMA, for states of at
 // least dk_dense_tile_bits(k) qubits (the index bits one block covers).
 unsigned dk_dense_tile_bits(unsigned k, int dtype);

@@ -1521,6 +1521,84 @@ int dk_launch_collapse(const dk_state *s, unsigned q, const double *dec_slot,
 	return DK_OK;
 }
 
+// A whole circuit of one-qubit gates on a small state in one workgroup: the
+// state in LDS (read through its norm), one sweep over the index pairs per
+// gate with a barrier between gates (the arithmetic of the one-qubit
+// kernels), then written back with the norm of the result.  One launch
+// instead of two (gate and norm) per gate.
+constexpr int SMALL_THREADS = 1024;
+constexpr int SMALL_LDS = 65536;
+
+template <typename T>
+__global__ __launch_bounds__(SMALL_THREADS) void k_small_run(
+	const T *in, const double *__restrict__ in_scal, T *out, double *out_scal,
+	const dk_small_gate *__restrict__ gates, unsigned ngates, unsigned nq)
+{
+	typedef typename real_of<T>::type R;
+	__shared__ T st[SMALL_LDS / sizeof(T)];
+	__shared__ double red[SMALL_THREADS / 64];
+	const unsigned len = 1u << nq, half = len >> 1;
+	const R inv = (R)(1.0 / in_scal[DK_SCAL_NORM]);
+	for (unsigned i = threadIdx.x; i < len; i += SMALL_THREADS)
+		st[i] = cscale(in[i], inv);
+	__syncthreads();
+	for (unsigned gi = 0; gi < ngates; ++gi) {
+		const dk_small_gate &g = gates[gi];
+		const unsigned t = g.t;
+		const u64 cm = g.cm, am = g.am;
+		const T m00 = mk<T>((R)g.m[0], (R)g.m[1]), m01 = mk<T>((R)g.m[2], (R)g.m[3]);
+		const T m10 = mk<T>((R)g.m[4], (R)g.m[5]), m11 = mk<T>((R)g.m[6], (R)g.m[7]);
+		for (unsigned p = threadIdx.x; p < half; p += SMALL_THREADS) {
+			const unsigned i0 = ((p >> t) << (t + 1)) | (p & ((1u << t) - 1));
+			const unsigned i1 = i0 | (1u << t);
+			if (((u64)i0 & cm) == cm && ((u64)i0 & am) == 0) {
+				const T x = st[i0], y = st[i1];
+				st[i0] = cmac(cmac(czero<T>(), m00, x), m01, y);
+				st[i1] = cmac(cmac(czero<T>(), m10, x), m11, y);
+			}
+		}
+		__syncthreads();
+	}
+	double acc = 0.0;
+	for (unsigned i = threadIdx.x; i < len; i += SMALL_THREADS) {
+		out[i] = st[i];
+		acc += cabs2d(st[i]);
+	}
+	acc = wave_sum(acc);
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double s = 0.0;
+		for (int w = 0; w < SMALL_THREADS / 64; ++w)
+			s += red[w];
+		out_scal[DK_SCAL_NORM] = sqrt(s);
+	}
+}
+
+unsigned dk_small_qubits(int dtype)
+{
+	return dtype == DK_C64 ? 13 : 12;
+}
+
+int dk_launch_small_run(const dk_state *in, dk_state *out,
+			const dk_small_gate *gates, unsigned ngates)
+{
+	if (in->nq > dk_small_qubits(in->dtype) || out->nq != in->nq ||
+	    out->dtype != in->dtype) {
+		dk_set_error("one-workgroup circuit on a state of %u qubits", in->nq);
+		return DK_E_ARG;
+	}
+#define L(T)                                                                  \
+	hipLaunchKernelGGL(k_small_run<T>, dim3(1), dim3(SMALL_THREADS), 0,       \
+			   DK_STREAM, (const T *)in->amp, in->scal, (T *)out->amp, \
+			   out->scal, gates, ngates, in->nq)
+	DK_BY_TYPE(in, L);
+#undef L
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
 int dk_launch_join(const dk_state *a, const dk_state *b, dk_state *o)
 {
 	const unsigned nb = grid_full(o->len, TPB);

@@ -210,3 +210,32 @@ def test_zero_tiles_after_a_non_finite_lone_gate(capi):
     fresh = circ.run(capi.new(n), in_place=True).raw()[0]
     dense = circ.run(zero_state_without_support(capi, n, capi.C128), in_place=True).raw()[0]
     assert np.array_equal(fresh, dense, equal_nan=True)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 11])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_small_circuits_in_one_workgroup(capi, oracle, n, dtype):
+    """Below the pass tile (12 qubits) a planned circuit runs each run of
+    one-qubit gates in one launch of one workgroup (the state in LDS), lone
+    multi-qubit gates between: against the oracle, out of place (the input
+    untouched) and in place, on an input whose norm is not 1."""
+    dt = capi.C128 if dtype == "c128" else capi.C64
+    tol = ATOL if dt == capi.C128 else C64_TOL
+    steps, _ = W.random_circuit(3300 + n, n, 90)
+    if n >= 3:
+        u = W.haar_unitary(W.rng_for(n), 2)
+        steps.insert(40, (u, [n - 1, 0], set(), set()))
+        steps.insert(70, (u, [1, n - 1], set(), set()))
+    v = W.random_normalised_state(3400 + n, n) * 2.5
+    want = oracle_run(oracle, oracle.from_data(n, v), steps).values()
+    circ = capi.Circuit(n, steps)
+    circ.passes()
+    src = capi.from_host(v)
+    if dt == capi.C64:
+        src = capi.convert(src, dt)
+    before = src.values()
+    out = circ.run(src, in_place=False)
+    assert maxabs(out.values(), want) <= tol
+    assert np.array_equal(src.values(), before)
+    again = circ.run(src, in_place=True)
+    assert again is src and maxabs(src.values(), want) <= tol
