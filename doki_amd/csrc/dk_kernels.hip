@@ -1347,12 +1347,13 @@ int dk_launch_prob(const dk_state *s, unsigned q, double *dst)
 // that none takes 2^PROB_KB chunks; a power of two, 2^wbits waves.  The
 // join-and-marginals launch uses the same grid, so its sums are made in the
 // same order.
-static unsigned prob_all_grid(u64 len, int lu, unsigned *wbits)
+// cap: most blocks (a power of two), 0 for four times the grid-stride cap.
+static unsigned prob_all_grid(u64 len, int lu, unsigned *wbits, u64 cap = 0)
 {
 	static_assert((WAVES & (WAVES - 1)) == 0, "waves per block: a power of two");
 	const u64 nch = len >> (6 + lu);
 	u64 nb = std::min<u64>(std::max<u64>(1, nch / WAVES),
-			       4ull * grid_for(~0ull >> 1, 1));
+			       cap ? cap : 4ull * grid_for(~0ull >> 1, 1));
 	nb = std::max<u64>(nb, (nch >> PROB_KB) / WAVES);
 	*wbits = (unsigned)__builtin_ctzll(nb * WAVES);
 	return (unsigned)nb;
@@ -1373,6 +1374,12 @@ static void launch_finish_marginals(const double *parts, unsigned nq,
 // different orders and agree to rounding.
 constexpr int PROB_LU = 3;
 constexpr int JOIN_PROB_LU = 1;
+// Blocks of the one-pass join: with the marginals kernel's 8192 its write
+// rate depended on where the state sat in physical memory (32 qubits: 13.0
+// ms in a fresh process, 10.9 ms after the arena gave its chunks back and
+// made new ones); with 2^17 blocks (64 loop steps per wave) 10.7-10.8 ms
+// either way.  Partials: 2^17 x 40 doubles.
+constexpr u64 JOIN_PROB_BLOCKS = 1ull << 17;
 
 int dk_launch_prob_all(const dk_state *s, double *dst)
 {
@@ -1410,7 +1417,7 @@ int dk_launch_join_prob_all(const dk_state *a, const dk_state *b, dk_state *o,
 		return rc ? rc : dk_launch_prob_all(o, dst);
 	}
 	unsigned wbits;
-	const unsigned nb = prob_all_grid(o->len, LU, &wbits);
+	const unsigned nb = prob_all_grid(o->len, LU, &wbits, JOIN_PROB_BLOCKS);
 	double *parts;
 	DK_TRY(dk_partials_reserve((u64)nb * PQ, &parts));
 #define L(T)                                                                  \
