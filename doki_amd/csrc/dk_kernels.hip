@@ -1770,6 +1770,31 @@ int dk_launch_fill_zero_state(dk_state *s)
 	return DK_OK;
 }
 
+// All-ones bytes (a NaN in every double and float) over a new allocation:
+// the "poison_alloc" setting, which makes a read of memory no kernel wrote
+// show in the results.  Words first, then the bytes of a ragged tail.
+__global__ __launch_bounds__(TPB) void k_poison_bytes(unsigned *__restrict__ w, u64 words,
+						      unsigned char *__restrict__ tail,
+						      unsigned ntail)
+{
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < words;
+	     i += (u64)gridDim.x * TPB)
+		w[i] = 0xffffffffu;
+	if (blockIdx.x == 0 && threadIdx.x < ntail)
+		tail[threadIdx.x] = 0xff;
+}
+
+int dk_launch_poison(void *p, u64 bytes)
+{
+	const u64 words = bytes / 4;
+	const unsigned nb = grid_full(std::max<u64>(words, 1), TPB);
+	hipLaunchKernelGGL(k_poison_bytes, dim3(nb), dim3(TPB), 0, DK_STREAM,
+			   (unsigned *)p, words, (unsigned char *)p + 4 * words,
+			   (unsigned)(bytes & 3));
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
 int dk_launch_set_norm(dk_state *s, double v)
 {
 	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, DK_STREAM,

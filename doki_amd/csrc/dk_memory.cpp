@@ -281,6 +281,11 @@ void merge_mapped(void *a, void *b)
 // part's release.
 void cache_whole(Cached c, u64 bytes)
 {
+	if (!g_ctx.arena_reuse) {
+		g_whole_at[(u64)c.p] = bytes;
+		g_whole_free.emplace(bytes, std::move(c));
+		return;
+	}
 	void *resv = g_mapped.at(c.p).resv;
 	auto left = g_whole_at.lower_bound((u64)c.p);
 	if (left != g_whole_at.begin()) {
@@ -373,7 +378,7 @@ int alloc_whole(u64 bytes, void **out)
 	// the front of the smallest larger cached block, without mapping: the
 	// rest stays cached and still waits for the block's last users (this
 	// lane's stream waits for them too)
-	auto big = g_whole_free.upper_bound(bytes);
+	auto big = g_ctx.arena_reuse ? g_whole_free.upper_bound(bytes) : g_whole_free.end();
 	if (big != g_whole_free.end()) {
 		const u64 total = big->first;
 		Cached c = std::move(big->second);
@@ -508,7 +513,9 @@ u64 dk_block_bytes(u64 bytes)
 	return (bytes + CHUNK - 1) / CHUNK * CHUNK;
 }
 
-int dk_device_alloc(u64 bytes, void **out)
+namespace {
+
+int device_alloc(u64 bytes, void **out)
 {
 	*out = nullptr;
 	if (bytes < MID_BLOCK) {
@@ -527,6 +534,16 @@ int dk_device_alloc(u64 bytes, void **out)
 	const u64 b = dk_block_bytes(bytes);
 	std::lock_guard<std::mutex> lk(g_mem_mu);
 	return b >= CHUNK ? alloc_whole(b, out) : alloc_piece(b, out);
+}
+
+} // namespace
+
+int dk_device_alloc(u64 bytes, void **out)
+{
+	const int rc = device_alloc(bytes, out);
+	if (!rc && g_ctx.poison_alloc)
+		return dk_launch_poison(*out, bytes);
+	return rc;
 }
 
 void dk_device_free(void *p, u64 bytes)

@@ -49,22 +49,61 @@ dk_lane *dk_cur()
 	return t_lane ? t_lane : &g_ctx.lane0;
 }
 
-// Stream, scratch, pinned staging and timing events of a lane.
+// Stream, scratch, pinned staging and timing events of a lane; on failure
+// nothing stays allocated.
+static int lane_setup_once(dk_lane *l)
+{
+	hipStream_t st = nullptr;
+	hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+	// DK_PARTIALS doubles: a pool-sized block
+	if (e == hipSuccess)
+		e = hipMallocAsync((void **)&l->partials, DK_PARTIALS * sizeof(double), st);
+	if (e == hipSuccess)
+		e = hipMalloc((void **)&l->dec, 2 * 64 * sizeof(double) + 2048);
+	if (e == hipSuccess)
+		e = hipHostMalloc((void **)&l->host_pinned, DK_PINNED * sizeof(double),
+				  hipHostMallocDefault);
+	for (auto &ev : l->events)
+		if (e == hipSuccess)
+			e = hipEventCreate(&ev);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(st);
+	if (e == hipSuccess) {
+		l->stream = st;
+		l->partials_cap = DK_PARTIALS;
+		return DK_OK;
+	}
+	dk_set_error("lane setup failed: %s", hipGetErrorString(e));
+	(void)hipGetLastError();
+	for (auto &ev : l->events)
+		if (ev) {
+			(void)hipEventDestroy(ev);
+			ev = nullptr;
+		}
+	if (l->host_pinned)
+		(void)hipHostFree(l->host_pinned);
+	if (l->dec)
+		(void)hipFree(l->dec);
+	if (l->partials && st)
+		(void)hipFreeAsync(l->partials, st);
+	if (st) {
+		(void)hipStreamSynchronize(st);
+		(void)hipStreamDestroy(st);
+	}
+	l->host_pinned = nullptr;
+	l->dec = l->partials = nullptr;
+	return DK_E_DEVICE;
+}
+
+// The arena may hold nearly all device memory in cached blocks: when a
+// lane's scratch does not fit, they go back to the driver and setup runs
+// once more.
 int dk_lane_setup(dk_lane *l)
 {
-	hipStream_t st;
-	DK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-	DK_HIP(hipMallocAsync((void **)&l->partials, DK_PARTIALS * sizeof(double),
-			      st));  // DK_PARTIALS doubles: a pool-sized block
-	l->partials_cap = DK_PARTIALS;
-	DK_HIP(hipMalloc((void **)&l->dec, 2 * 64 * sizeof(double) + 2048));
-	DK_HIP(hipHostMalloc((void **)&l->host_pinned, DK_PINNED * sizeof(double),
-			     hipHostMallocDefault));
-	for (auto &ev : l->events)
-		DK_HIP(hipEventCreate(&ev));
-	DK_HIP(hipStreamSynchronize(st));
-	l->stream = st;
-	return DK_OK;
+	if (lane_setup_once(l) == DK_OK)
+		return DK_OK;
+	(void)dk_memory_trim();
+	return lane_setup_once(l);
 }
 
 extern "C" int dk_init(int device)
@@ -97,7 +136,9 @@ extern "C" int dk_init(int device)
 	double *sp = nullptr;
 	DK_HIP(hipGetSymbolAddress((void **)&sp, HIP_SYMBOL(scratch)));
 	g_ctx.noalloc_scratch = sp;
-	int rc = dk_lane_setup(&g_ctx.lane0);
+	// (nothing to hand back yet: the arena is empty, and trimming it would
+	// take this lock again)
+	int rc = lane_setup_once(&g_ctx.lane0);
 	if (rc)
 		return rc;
 	hipMemPool_t pool;
@@ -1283,6 +1324,18 @@ extern "C" int dk_tune(const char *key, int64_t value)
 	if (!strcmp(key, "arena_max_chunks")) {
 		g_ctx.arena_max_chunks = value > 0 ? (u64)value
 						   : g_ctx.arena_default_chunks;
+		return DK_OK;
+	}
+	if (!strcmp(key, "poison_alloc")) {
+		g_ctx.poison_alloc = value != 0;
+		return DK_OK;
+	}
+	if (!strcmp(key, "arena_reuse")) {
+		if (value < 0 || value > 1) {
+			dk_set_error("arena_reuse is 0 (exact sizes) or 1 (carve and merge)");
+			return DK_E_ARG;
+		}
+		g_ctx.arena_reuse = (int)value;
 		return DK_OK;
 	}
 	dk_set_error("unknown tunable %s", key);

@@ -172,7 +172,8 @@ int run_circuit(dk_server *S, dk_state *st, const SReg &r)
 					 r.am.data());
 	if (!rc) {
 		const unsigned n = dk_state_num_qubits(st);
-		if (S->plan < 0 && n >= 12)
+		// (below 12 qubits the plan runs in one workgroup, one launch)
+		if (S->plan < 0)
 			rc = dk_circuit_passes(c, nullptr);
 		else if (S->plan >= 3 && n >= (unsigned)S->plan + 6)
 			rc = dk_circuit_fuse(c, (unsigned)S->plan, nullptr);
@@ -319,11 +320,12 @@ int run_large(dk_server *S, SReq *q)
 	return rc;
 }
 
-void worker(dk_server *S)
+// Every thread of a server runs on a lane of its own, made by dk_server_new
+// (a thread without one would share the default lane, its stream and its
+// scratch, with any other such thread).
+void worker(dk_server *S, dk_lane *lane)
 {
-	dk_lane *lane = nullptr;
-	if (dk_lane_new(&lane) == DK_OK)
-		dk_lane_bind(lane);
+	dk_lane_bind(lane);
 	for (;;) {
 		SReq *q;
 		{
@@ -349,11 +351,9 @@ void worker(dk_server *S)
 		dk_lane_free(lane);
 }
 
-void batcher(dk_server *S)
+void batcher(dk_server *S, dk_lane *lane)
 {
-	dk_lane *lane = nullptr;
-	if (dk_lane_new(&lane) == DK_OK)
-		dk_lane_bind(lane);
+	dk_lane_bind(lane);
 	dk_batch *b = nullptr;
 	int brc = dk_batch_new(S->dtype, &b);
 	if (!brc)
@@ -473,13 +473,23 @@ extern "C" int dk_server_new(int workers, unsigned small_qubits, int dtype,
 	rc = dk_memory_budget(&lib_budget, &lib_used);
 	if (rc)
 		return rc;
+	std::vector<dk_lane *> lanes((size_t)workers + 1, nullptr);
+	for (auto &l : lanes) {
+		rc = dk_lane_new(&l);
+		if (rc) {
+			for (auto &m : lanes)
+				if (m)
+					dk_lane_free(m);
+			return rc;
+		}
+	}
 	dk_server *S = new dk_server;
 	S->dtype = dtype;
 	S->small_qubits = std::min(small_qubits, 26u);
 	S->budget = budget ? budget : lib_budget - std::min(lib_budget, lib_used);
-	S->threads.emplace_back(batcher, S);
+	S->threads.emplace_back(batcher, S, lanes[0]);
 	for (int i = 0; i < workers; ++i)
-		S->threads.emplace_back(worker, S);
+		S->threads.emplace_back(worker, S, lanes[(size_t)i + 1]);
 	*out = S;
 	return DK_OK;
 }

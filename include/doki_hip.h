@@ -539,7 +539,13 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
  * (cap on the 2 GiB chunks of the state arena; 0 restores the default, the
  * free memory at init less 6 GiB), "arena_sub_mib" (chunks created from
  * then on are made of physical pieces of this many MiB, a power of two,
- * mapped in a scrambled order; 0, the default, one piece per chunk). */
+ * mapped in a scrambled order; 0, the default, one piece per chunk),
+ * "arena_reuse" (1, the default: whole-chunk blocks are carved from larger
+ * cached blocks and neighbouring cached blocks merge; 0: only exact sizes
+ * are reused), "poison_alloc" (1: every device block the library hands out,
+ * states, scratch and gate matrices, is first filled with all-ones bytes, a
+ * NaN in every amplitude, so that a read of memory nothing wrote shows in
+ * the results; a testing aid, 0 by default). */
 int dk_tune(const char *key, int64_t value);
 /* Compute units and architecture name of the library's device. */
 int dk_device_info(int *num_cus, char *arch, size_t arch_len);

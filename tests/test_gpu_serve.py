@@ -32,13 +32,19 @@ def oracle_result(O, spec):
     return {"rho": reduced_density_einsum(st.values(), spec["keep"])}
 
 
-def assert_same(a, b):
+def assert_same(a, b, what=""):
     assert a.keys() == b.keys()
     for k in a:
         if k == "outcomes":
-            assert a[k] == b[k]
+            assert a[k] == b[k], (what, k)
         else:
-            assert np.array_equal(a[k], b[k]), k
+            x, y = np.asarray(a[k]), np.asarray(b[k])
+            if not np.array_equal(x, y):
+                idx = np.nonzero(x != y)[0]
+                raise AssertionError(
+                    f"{what} {k}: {idx.size} entries differ, max |diff| "
+                    f"{np.max(np.abs(x - y)):.3e}, at {idx[:8].tolist()}: "
+                    f"{x[idx[:4]].tolist()} against {y[idx[:4]].tolist()}")
 
 
 def assert_close(got, want, spec):
@@ -125,7 +131,8 @@ def test_serve_alone_equals_mix_and_oracle(capi, oracle):
         mix = {h.spec["rid"]: h.wait(0) for h in hs}
         assert srv.batches >= 1 and srv.batched >= 1
     for s in specs:
-        assert_same(alone[s["rid"]], mix[s["rid"]])
+        assert_same(alone[s["rid"]], mix[s["rid"]],
+                    f"request {s['rid']} ({s['op']}, {[r[0] for r in s['regs']]} qubits) alone vs mix:")
         assert_close(mix[s["rid"]], oracle_result(oracle, s), s)
 
 
@@ -219,7 +226,8 @@ def test_serve_large_band_and_big_joins(capi):
         hs, _ = serve.serve_trace(srv, specs, time_scale=0.0)
         mix = {h.spec["rid"]: h.wait(0) for h in hs}
     for s in specs:
-        assert_same(alone[s["rid"]], mix[s["rid"]])
+        assert_same(alone[s["rid"]], mix[s["rid"]],
+                    f"request {s['rid']} ({s['op']}, {[r[0] for r in s['regs']]} qubits) alone vs mix:")
     print("[serve-large] alone == mix", flush=True)
 
     mixed = [s for s in specs if kinds[s["rid"]] == "mixed"]
