@@ -179,3 +179,63 @@ def test_scrambled_chunk_pieces_are_invisible(capi):
     del plain, scrambled, st, ref
     capi.synchronize()
     capi.memory_trim()
+
+
+def _poison_workload(capi):
+    """Each path once on fresh device blocks: passes (from |0...0>, so tiles
+    outside the support are skipped, and on a dense input), per-gate
+    execution with a dense 5-qubit gate, marginals, a measurement of every
+    qubit, a join with its marginals, a reduced density, and a batch."""
+    rng = W.rng_for(5150)
+    out = {}
+    steps = W.random_circuit(5151, 20, 120)[0]
+    st = capi.new(20)
+    c = capi.Circuit(20, steps)
+    c.passes()
+    c.run(st, in_place=True)
+    out["passes_zero"] = st.values()
+    c.run(st, in_place=True)
+    out["passes_dense"] = st.values()
+    g = capi.Gate(W.haar_unitary(rng, 5))
+    b = capi.apply(st, g, [3, 9, 14, 1, 18])
+    for m, t, cc, a in steps[:40]:
+        b = capi.apply(b, capi.Gate(m), t, cc, a)
+    out["gates"] = b.values()
+    out["probs"] = capi.prob_all(b)
+    left, outs, p1 = capi.measure(b, (1 << 20) - 1, rng.random(20).tolist())
+    out["outcomes"] = np.array(outs, dtype=float)
+    out["p1"] = p1
+    x = capi.new(11)
+    for m, t, cc, a in W.random_circuit(5152, 11, 50)[0]:
+        x = capi.apply(x, capi.Gate(m), t, cc, a)
+    j, pj = capi.join_prob_all(b, x)
+    out["join_probs"] = pj
+    out["join_head"] = j.raw(0, 4096)[0]
+    out["rho"] = capi.reduced_density(b, [0, 7, 19]).ravel()
+    bt = capi.Batch(probs=1)
+    for i in range(64):
+        n = 6 + i % 9
+        bt.add(n, W.random_circuit(5200 + i, n, 30)[0], density_keep=(0, n - 1))
+    bt.run()
+    for i in range(0, 64, 7):
+        out[f"batch_probs{i}"] = bt.probs(i)
+        out[f"batch_rho{i}"] = bt.density(i).ravel()
+    capi.synchronize()
+    return out
+
+
+def test_poisoned_allocations_change_no_result(capi):
+    """With every block the library hands out first filled with NaN bytes
+    (states, scratch, gate matrices, plans), no result changes by a bit: no
+    kernel on these paths reads memory that it or an earlier step did not
+    write."""
+    want = _poison_workload(capi)
+    capi.memory_trim()
+    capi.tune("poison_alloc", 1)
+    try:
+        got = _poison_workload(capi)
+    finally:
+        capi.tune("poison_alloc", 0)
+    for k in want:
+        assert np.isfinite(got[k]).all(), k
+        assert np.array_equal(got[k], want[k]), k
