@@ -131,6 +131,7 @@ struct dk_ctx {
 	u64 arena_sub_bytes = 0;  // physical pieces of an arena chunk (0: one piece)
 	int arena_reuse = 1;      // whole-chunk blocks: carve and merge (0: exact sizes)
 	int poison_alloc = 0;     // all-ones bytes over every block handed out
+	int guard_alloc = 0;      // guard bands around pool blocks, checked at release
 	u64 arena_default_chunks = 0; // the cap at init: free memory less 6 GiB
 	// device address of the file-scope scratch array of dk_runtime.hip
 	// (block partials of dk_apply_noalloc), taken once at init with the
@@ -267,4 +268,6 @@ int dk_launch_density_elements(const dk_state *s, u64 keep_mask, size_t count,
 int dk_launch_fill_zero_state(dk_state *s);
 int dk_launch_set_norm(dk_state *s, double v);
 int dk_launch_poison(void *p, u64 bytes);
+int dk_launch_guard_fill(void *g, u64 bytes);
+int dk_launch_guard_check(const void *g, u64 bytes, unsigned *bad);
 int dk_launch_widen(const dk_state *s, u64 first, u64 count, double2 *dst);

@@ -1795,6 +1795,44 @@ int dk_launch_poison(void *p, u64 bytes)
 	return DK_OK;
 }
 
+// Guard bands around a block (the "guard_alloc" setting): a known word
+// pattern before and after it, counted back when the block is released.
+constexpr unsigned GUARD_WORD = 0xa5c3e1f7u;
+
+__global__ __launch_bounds__(TPB) void k_guard_fill(unsigned *__restrict__ g, u64 words)
+{
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < words;
+	     i += (u64)gridDim.x * TPB)
+		g[i] = GUARD_WORD;
+}
+
+__global__ __launch_bounds__(TPB) void k_guard_check(const unsigned *__restrict__ g,
+						     u64 words, unsigned *__restrict__ bad)
+{
+	unsigned n = 0;
+	for (u64 i = (u64)blockIdx.x * TPB + threadIdx.x; i < words;
+	     i += (u64)gridDim.x * TPB)
+		n += g[i] != GUARD_WORD;
+	if (n)
+		atomicAdd(bad, n);
+}
+
+int dk_launch_guard_fill(void *g, u64 bytes)
+{
+	hipLaunchKernelGGL(k_guard_fill, dim3(64), dim3(TPB), 0, DK_STREAM,
+			   (unsigned *)g, bytes / 4);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
+int dk_launch_guard_check(const void *g, u64 bytes, unsigned *bad)
+{
+	hipLaunchKernelGGL(k_guard_check, dim3(64), dim3(TPB), 0, DK_STREAM,
+			   (const unsigned *)g, bytes / 4, bad);
+	DK_LAUNCHED();
+	return DK_OK;
+}
+
 int dk_launch_set_norm(dk_state *s, double v)
 {
 	hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, DK_STREAM,

@@ -31,6 +31,7 @@
 // on the device; unmapping and merging wait for it on the host.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <set>
@@ -536,11 +537,77 @@ int device_alloc(u64 bytes, void **out)
 	return b >= CHUNK ? alloc_whole(b, out) : alloc_piece(b, out);
 }
 
+// Guard bands ("guard_alloc"): a pool block is handed out GUARD bytes into
+// a larger one, with GUARD bytes of a known pattern before it and after its
+// size rounded up to 256; its release counts the changed guard words (the
+// stream is synchronised) and reports them with the block's size.
+constexpr u64 GUARD = 64 << 10;
+struct Guarded {
+	char *raw;
+	u64 span;
+};
+std::mutex g_guard_mu;
+std::unordered_map<u64, Guarded> g_guarded;
+unsigned *g_guard_bad = nullptr;
+
+int guarded_alloc(u64 bytes, void **out)
+{
+	const u64 body = (bytes + 255) / 256 * 256;
+	const u64 span = body + 2 * GUARD;
+	if (span >= MID_BLOCK)
+		return device_alloc(bytes, out);
+	char *raw = nullptr;
+	int rc = device_alloc(span, (void **)&raw);
+	if (rc)
+		return rc;
+	(void)dk_launch_guard_fill(raw, GUARD);
+	(void)dk_launch_guard_fill(raw + GUARD + body, GUARD);
+	*out = raw + GUARD;
+	std::lock_guard<std::mutex> lk(g_guard_mu);
+	g_guarded[(u64)*out] = Guarded{raw, span};
+	return DK_OK;
+}
+
+// True when p was a guarded block (then released here).
+bool guarded_free(void *p, u64 bytes)
+{
+	Guarded g;
+	{
+		std::lock_guard<std::mutex> lk(g_guard_mu);
+		auto it = g_guarded.find((u64)p);
+		if (it == g_guarded.end())
+			return false;
+		g = it->second;
+		g_guarded.erase(it);
+		if (!g_guard_bad && hipMalloc((void **)&g_guard_bad, 2 * sizeof(unsigned)) != hipSuccess)
+			g_guard_bad = nullptr;
+	}
+	if (g_guard_bad) {
+		unsigned h[2] = {0, 0};
+		unsigned *bad = nullptr;
+		if (hipMallocAsync((void **)&bad, 2 * sizeof(unsigned), DK_STREAM) == hipSuccess) {
+			(void)hipMemcpyAsync(bad, h, sizeof(h), hipMemcpyHostToDevice, DK_STREAM);
+			(void)dk_launch_guard_check(g.raw, GUARD, bad);
+			(void)dk_launch_guard_check(g.raw + g.span - GUARD, GUARD, bad + 1);
+			(void)hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, DK_STREAM);
+			(void)hipStreamSynchronize(DK_STREAM);
+			(void)hipFreeAsync(bad, DK_STREAM);
+			if (h[0] || h[1])
+				fprintf(stderr, "[guard] block of %llu bytes at %p: %u words "
+					"before and %u after it changed\n",
+					(unsigned long long)bytes, p, h[0], h[1]);
+		}
+	}
+	(void)hipFreeAsync(g.raw, DK_STREAM);
+	return true;
+}
+
 } // namespace
 
 int dk_device_alloc(u64 bytes, void **out)
 {
-	const int rc = device_alloc(bytes, out);
+	const int rc = g_ctx.guard_alloc && bytes < MID_BLOCK ? guarded_alloc(bytes, out)
+							     : device_alloc(bytes, out);
 	if (!rc && g_ctx.poison_alloc)
 		return dk_launch_poison(*out, bytes);
 	return rc;
@@ -551,7 +618,8 @@ void dk_device_free(void *p, u64 bytes)
 	if (!p || !g_ctx.ready || dk_closed())
 		return;
 	if (bytes < MID_BLOCK) {
-		(void)hipFreeAsync(p, DK_STREAM);
+		if (!guarded_free(p, bytes))
+			(void)hipFreeAsync(p, DK_STREAM);
 		return;
 	}
 	const u64 b = dk_block_bytes(bytes);

@@ -145,6 +145,13 @@ extern "C" int dk_init(int device)
 	DK_HIP(hipDeviceGetDefaultMemPool(&pool, device));
 	uint64_t thr = UINT64_MAX;
 	DK_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+	// A block freed on one lane's stream comes back to that stream, or to
+	// one that has waited for an event recorded after the free: no reuse
+	// on the runtime's own view of completion, and no waits it would add
+	// by itself between the lanes' streams.
+	int no = 0;
+	DK_HIP(hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowOpportunistic, &no));
+	DK_HIP(hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no));
 	size_t fr = 0, tot = 0;
 	DK_HIP(hipMemGetInfo(&fr, &tot));
 	g_ctx.budget = (u64)((double)fr * 0.92);
@@ -1324,6 +1331,10 @@ extern "C" int dk_tune(const char *key, int64_t value)
 	if (!strcmp(key, "arena_max_chunks")) {
 		g_ctx.arena_max_chunks = value > 0 ? (u64)value
 						   : g_ctx.arena_default_chunks;
+		return DK_OK;
+	}
+	if (!strcmp(key, "guard_alloc")) {
+		g_ctx.guard_alloc = value != 0;
 		return DK_OK;
 	}
 	if (!strcmp(key, "poison_alloc")) {

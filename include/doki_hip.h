@@ -545,7 +545,11 @@ int dk_time_ring_read(float *ms, size_t capacity, size_t *count);
  * are reused), "poison_alloc" (1: every device block the library hands out,
  * states, scratch and gate matrices, is first filled with all-ones bytes, a
  * NaN in every amplitude, so that a read of memory nothing wrote shows in
- * the results; a testing aid, 0 by default). */
+ * the results; a testing aid, 0 by default), "guard_alloc" (1: every
+ * block below 64 MiB gets 64 KiB guard bands of a known pattern before and
+ * after it; its release synchronises the lane's stream, counts changed
+ * guard words and reports them on stderr with the block's size; a testing
+ * aid, 0 by default). */
 int dk_tune(const char *key, int64_t value);
 /* Compute units and architecture name of the library's device. */
 int dk_device_info(int *num_cus, char *arch, size_t arch_len);
